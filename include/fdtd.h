@@ -1,0 +1,140 @@
+/*
+ * fdtd.h -- C ABI of the MI355X (gfx950) 3D Yee FDTD library, libfdtd_hip.so.
+ *
+ * One entry point per op of the reference MATLAB MEX gateway (src/simFDTD.cu:19-187), with the
+ * op's argument meaning and order kept, plus the extensions the hot path needs (fused step/run,
+ * runtime CPML parameters, oblique and layered plane waves, uint8 material index, probes,
+ * snapshots).  Plain pointers and sizes only; all arrays are host arrays in the reference's
+ * x-fastest layout (index i + j*I + k*I*J, as MATLAB's column-major x(:) of an I x J x K array).
+ *
+ * Every function returns 0 on success or an FDTD_ERR_* code; fdtd_last_error() gives a
+ * thread-local message and fdtd_last_error_id() the MEX error identifier the reference would
+ * raise ("simFDTD:InvalidInput", "simFDTD:AlreadyInitialized", ...).  GPU work is queued on the
+ * simulation's own HIP stream and is asynchronous except where a function returns host data.
+ */
+#ifndef MAXWELL_FDTD_H
+#define MAXWELL_FDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdtd_sim fdtd_sim; /* one simulation; reference: class simulation, src/simulation.h:24 */
+
+enum {
+  FDTD_OK = 0,
+  FDTD_ERR_INVALID = 1,     /* bad argument                        -> simFDTD:InvalidInput */
+  FDTD_ERR_STATE = 2,       /* op called in the wrong order        -> simFDTD:InvalidInput */
+  FDTD_ERR_ALREADY = 3,     /* repeated one-time op                -> simFDTD:AlreadyInitialized */
+  FDTD_ERR_DEVICE = 4,      /* HIP runtime failure                 -> simFDTD:DeviceError */
+  FDTD_ERR_NOMEM = 5,       /* allocation failed / over budget     -> simFDTD:OutOfMemory */
+  FDTD_ERR_UNKNOWN_OP = 6   /* unknown op string (gateway level)   -> simFDTD:UnknownOperation */
+};
+
+/* Field / probe component ids: 1..3 = Ex, Ey, Ez; 4..6 = Hx, Hy, Hz. */
+
+const char *fdtd_last_error(void);
+const char *fdtd_last_error_id(void);
+const char *fdtd_version(void);
+
+/* ---------------- ops of the reference MEX gateway ---------------- */
+
+/* 'CreateSimulation': src/simFDTD.cu:30-38, constructor src/simulation.h:70-77 */
+int fdtd_create(fdtd_sim **out);
+/* 'DeleteSimulation': src/simFDTD.cu:49-52, destructor src/simulation.h:1153-1269 */
+int fdtd_destroy(fdtd_sim *s);
+/* 'SetConst': freqRes = [fctr, dx, dt], timeDim = [T, I, J, K, nAnt]; src/simulation.h:79-131 */
+int fdtd_set_const(fdtd_sim *s, const float freqRes[3], const int32_t timeDim[5]);
+/* 'SetSource' (update = 0) / update mode with a 4th MEX argument (update = 1);
+ * source has T values; src/simFDTD.cu:111-125, src/simulation.h:1068-1095 */
+int fdtd_set_source(fdtd_sim *s, const float *source, int32_t update);
+/* 'SetAntennaLocations': locations int32[nAnt x 3] column-major (x, y, z offsets from the CPML);
+ * src/simulation.h:1047-1067 */
+int fdtd_set_antenna_locations(fdtd_sim *s, const int32_t *locations);
+/* 'UpdateMaterial': flag 1..3 = sigma x,y,z; 4..6 = eps_inf x,y,z; values has I*J*K entries;
+ * src/simulation.h:288-328 */
+int fdtd_update_material(fdtd_sim *s, int32_t flag, const float *values);
+/* 'ReadySimulation': CPML profiles, coefficients, device state; src/simulation.h:640-1045 */
+int fdtd_ready(fdtd_sim *s);
+/* 'Init1D': incident-line histories for all T steps; src/simulation.h:134-254 */
+int fdtd_init_1d(fdtd_sim *s, float rate, const float *pw_source);
+/* 'UpdateCuda': recompute the E coefficients from the material arrays; src/simulation.h:1097-1127 */
+int fdtd_update_coeffs(fdtd_sim *s);
+/* 'ResetFieldAll': zero the six fields and twelve CPML psi arrays; src/simulation.h:1128-1151 */
+int fdtd_reset_fields(fdtd_sim *s);
+/* 'UpdateH_All' src/simulation.h:371-385 (gpu.h:575-602); 'UpdateE_All' :386-391 (gpu.h:727-763) */
+int fdtd_update_h_all(fdtd_sim *s);
+int fdtd_update_e_all(fdtd_sim *s);
+/* 'UpdateH_PML' src/simulation.h:392-403 (gpu.h:604-725); 'UpdateE_PML' :404-418 (gpu.h:765-868).
+ * UpdateE_PML also closes a step for the probe recorder (row = steps since ResetFieldAll). */
+int fdtd_update_h_pml(fdtd_sim *s);
+int fdtd_update_e_pml(fdtd_sim *s);
+/* 'UpdateH_PlaneWave' / 'UpdateE_PlaneWave': time = [n, mode]; modes 1..4 as the reference
+ * (1 ZY, 2 ZX, 3 XZ, 4 XY), 5 = oblique; src/simulation.h:330-369, :421-570 */
+int fdtd_update_h_planewave(fdtd_sim *s, const int32_t time[2]);
+int fdtd_update_e_planewave(fdtd_sim *s, const int32_t time[2]);
+/* 'AddCurrents': time = [n, pol], box = [i0,i1,j0,j1,k0,k1]; src/simulation.h:256-286 (gpu.h:55-82) */
+int fdtd_add_currents(fdtd_sim *s, const int32_t time[2], const int32_t box[6]);
+/* 'OutputAnt': out has `size` floats, filled with the probe record [ncomp][T][nAnt];
+ * src/simulation.h:621-638 */
+int fdtd_output_ant(fdtd_sim *s, float *out, uint32_t size);
+/* 'OutputVis': flag 1..3 = Ex,Ey,Ez, 4..6 = Hx,Hy,Hz, 0/-1/-2 = sigma x/y/z; out has `size` >= I*J*K
+ * floats; src/simulation.h:572-619 */
+int fdtd_output_vis(fdtd_sim *s, float *out, uint32_t size, int32_t flag);
+
+/* ---------------- extensions (no counterpart in the reference) ---------------- */
+
+/* CPML parameters [P, poly_m, kappa_max, alpha_max, alpha_m, sigma_factor]; between SetConst and
+ * ReadySimulation.  sigma_max = sigma_factor*(m+1)/(eta0*dx) as src/simulation.h:87.  The
+ * reference values are [10, 3, 1, 0.08, 1, 0.8]; kappa is applied per axis. */
+int fdtd_set_cpml(fdtd_sim *s, const double params[6]);
+/* TF/SF box inset from the grid faces (reference 15, src/simulation.h:102); before ReadySimulation */
+int fdtd_set_tfsf(fdtd_sim *s, int32_t inset);
+/* Plane-wave set-up used by Init1D.  mode 1..4: params[0] != 0 makes the 1D line carry the
+ * grid's centre-column material profile (layered media).  mode 5: oblique incidence,
+ * params = [theta, phi, psi] (radians), dispersion-matched 1D line. */
+int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);
+/* Per-cell uint8 material index into a table of n rows [sigma_x, sigma_y, sigma_z, eps_x, eps_y,
+ * eps_z] (n <= 256); replaces the six UpdateMaterial arrays. */
+int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *table, int32_t n);
+/* Probe components (1..6), recorded after every step; default {3} (Ez). */
+int fdtd_set_probe_components(fdtd_sim *s, const int32_t *comps, int32_t n);
+/* One full time step n (H update, plane wave, CPML, E update, plane wave, CPML, probes) as one
+ * fused kernel; mode 0 = no plane wave. */
+int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode);
+/* Steps n0 .. n0+count-1 (fused).  If snap_every > 0, component snap_comp (1..6) is copied to a
+ * device snapshot buffer after every snap_every-th step (counted since ResetFieldAll). */
+int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_every, int32_t snap_comp);
+/* Number of snapshots held and their copy to host: out has count*I*J*K floats. */
+int fdtd_snapshot_count(fdtd_sim *s, int32_t *count);
+int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size);
+/* Wait for all queued work of this simulation. */
+int fdtd_synchronize(fdtd_sim *s);
+/* Read any internal array to host for verification: what = 0..5 fields, 6..11 Ce/Ch per cell,
+ * 20..31 psi (reference order Exy,Exz,Eyx,Eyz,Ezx,Ezy,Hxy,Hxz,Hyx,Hyz,Hzx,Hzy), 40/41 E/H line
+ * history, 42/43 inverse line history.  *count receives the element count when out is NULL. */
+int fdtd_read_array(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
+/* Host-only part of ReadySimulation (CPML profiles, 1/kappa, material table or per-cell
+ * coefficients, oblique plane-wave tables); makes no device call, so it can run without a GPU.
+ * ReadySimulation calls it itself. */
+int fdtd_prepare(fdtd_sim *s);
+/* Read a host set-up array (after fdtd_prepare): 0..3 bE,cE,bH,cH [2P]; 4,5 kappa E,H [P];
+ * 6..8 1/kappa E along x,y,z; 9..11 1/kappa H; 12 scalars [Chh, Che, sigma_max, N1, i0, i1, j0,
+ * j1, k0, k1, P, dx/ds, source node]; 13..15 Ce x,y,z per cell; 16..18 Ch x,y,z per cell;
+ * 19 material id per cell; 20 + 4*axis + {0,1,2,3} oblique tables (E integer, E half, H integer,
+ * H half); 32 polarisation [eE x,y,z, eH x,y,z].  *count receives the element count. */
+int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
+/* Launch tuning for the fused kernel: rows per tile (4 or 8) and z-chunk depth (0 = automatic). */
+int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk);
+/* Average device time of the fused step kernel over the last fdtd_run with timing on
+ * (HIP events around each launch on the simulation's stream). */
+int fdtd_set_timing(fdtd_sim *s, int32_t on);
+int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAXWELL_FDTD_H */

@@ -1,0 +1,66 @@
+// fdtd_kernels.hpp -- device-side argument blocks and host launchers of the FDTD kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "fdtd_internal.hpp"
+
+namespace fdtd {
+
+struct GridDev {
+  int I, J, K, P;
+  int i0, i1, j0, j1, k0, k1;  // TF/SF box
+};
+
+struct CoefDev {
+  const uint8_t *mat = nullptr;  // index mode: per-cell material id
+  const float *tab = nullptr;    // index mode: [ntab][6] = Ce x,y,z, Ch x,y,z (Ch with 1/dx)
+  int ntab = 0;
+  const float *Ce[3] = {nullptr, nullptr, nullptr};  // full mode: per-cell arrays
+  const float *Ch[3] = {nullptr, nullptr, nullptr};
+  const float *ikE[3] = {nullptr, nullptr, nullptr}, *ikH[3] = {nullptr, nullptr, nullptr};
+  const float *bE = nullptr, *cE = nullptr, *bH = nullptr, *cH = nullptr;
+  float Chh = 1, Che = 0, dx = 0;
+};
+
+struct FieldSet {
+  float *f[6];
+};
+struct PsiSet {
+  float *p[12];
+};
+
+struct LineDev {
+  int N, T, P, src;
+  float Ch, ds;
+  const float *Ce, *Cb, *ikE, *ikH;  // per node
+  const float *bE, *cE, *bH, *cH;    // [2P]
+  const float *pw;                   // [T] source samples (already scaled by rate)
+  float *Eh, *Hh;                    // [T][N] histories
+};
+
+// Per-op kernels (state updated in place), one launcher per reference op.
+hipError_t launch_update_h_all(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co);
+hipError_t launch_update_e_all(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co);
+hipError_t launch_update_h_pml(hipStream_t st, const GridDev &g, const FieldSet &F, const PsiSet &ps, const CoefDev &co);
+hipError_t launch_update_e_pml(hipStream_t st, const GridDev &g, const FieldSet &F, const PsiSet &ps, const CoefDev &co);
+hipError_t launch_pw_h(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, const PWDev &pw, int n);
+hipError_t launch_pw_e(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, const PWDev &pw, int n);
+hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, int comp,
+                               const int box[6], float s, float dx);
+hipError_t launch_line(hipStream_t st, const LineDev &L);
+hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int nAnt, const int *comps, int ncomp,
+                         float *port, int row, int T);
+
+// Fused step: reads (cur, pcur), writes (nxt, pnxt).  Returns hipErrorInvalidValue when the grid
+// is wider than the fused kernel supports (I > 512).
+struct StepTuning {
+  int tile_rows = 8;  // 4 or 8
+  int z_chunk = 0;    // 0: automatic
+};
+hipError_t launch_step_fused(hipStream_t st, const GridDev &g, const FieldSet &cur, const FieldSet &nxt,
+                             const PsiSet &pcur, const PsiSet &pnxt, const CoefDev &co, const PWDev &pw, int n,
+                             const StepTuning &tune);
+bool fused_supported(const GridDev &g);
+
+}  // namespace fdtd
