@@ -1,0 +1,948 @@
+// fdtd_api.cpp -- simulation state and C ABI of libfdtd_hip.so (declared in include/fdtd.h).
+//
+// One fdtd_sim per simulation, as the reference's `class simulation` (src/simulation.h:24).  Host
+// set-up (CPML grading, coefficients, plane-wave tables) runs in fdtd_prepare without touching the
+// device; ReadySimulation then allocates and uploads.  The fields and CPML psi arrays live in HBM
+// for the life of the simulation; the fused step ping-pongs between two copies of them.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fdtd.h"
+#include "fdtd_internal.hpp"
+#include "fdtd_kernels.hpp"
+
+using namespace fdtd;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local std::string g_err_id;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  switch (code) {
+    case FDTD_ERR_ALREADY: g_err_id = "simFDTD:AlreadyInitialized"; break;
+    case FDTD_ERR_DEVICE: g_err_id = "simFDTD:DeviceError"; break;
+    case FDTD_ERR_NOMEM: g_err_id = "simFDTD:OutOfMemory"; break;
+    case FDTD_ERR_UNKNOWN_OP: g_err_id = "simFDTD:UnknownOperation"; break;
+    default: g_err_id = "simFDTD:InvalidInput"; break;
+  }
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(FDTD_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct fdtd_sim {
+  // ---- flags (src/simulation.h:1305-1311) ----
+  bool init_const = false, init_source = false, init_antenna = false, init_ready = false;
+  bool prepared = false, have_1d = false;
+
+  // ---- SetConst ----
+  float fctr = 0, delX = 0, delT = 0;
+  int T = 0, I = 0, J = 0, K = 0, nAnt = 0;
+  size_t IJ = 0, IJK = 0;
+  CpmlParams cp;
+  int width = 15;
+
+  // ---- materials ----
+  bool have_arrays = false;  // six host arrays (UpdateMaterial layout)
+  std::vector<float> eps[3], sig[3];
+  bool have_index = false;   // SetMaterialIndex form
+  std::vector<uint8_t> midx;
+  std::vector<float> mtab;   // [n][6] sigma x,y,z, eps x,y,z
+  int nmat = 0;
+
+  // ---- prepared host set-up ----
+  CpmlProfiles prof;
+  std::vector<float> ikE[3], ikH[3];
+  float Chh = 1, Che = 0, den = 0;
+  GridDev g{};
+  bool index_mode = true;
+  std::vector<uint8_t> cell_mat;   // index mode: material id per cell
+  std::vector<float> coef_tab;     // index mode: [n][6] Ce x,y,z, Ch x,y,z
+  std::vector<float> cell_Ce[3], cell_Ch[3];  // full mode
+  int pw_mode = 0, pw_layered = 0;
+  double pw_prm[3] = {0, 0, 0};
+  ObliqueSetup obl;
+  int N1 = 0, line_src = 0;
+
+  // ---- source / probes ----
+  std::vector<float> source;
+  std::vector<int32_t> ant_loc;
+  std::vector<int> coax;
+  std::vector<int> probe_comps = {EZ};
+  int rec_step = 0;
+
+  // ---- device ----
+  hipStream_t stream = nullptr;
+  float *dF[2][6] = {};
+  float *dPsi[2][12] = {};
+  size_t psi_len[12] = {};
+  int cur = 0;
+  bool second = false;
+  uint8_t *dMat = nullptr;
+  float *dTab = nullptr;
+  float *dCe[3] = {}, *dCh[3] = {};
+  float *dIkE[3] = {}, *dIkH[3] = {};
+  float *dbE = nullptr, *dcE = nullptr, *dbH = nullptr, *dcH = nullptr;
+  float *dE1h = nullptr, *dH1h = nullptr, *dEinv = nullptr, *dHinv = nullptr;
+  float *dZero = nullptr, *dTabX = nullptr, *dTabZ = nullptr;
+  float *dObl[3][2][2] = {};  // [axis][E/H][int/half]
+  int *dProbeIdx = nullptr, *dProbeComps = nullptr;
+  float *dPort = nullptr;
+  size_t port_len = 0;
+  std::vector<float *> dSnaps;
+  int nsnap = 0;
+  PWDev pwdev[6];
+  StepTuning tune;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double kernel_ms = 0;
+  int64_t kernel_launches = 0;
+  std::vector<void *> allocs;
+
+  ~fdtd_sim() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void *p : allocs) hipFree(p);
+    for (float *p : dSnaps) hipFree(p);
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  template <typename Tp>
+  int dalloc(Tp **p, size_t n) {
+    void *q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(Tp));
+    if (e != hipSuccess) return fail(FDTD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (Tp *)q;
+    return 0;
+  }
+  template <typename Tp>
+  int upload(Tp **p, const std::vector<Tp> &v) {
+    if (int rc = dalloc(p, v.size())) return rc;
+    HIPCHK(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(Tp), hipMemcpyHostToDevice, stream));
+    return 0;
+  }
+
+  FieldSet fields(int which) const {
+    FieldSet F;
+    for (int c = 0; c < 6; ++c) F.f[c] = dF[which][c];
+    return F;
+  }
+  PsiSet psis(int which) const {
+    PsiSet P;
+    for (int q = 0; q < 12; ++q) P.p[q] = dPsi[which][q];
+    return P;
+  }
+  CoefDev coefs() const {
+    CoefDev co;
+    co.mat = index_mode ? dMat : nullptr;
+    co.tab = dTab;
+    co.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
+    for (int c = 0; c < 3; ++c) {
+      co.Ce[c] = dCe[c];
+      co.Ch[c] = dCh[c];
+      co.ikE[c] = dIkE[c];
+      co.ikH[c] = dIkH[c];
+    }
+    co.bE = dbE; co.cE = dcE; co.bH = dbH; co.cH = dcH;
+    co.Chh = Chh; co.Che = Che; co.dx = delX;
+    return co;
+  }
+
+  void ensure_arrays() {
+    if (have_arrays) return;
+    for (int c = 0; c < 3; ++c) {
+      eps[c].assign(IJK, 1.0f);
+      sig[c].assign(IJK, 0.0f);
+    }
+    if (have_index) {  // expand the index form into the six arrays
+      for (size_t id = 0; id < IJK; ++id) {
+        const float *r = &mtab[6 * midx[id]];
+        for (int c = 0; c < 3; ++c) { sig[c][id] = r[c]; eps[c][id] = r[3 + c]; }
+      }
+      have_index = false;
+    }
+    have_arrays = true;
+  }
+
+  // Material classification: the uint8 table form when the grid has at most 256 distinct
+  // (sigma, eps) tuples, per-cell coefficient arrays otherwise.
+  void build_coefficients() {
+    std::vector<float> rows;  // [n][6] sigma, eps
+    cell_mat.clear();
+    index_mode = true;
+    if (have_index) {
+      rows = mtab;
+      cell_mat = midx;
+    } else if (have_arrays) {
+      struct Key {
+        float v[6];
+        bool operator==(const Key &o) const { return std::memcmp(v, o.v, sizeof(v)) == 0; }
+      };
+      struct KH {
+        size_t operator()(const Key &k) const {
+          uint32_t u[6];
+          std::memcpy(u, k.v, sizeof(u));
+          size_t h = 1469598103934665603ull;
+          for (int q = 0; q < 6; ++q) h = (h ^ u[q]) * 1099511628211ull;
+          return h;
+        }
+      };
+      std::unordered_map<Key, int, KH> ids;
+      cell_mat.assign(IJK, 0);
+      Key last{};
+      int last_id = -1;
+      for (size_t id = 0; id < IJK && index_mode; ++id) {
+        Key kk;
+        for (int c = 0; c < 3; ++c) { kk.v[c] = sig[c][id]; kk.v[3 + c] = eps[c][id]; }
+        if (last_id >= 0 && kk == last) { cell_mat[id] = (uint8_t)last_id; continue; }
+        auto it = ids.find(kk);
+        int m;
+        if (it == ids.end()) {
+          m = (int)ids.size();
+          if (m >= 256) { index_mode = false; break; }
+          ids.emplace(kk, m);
+          rows.insert(rows.end(), kk.v, kk.v + 6);
+        } else {
+          m = it->second;
+        }
+        cell_mat[id] = (uint8_t)m;
+        last = kk;
+        last_id = m;
+      }
+    } else {
+      rows = {0, 0, 0, 1, 1, 1};
+      cell_mat.assign(IJK, 0);
+    }
+    coef_tab.clear();
+    for (int c = 0; c < 3; ++c) { cell_Ce[c].clear(); cell_Ch[c].clear(); }
+    if (index_mode) {
+      const size_t n = rows.size() / 6;
+      coef_tab.assign(n * 6, 0.0f);
+      for (size_t m = 0; m < n; ++m)
+        for (int c = 0; c < 3; ++c)
+          material_coeffs(rows[6 * m + 3 + c], rows[6 * m + c], delT, den, &coef_tab[6 * m + c], &coef_tab[6 * m + 3 + c]);
+    } else {
+      cell_mat.clear();
+      for (int c = 0; c < 3; ++c) {
+        cell_Ce[c].assign(IJK, 0.0f);
+        cell_Ch[c].assign(IJK, 0.0f);
+      }
+      for (size_t id = 0; id < IJK; ++id) {
+        const float d = ((int)(id % (size_t)I) == I - 1) ? 0.0f : den;
+        for (int c = 0; c < 3; ++c) material_coeffs(eps[c][id], sig[c][id], delT, d, &cell_Ce[c][id], &cell_Ch[c][id]);
+      }
+    }
+  }
+
+  int prepare() {
+    if (!init_const) return fail(FDTD_ERR_STATE, "ReadySimulation before SetConst");
+    const int P = cp.P;
+    if (I < 2 * P + 2 || J < 2 * P + 2 || K < 2 * P + 2) return fail(FDTD_ERR_INVALID, "grid smaller than 2P+2 cells along an axis");
+    if (width < 1 || 2 * width + 3 > I || 2 * width + 3 > J || 2 * width + 3 > K)
+      return fail(FDTD_ERR_INVALID, "TF/SF inset too large for the grid");
+    g.I = I; g.J = J; g.K = K; g.P = P;
+    g.i0 = width; g.i1 = I - width - 1;
+    g.j0 = width; g.j1 = J - width - 1;
+    g.k0 = width; g.k1 = K - width - 1;
+    prof = compute_cpml(cp, delX, delT);
+    ikE[0] = inv_kappa_E(prof, P, I); ikE[1] = inv_kappa_E(prof, P, J); ikE[2] = inv_kappa_E(prof, P, K);
+    ikH[0] = inv_kappa_H(prof, P, I); ikH[1] = inv_kappa_H(prof, P, J); ikH[2] = inv_kappa_H(prof, P, K);
+    // src/simulation.h:914-922 with sigma_m = 0; den_ex = 1/dx (kappa is in the 1/kappa tables)
+    const float sigma_m = 0;
+    Chh = (float)((1 - sigma_m * delT / 2 / kMu0) / (1 + sigma_m * delT / 2 / kMu0));
+    Che = (float)(delT / kMu0 / (1 + sigma_m * delT / 2 / kMu0) / delX);
+    den = (float)(1.0 / (delX * 1.0f));
+    build_coefficients();
+    if (pw_mode == 5) {
+      const int lo[3] = {g.i0, g.j0, g.k0}, hi[3] = {g.i1, g.j1, g.k1}, Q[3] = {I, J, K};
+      obl = oblique_setup(pw_prm[0], pw_prm[1], pw_prm[2], lo, hi, Q, P, delX, delT, fctr);
+    }
+    prepared = true;
+    return 0;
+  }
+
+  int resolve_antenna() {
+    coax.assign(nAnt, 0);
+    for (int a = 0; a < nAnt; ++a) {
+      const long x = cp.P + (long)ant_loc[a], y = cp.P + (long)ant_loc[a + nAnt], z = cp.P + (long)ant_loc[a + 2 * nAnt];
+      if (x < 0 || y < 0 || z < 0 || x >= I || y >= J || z >= K) {
+        init_antenna = false;
+        return fail(FDTD_ERR_INVALID, "antenna location outside the grid");
+      }
+      coax[a] = (int)(x + y * (long)I + z * (long)IJ);
+    }
+    return 0;
+  }
+
+  int alloc_probes() {
+    if (!init_antenna || !init_ready) return 0;
+    if (int rc = resolve_antenna()) return rc;
+    port_len = probe_comps.size() * (size_t)T * nAnt;
+    if (int rc = dalloc(&dPort, port_len)) return rc;
+    HIPCHK(hipMemsetAsync(dPort, 0, port_len * sizeof(float), stream));
+    if (int rc = upload(&dProbeIdx, coax)) return rc;
+    if (int rc = upload(&dProbeComps, probe_comps)) return rc;
+    return 0;
+  }
+
+  int upload_coefficients() {
+    if (index_mode) {
+      if (int rc = upload(&dMat, cell_mat)) return rc;
+      if (int rc = upload(&dTab, coef_tab)) return rc;
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        if (int rc = upload(&dCe[c], cell_Ce[c])) return rc;
+        if (int rc = upload(&dCh[c], cell_Ch[c])) return rc;
+      }
+    }
+    return 0;
+  }
+
+  int ready() {
+    if (init_ready) return fail(FDTD_ERR_ALREADY, "ReadySimulation: already called.");
+    if (int rc = prepare()) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FDTD_ERR_DEVICE, "no HIP device");
+    HIPCHK(hipStreamCreate(&stream));
+    for (int c = 0; c < 6; ++c)
+      if (int rc = dalloc(&dF[0][c], IJK)) return rc;
+    const size_t P = (size_t)cp.P;
+    const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
+    const size_t sizes[12] = {sy, sz, sx, sz, sx, sy, sy, sz, sx, sz, sx, sy};
+    for (int q = 0; q < 12; ++q) {
+      psi_len[q] = sizes[q];
+      if (int rc = dalloc(&dPsi[0][q], sizes[q])) return rc;
+    }
+    if (int rc = upload_coefficients()) return rc;
+    for (int c = 0; c < 3; ++c) {
+      if (int rc = upload(&dIkE[c], ikE[c])) return rc;
+      if (int rc = upload(&dIkH[c], ikH[c])) return rc;
+    }
+    if (int rc = upload(&dbE, prof.bE)) return rc;
+    if (int rc = upload(&dcE, prof.cE)) return rc;
+    if (int rc = upload(&dbH, prof.bH)) return rc;
+    if (int rc = upload(&dcH, prof.cH)) return rc;
+    init_ready = true;
+    if (int rc = alloc_probes()) return rc;
+    return zero_state();
+  }
+
+  int zero_state() {
+    for (int c = 0; c < 6; ++c) HIPCHK(hipMemsetAsync(dF[cur][c], 0, IJK * sizeof(float), stream));
+    for (int q = 0; q < 12; ++q) HIPCHK(hipMemsetAsync(dPsi[cur][q], 0, psi_len[q] * sizeof(float), stream));
+    if (dPort) HIPCHK(hipMemsetAsync(dPort, 0, port_len * sizeof(float), stream));
+    rec_step = 0;
+    return 0;
+  }
+
+  int ensure_second() {
+    if (second) return 0;
+    for (int c = 0; c < 6; ++c)
+      if (int rc = dalloc(&dF[1][c], IJK)) return rc;
+    for (int q = 0; q < 12; ++q)
+      if (int rc = dalloc(&dPsi[1][q], psi_len[q])) return rc;
+    second = true;
+    return 0;
+  }
+
+  // ---- 1D line and plane-wave parameter blocks ----
+  int run_line(int N, float ds, float Ch1, const std::vector<float> &Ce, const std::vector<float> &Cb, int src,
+               const std::vector<float> &pw, float **dEh, float **dHh) {
+    std::vector<float> ikE1 = inv_kappa_E(prof, cp.P, (size_t)N), ikH1 = inv_kappa_H(prof, cp.P, (size_t)N);
+    float *dCe1, *dCb1, *dikE1, *dikH1, *dpw;
+    if (int rc = upload(&dCe1, Ce)) return rc;
+    if (int rc = upload(&dCb1, Cb)) return rc;
+    if (int rc = upload(&dikE1, ikE1)) return rc;
+    if (int rc = upload(&dikH1, ikH1)) return rc;
+    if (int rc = upload(&dpw, pw)) return rc;
+    if (int rc = dalloc(dEh, (size_t)T * N)) return rc;
+    if (int rc = dalloc(dHh, (size_t)T * N)) return rc;
+    LineDev L;
+    L.N = N; L.T = T; L.P = cp.P; L.src = src; L.Ch = Ch1; L.ds = ds;
+    L.Ce = dCe1; L.Cb = dCb1; L.ikE = dikE1; L.ikH = dikH1;
+    L.bE = dbE; L.cE = dcE; L.bH = dbH; L.cH = dcH; L.pw = dpw; L.Eh = *dEh; L.Hh = *dHh;
+    HIPCHK(launch_line(stream, L));
+    return 0;
+  }
+
+  int init_1d(float rate, const float *pwsrc) {
+    if (!init_ready) return fail(FDTD_ERR_STATE, "Init1D before ReadySimulation");
+    std::vector<float> pw((size_t)T);
+    for (int i = 0; i < T; ++i) pw[i] = pwsrc[i] * rate;  // src/simulation.h:188-192
+    dE1h = dH1h = dEinv = dHinv = nullptr;
+    if (pw_mode == 5) {
+      N1 = obl.N1;
+      line_src = obl.src;
+      if (int rc = run_line(N1, obl.ds, obl.Ch1, std::vector<float>(N1, 1.0f), std::vector<float>(N1, obl.Cb1), obl.src, pw, &dE1h, &dH1h)) return rc;
+      for (int u = 0; u < 3; ++u)
+        for (int t = 0; t < 2; ++t) {
+          if (int rc = upload(&dObl[u][0][t], obl.tabE[u][t])) return rc;
+          if (int rc = upload(&dObl[u][1][t], obl.tabH[u][t])) return rc;
+        }
+    } else if (pw_layered && pw_mode >= 1 && pw_mode <= 4) {
+      // exact restriction of the centre column along the propagation axis
+      const int ax = pw_mode <= 2 ? 2 : 0;
+      const int pol = pw_mode == 1 ? EY : pw_mode == 2 ? EX : pw_mode == 3 ? EZ : EY;
+      const int Q = ax == 2 ? K : I;
+      const size_t ic = I / 2, jc = J / 2, kc = K / 2;
+      std::vector<float> Ce(Q), Cb(Q);
+      for (int q = 0; q < Q; ++q) {
+        const size_t id = ax == 2 ? (ic + jc * I + (size_t)q * IJ) : ((size_t)q + jc * I + kc * IJ);
+        cell_coeff(id, pol, &Ce[q], &Cb[q]);
+      }
+      N1 = Q;
+      if (pw_mode == 4) {
+        line_src = 13;
+        if (int rc = run_line(Q, delX, Che, Ce, Cb, 13, pw, &dEinv, &dHinv)) return rc;
+      } else {
+        line_src = Q - 13;
+        if (int rc = run_line(Q, delX, Che, Ce, Cb, Q - 13, pw, &dE1h, &dH1h)) return rc;
+      }
+    } else {
+      // src/simulation.h:134-254: vacuum line of max(I,J,K) nodes, forward and inverse passes
+      N1 = std::max(std::max(I, J), K);
+      const float Ceh_1D = (float)(delT / kEps0 / delX);
+      std::vector<float> Ce(N1, 1.0f), Cb(N1, Ceh_1D);
+      line_src = N1 - 13;
+      if (int rc = run_line(N1, delX, Che, Ce, Cb, N1 - 13, pw, &dE1h, &dH1h)) return rc;
+      if (int rc = run_line(N1, delX, Che, Ce, Cb, 13, pw, &dEinv, &dHinv)) return rc;
+    }
+    if (int rc = build_pw()) return rc;
+    have_1d = true;
+    return 0;
+  }
+
+  void cell_coeff(size_t id, int comp, float *ce, float *ch) const {
+    if (index_mode) {
+      const float *r = &coef_tab[6 * cell_mat[id]];
+      *ce = r[comp];
+      *ch = ((int)(id % (size_t)I) == I - 1) ? 0.0f : r[3 + comp];
+    } else {
+      *ce = cell_Ce[comp][id];
+      *ch = cell_Ch[comp][id];
+    }
+  }
+
+  int build_pw() {
+    const int Qm = std::max(std::max(I, J), K);
+    std::vector<float> zero((size_t)Qm, 0.0f), tx((size_t)I), tz((size_t)K);
+    for (int q = 0; q < I; ++q) tx[q] = (float)(q + N1 - I);
+    for (int q = 0; q < K; ++q) tz[q] = (float)(q + N1 - K);
+    if (!dZero && upload(&dZero, zero)) return FDTD_ERR_NOMEM;
+    if (upload(&dTabX, tx) || upload(&dTabZ, tz)) return FDTD_ERR_NOMEM;
+    for (int m = 0; m < 6; ++m) pwdev[m] = PWDev();
+    for (int m = 1; m <= 4; ++m) {
+      PWDev &p = pwdev[m];
+      const bool zax = m <= 2;
+      const float *ax = zax ? dTabZ : dTabX;
+      for (int u = 0; u < 3; ++u)
+        for (int t = 0; t < 2; ++t) {
+          const bool on = (zax && u == 2) || (!zax && u == 0);
+          p.tabE[u][t] = on ? ax : dZero;
+          p.tabH[u][t] = on ? ax : dZero;
+        }
+      p.N1 = N1;
+      p.Eh = m == 4 ? dEinv : dE1h;
+      p.Hh = m == 4 ? dHinv : dH1h;
+      if (!p.Eh || !p.Hh) continue;  // line of this mode not computed
+      if (m == 1) { p.polE[1] = 1; p.polH[0] = 1; p.active_h = (1 << 6) | (1 << 2); p.active_e = (1 << 7) | (1 << 5); }
+      if (m == 2) { p.polE[0] = 1; p.polH[1] = -1; p.active_h = (1 << 7) | (1 << 5); p.active_e = (1 << 6) | (1 << 2); }
+      if (m == 3) { p.polE[2] = 1; p.polH[1] = 1; p.active_h = (1 << 3) | (1 << 1); p.active_e = (1 << 6) | (1 << 2); }
+      if (m == 4) { p.polE[1] = 1; p.polH[2] = -1; p.active_h = (1 << 6) | (1 << 2); p.active_e = (1 << 3) | (1 << 1); }
+    }
+    if (pw_mode == 5) {
+      PWDev &p = pwdev[5];
+      for (int u = 0; u < 3; ++u)
+        for (int t = 0; t < 2; ++t) {
+          p.tabE[u][t] = dObl[u][0][t];
+          p.tabH[u][t] = dObl[u][1][t];
+        }
+      for (int c = 0; c < 3; ++c) { p.polE[c] = obl.polE[c]; p.polH[c] = obl.polH[c]; }
+      p.N1 = N1; p.Eh = dE1h; p.Hh = dH1h;
+      const int all = (1 << 1) | (1 << 2) | (1 << 3) | (1 << 5) | (1 << 6) | (1 << 7);
+      p.active_h = all;
+      p.active_e = all;
+    }
+    return 0;
+  }
+
+  int check_pw(int n, int mode, const PWDev **out) {
+    *out = nullptr;
+    if (mode < 1 || mode > 5) return 0;  // the reference ignores other modes
+    if (!have_1d) return fail(FDTD_ERR_STATE, "plane wave before Init1D");
+    if (n < 0 || n >= T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
+    if (!pwdev[mode].active_h) return fail(FDTD_ERR_STATE, "the 1D line for this plane-wave mode was not initialised");
+    *out = &pwdev[mode];
+    return 0;
+  }
+
+  int record_probes(int which) {
+    if (dPort && rec_step < T)
+      HIPCHK(launch_probes(stream, fields(which), dProbeIdx, nAnt, dProbeComps, (int)probe_comps.size(), dPort, rec_step, T));
+    rec_step++;
+    return 0;
+  }
+
+  int step(int n, int mode, bool timed, int *ev_slot) {
+    if (!init_ready) return fail(FDTD_ERR_STATE, "Step before ReadySimulation");
+    const PWDev *pw = nullptr;
+    if (int rc = check_pw(n, mode, &pw)) return rc;
+    if (int rc = ensure_second()) return rc;
+    const PWDev none;
+    if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
+    HIPCHK(launch_step_fused(stream, g, fields(cur), fields(1 - cur), psis(cur), psis(1 - cur), coefs(), pw ? *pw : none, n, tune));
+    if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
+    cur = 1 - cur;
+    return record_probes(cur);
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+#define NEED(s)                                                     \
+  do {                                                              \
+    if (!(s)) return fail(FDTD_ERR_INVALID, "null simulation handle"); \
+  } while (0)
+#define NEED_READY(s)                                                          \
+  do {                                                                         \
+    NEED(s);                                                                   \
+    if (!(s)->init_ready) return fail(FDTD_ERR_STATE, "ReadySimulation has not been called"); \
+  } while (0)
+
+extern "C" {
+
+const char *fdtd_last_error(void) { return g_err.c_str(); }
+const char *fdtd_last_error_id(void) { return g_err_id.c_str(); }
+const char *fdtd_version(void) { return "maxwell-mi355x fdtd 0.1 (gfx950)"; }
+
+int fdtd_create(fdtd_sim **out) {
+  if (!out) return fail(FDTD_ERR_INVALID, "null output pointer");
+  *out = new fdtd_sim();
+  return 0;
+}
+
+int fdtd_destroy(fdtd_sim *s) {
+  NEED(s);
+  delete s;
+  return 0;
+}
+
+int fdtd_set_const(fdtd_sim *s, const float freqRes[3], const int32_t timeDim[5]) {
+  NEED(s);
+  if (s->init_const) return fail(FDTD_ERR_ALREADY, "SetConst: already called.");
+  if (!freqRes || !timeDim) return fail(FDTD_ERR_INVALID, "freqRes and TimeDim required.");
+  if (timeDim[0] < 1 || timeDim[1] < 1 || timeDim[2] < 1 || timeDim[3] < 1 || timeDim[4] < 0)
+    return fail(FDTD_ERR_INVALID, "TimeDim must be [T>0, I>0, J>0, K>0, nAnt>=0]");
+  s->fctr = freqRes[0];
+  s->delX = freqRes[1];
+  s->delT = freqRes[2];
+  s->T = timeDim[0];
+  s->I = timeDim[1];
+  s->J = timeDim[2];
+  s->K = timeDim[3];
+  s->nAnt = timeDim[4];
+  s->IJ = (size_t)s->I * s->J;
+  s->IJK = s->IJ * s->K;
+  s->init_const = true;
+  return 0;
+}
+
+int fdtd_set_cpml(fdtd_sim *s, const double p[6]) {
+  NEED(s);
+  if (!s->init_const || s->init_ready) return fail(FDTD_ERR_STATE, "SetCPML: call between SetConst and ReadySimulation");
+  if (!p || p[0] < 2 || p[0] > 64) return fail(FDTD_ERR_INVALID, "SetCPML: 2 <= P <= 64");
+  s->cp.P = (int)p[0];
+  s->cp.poly_m = (float)p[1];
+  s->cp.kappa_max = (float)p[2];
+  s->cp.alpha_max = (float)p[3];
+  s->cp.alpha_m = (float)p[4];
+  s->cp.sig_factor = p[5];
+  return 0;
+}
+
+int fdtd_set_tfsf(fdtd_sim *s, int32_t inset) {
+  NEED(s);
+  if (s->init_ready) return fail(FDTD_ERR_STATE, "SetTFSF after ReadySimulation");
+  if (inset < 1) return fail(FDTD_ERR_INVALID, "TF/SF inset must be >= 1");
+  s->width = inset;
+  return 0;
+}
+
+int fdtd_set_source(fdtd_sim *s, const float *src, int32_t update) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "SetSource before SetConst");
+  if (!src) return fail(FDTD_ERR_INVALID, "SourceIn required.");
+  if (update ? !s->init_source : s->init_source) return 0;  // src/simulation.h:1070-1074, :1085-1089
+  s->source.assign(src, src + s->T);
+  s->init_source = true;
+  return 0;
+}
+
+int fdtd_set_antenna_locations(fdtd_sim *s, const int32_t *loc) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "SetAntennaLocations before SetConst");
+  if (!loc && s->nAnt > 0) return fail(FDTD_ERR_INVALID, "Locations required.");
+  s->ant_loc.assign(loc, loc + 3 * (size_t)s->nAnt);
+  s->init_antenna = s->nAnt > 0;
+  if (s->init_ready) return s->alloc_probes();
+  return 0;
+}
+
+int fdtd_set_probe_components(fdtd_sim *s, const int32_t *comps, int32_t n) {
+  NEED(s);
+  if (!comps || n < 1 || n > 6) return fail(FDTD_ERR_INVALID, "1..6 probe components");
+  std::vector<int> v;
+  for (int q = 0; q < n; ++q) {
+    if (comps[q] < 1 || comps[q] > 6) return fail(FDTD_ERR_INVALID, "probe component must be 1..6");
+    v.push_back(comps[q] - 1);
+  }
+  s->probe_comps = v;
+  if (s->init_ready) return s->alloc_probes();
+  return 0;
+}
+
+int fdtd_update_material(fdtd_sim *s, int32_t flag, const float *values) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "UpdateMaterial before SetConst");
+  if (!values) return fail(FDTD_ERR_INVALID, "flag, values required.");
+  if (flag < 1 || flag > 6) return 0;  // src/simulation.h:288-328 ignores other flags
+  s->ensure_arrays();
+  std::vector<float> &dst = flag <= 3 ? s->sig[flag - 1] : s->eps[flag - 4];
+  std::memcpy(dst.data(), values, s->IJK * sizeof(float));
+  return 0;
+}
+
+int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *table, int32_t n) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "SetMaterialIndex before SetConst");
+  if (!index || !table || n < 1 || n > 256) return fail(FDTD_ERR_INVALID, "SetMaterialIndex: index, table and 1 <= n <= 256 required");
+  for (size_t id = 0; id < s->IJK; ++id)
+    if (index[id] >= n) return fail(FDTD_ERR_INVALID, "material index beyond the table");
+  s->midx.assign(index, index + s->IJK);
+  s->mtab.assign(table, table + 6 * (size_t)n);
+  s->nmat = n;
+  s->have_index = true;
+  s->have_arrays = false;
+  for (int c = 0; c < 3; ++c) { s->eps[c].clear(); s->sig[c].clear(); }
+  return 0;
+}
+
+int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "SetPlaneWave before SetConst");
+  if (mode < 1 || mode > 5 || !params) return fail(FDTD_ERR_INVALID, "plane-wave mode must be 1..5");
+  s->pw_mode = mode;
+  if (mode == 5) {
+    s->pw_prm[0] = params[0]; s->pw_prm[1] = params[1]; s->pw_prm[2] = params[2];
+    s->pw_layered = 0;
+    if (s->prepared) {
+      const int lo[3] = {s->g.i0, s->g.j0, s->g.k0}, hi[3] = {s->g.i1, s->g.j1, s->g.k1}, Q[3] = {s->I, s->J, s->K};
+      s->obl = oblique_setup(params[0], params[1], params[2], lo, hi, Q, s->cp.P, s->delX, s->delT, s->fctr);
+    }
+  } else {
+    s->pw_layered = params[0] != 0.0;
+  }
+  return 0;
+}
+
+int fdtd_prepare(fdtd_sim *s) {
+  NEED(s);
+  return s->prepare();
+}
+
+int fdtd_ready(fdtd_sim *s) {
+  NEED(s);
+  return s->ready();
+}
+
+int fdtd_init_1d(fdtd_sim *s, float rate, const float *pw) {
+  NEED(s);
+  if (!pw) return fail(FDTD_ERR_INVALID, "rate and PW_source required.");
+  return s->init_1d(rate, pw);
+}
+
+int fdtd_update_coeffs(fdtd_sim *s) {
+  NEED_READY(s);
+  s->build_coefficients();
+  return s->upload_coefficients();
+}
+
+int fdtd_reset_fields(fdtd_sim *s) {
+  NEED_READY(s);
+  return s->zero_state();
+}
+
+int fdtd_update_h_all(fdtd_sim *s) {
+  NEED_READY(s);
+  HIPCHK(launch_update_h_all(s->stream, s->g, s->fields(s->cur), s->coefs()));
+  return 0;
+}
+
+int fdtd_update_e_all(fdtd_sim *s) {
+  NEED_READY(s);
+  HIPCHK(launch_update_e_all(s->stream, s->g, s->fields(s->cur), s->coefs()));
+  return 0;
+}
+
+int fdtd_update_h_pml(fdtd_sim *s) {
+  NEED_READY(s);
+  HIPCHK(launch_update_h_pml(s->stream, s->g, s->fields(s->cur), s->psis(s->cur), s->coefs()));
+  return 0;
+}
+
+int fdtd_update_e_pml(fdtd_sim *s) {
+  NEED_READY(s);
+  HIPCHK(launch_update_e_pml(s->stream, s->g, s->fields(s->cur), s->psis(s->cur), s->coefs()));
+  return s->record_probes(s->cur);
+}
+
+int fdtd_update_h_planewave(fdtd_sim *s, const int32_t time[2]) {
+  NEED_READY(s);
+  if (!time) return fail(FDTD_ERR_INVALID, "time required.");
+  const PWDev *pw = nullptr;
+  if (int rc = s->check_pw(time[0], time[1], &pw)) return rc;
+  if (pw) HIPCHK(launch_pw_h(s->stream, s->g, s->fields(s->cur), s->coefs(), *pw, time[0]));
+  return 0;
+}
+
+int fdtd_update_e_planewave(fdtd_sim *s, const int32_t time[2]) {
+  NEED_READY(s);
+  if (!time) return fail(FDTD_ERR_INVALID, "time required.");
+  const PWDev *pw = nullptr;
+  if (int rc = s->check_pw(time[0], time[1], &pw)) return rc;
+  if (pw) HIPCHK(launch_pw_e(s->stream, s->g, s->fields(s->cur), s->coefs(), *pw, time[0]));
+  return 0;
+}
+
+int fdtd_add_currents(fdtd_sim *s, const int32_t time[2], const int32_t box[6]) {
+  NEED_READY(s);
+  if (!time || !box) return fail(FDTD_ERR_INVALID, "time and indices required.");
+  if (!s->init_source) return fail(FDTD_ERR_STATE, "AddCurrents before SetSource");
+  const int n = time[0], pol = time[1];
+  if (n < 0 || n >= s->T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
+  if (pol < 1 || pol > 3) return 0;
+  if (box[1] < box[0] || box[3] < box[2] || box[5] < box[4]) return 0;  // src/gpu.h:60-61
+  const int b[6] = {box[0], box[1], box[2], box[3], box[4], box[5]};
+  HIPCHK(launch_add_currents(s->stream, s->g, s->fields(s->cur), s->coefs(), pol - 1, b, s->source[n], s->delX));
+  return 0;
+}
+
+int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode) {
+  NEED(s);
+  int slot = 0;
+  return s->step(n, mode, false, &slot);
+}
+
+int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_every, int32_t snap_comp) {
+  NEED_READY(s);
+  if (count < 0) return fail(FDTD_ERR_INVALID, "negative step count");
+  if (snap_every > 0 && (snap_comp < 1 || snap_comp > 6)) return fail(FDTD_ERR_INVALID, "snapshot component must be 1..6");
+  if (int rc = s->ensure_second()) return rc;
+  // snapshot buffers are allocated before the loop so the loop itself allocates nothing
+  s->nsnap = 0;
+  if (snap_every > 0) {
+    const int need = (s->rec_step + count) / snap_every - s->rec_step / snap_every;
+    while ((int)s->dSnaps.size() < need) {
+      void *p = nullptr;
+      hipError_t e = hipMalloc(&p, s->IJK * sizeof(float));
+      if (e != hipSuccess) return fail(FDTD_ERR_NOMEM, "snapshot buffer allocation failed");
+      s->dSnaps.push_back((float *)p);
+    }
+  }
+  if (s->timing) {
+    while ((int)s->ev.size() < 2 * count) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      s->ev.push_back(e);
+    }
+  }
+  int slot = 0;
+  for (int q = 0; q < count; ++q) {
+    if (int rc = s->step(n0 + q, mode, s->timing, &slot)) return rc;
+    if (snap_every > 0 && s->rec_step % snap_every == 0) {
+      HIPCHK(hipMemcpyAsync(s->dSnaps[s->nsnap], s->dF[s->cur][snap_comp - 1], s->IJK * sizeof(float),
+                            hipMemcpyDeviceToDevice, s->stream));
+      s->nsnap++;
+    }
+  }
+  if (s->timing && count > 0) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double tot = 0;
+    for (int q = 0; q < count; ++q) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, s->ev[2 * q], s->ev[2 * q + 1]));
+      tot += ms;
+    }
+    s->kernel_ms = tot;
+    s->kernel_launches = count;
+  }
+  return 0;
+}
+
+int fdtd_snapshot_count(fdtd_sim *s, int32_t *count) {
+  NEED(s);
+  if (!count) return fail(FDTD_ERR_INVALID, "null count");
+  *count = s->nsnap;
+  return 0;
+}
+
+int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size) {
+  NEED_READY(s);
+  if (!out || size < (uint64_t)s->nsnap * s->IJK) return fail(FDTD_ERR_INVALID, "snapshot output too small");
+  for (int q = 0; q < s->nsnap; ++q)
+    HIPCHK(hipMemcpyAsync(out + (size_t)q * s->IJK, s->dSnaps[q], s->IJK * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_synchronize(fdtd_sim *s) {
+  NEED(s);
+  if (s->stream) HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_output_ant(fdtd_sim *s, float *out, uint32_t size) {
+  NEED_READY(s);
+  if (!out) return fail(FDTD_ERR_INVALID, "OutAntSize required.");
+  if (!s->dPort) return fail(FDTD_ERR_STATE, "Antenna data is not initialized.");
+  const size_t n = std::min((size_t)size, s->port_len);
+  HIPCHK(hipMemcpyAsync(out, s->dPort, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_output_vis(fdtd_sim *s, float *out, uint32_t size, int32_t flag) {
+  NEED_READY(s);
+  if (!out) return fail(FDTD_ERR_INVALID, "OutAntSize and flag required.");
+  if (size < s->IJK) return fail(FDTD_ERR_INVALID, "OutputVis: size smaller than I*J*K");
+  if (flag >= 1 && flag <= 6) {
+    HIPCHK(hipMemcpyAsync(out, s->dF[s->cur][flag - 1], s->IJK * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+  }
+  if (flag <= 0 && flag >= -2) {  // host sigma arrays, src/simulation.h:574-588
+    const int c = -flag;
+    if (s->have_arrays) std::memcpy(out, s->sig[c].data(), s->IJK * sizeof(float));
+    else if (s->have_index)
+      for (size_t id = 0; id < s->IJK; ++id) out[id] = s->mtab[6 * s->midx[id] + c];
+    else std::memset(out, 0, s->IJK * sizeof(float));
+    return 0;
+  }
+  return 0;  // other flags leave the output untouched, as the reference
+}
+
+int fdtd_read_array(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
+  NEED_READY(s);
+  const float *src = nullptr;
+  size_t n = 0;
+  std::vector<float> tmp;
+  bool host = false;
+  if (what >= 0 && what < 6) { src = s->dF[s->cur][what]; n = s->IJK; }
+  else if (what >= 6 && what < 12) {
+    const int c = (what - 6) % 3;
+    const bool isCh = what >= 9;
+    tmp.resize(s->IJK);
+    for (size_t id = 0; id < s->IJK; ++id) {
+      float ce, ch;
+      s->cell_coeff(id, c, &ce, &ch);
+      tmp[id] = isCh ? ch : ce;
+    }
+    host = true; n = s->IJK;
+  } else if (what >= 20 && what < 32) { src = s->dPsi[s->cur][what - 20]; n = s->psi_len[what - 20]; }
+  else if (what >= 40 && what < 44) {
+    src = what == 40 ? s->dE1h : what == 41 ? s->dH1h : what == 42 ? s->dEinv : s->dHinv;
+    if (!src) return fail(FDTD_ERR_STATE, "line history not initialised");
+    n = (size_t)s->T * s->N1;
+  } else {
+    return fail(FDTD_ERR_INVALID, "unknown array id");
+  }
+  if (count) *count = n;
+  if (!out) return 0;
+  if (host) { std::memcpy(out, tmp.data(), n * sizeof(float)); return 0; }
+  HIPCHK(hipMemcpyAsync(out, src, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
+  NEED(s);
+  if (!s->prepared) return fail(FDTD_ERR_STATE, "set-up not prepared");
+  std::vector<float> v;
+  if (what == 0) v = s->prof.bE;
+  else if (what == 1) v = s->prof.cE;
+  else if (what == 2) v = s->prof.bH;
+  else if (what == 3) v = s->prof.cH;
+  else if (what == 4) v = s->prof.kapE;
+  else if (what == 5) v = s->prof.kapH;
+  else if (what >= 6 && what < 9) v = s->ikE[what - 6];
+  else if (what >= 9 && what < 12) v = s->ikH[what - 9];
+  else if (what == 12) {
+    const int n1 = s->pw_mode == 5 ? s->obl.N1 : s->N1;
+    v = {s->Chh, s->Che, s->prof.sig_max, (float)n1, (float)s->g.i0, (float)s->g.i1, (float)s->g.j0,
+         (float)s->g.j1, (float)s->g.k0, (float)s->g.k1, (float)s->cp.P, (float)s->obl.ratio, (float)s->obl.src};
+  } else if (what >= 13 && what < 19) {
+    const int c = (what - 13) % 3;
+    const bool isCh = what >= 16;
+    v.resize(s->IJK);
+    for (size_t id = 0; id < s->IJK; ++id) {
+      float ce, ch;
+      s->cell_coeff(id, c, &ce, &ch);
+      v[id] = isCh ? ch : ce;
+    }
+  } else if (what == 19) {
+    v.resize(s->IJK);
+    for (size_t id = 0; id < s->IJK; ++id) v[id] = s->index_mode ? (float)s->cell_mat[id] : -1.0f;
+  } else if (what >= 20 && what < 32) {
+    const int u = (what - 20) / 4, q = (what - 20) % 4;
+    v = q < 2 ? s->obl.tabE[u][q] : s->obl.tabH[u][q - 2];
+  } else if (what == 32) {
+    v = {s->obl.polE[0], s->obl.polE[1], s->obl.polE[2], s->obl.polH[0], s->obl.polH[1], s->obl.polH[2]};
+  } else {
+    return fail(FDTD_ERR_INVALID, "unknown set-up array id");
+  }
+  if (count) *count = v.size();
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(float));
+  return 0;
+}
+
+int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk) {
+  NEED(s);
+  if (tile_rows != 4 && tile_rows != 8) return fail(FDTD_ERR_INVALID, "tile_rows must be 4 or 8");
+  if (z_chunk < 0) return fail(FDTD_ERR_INVALID, "z_chunk must be >= 0");
+  s->tune.tile_rows = tile_rows;
+  s->tune.z_chunk = z_chunk;
+  return 0;
+}
+
+int fdtd_set_timing(fdtd_sim *s, int32_t on) {
+  NEED(s);
+  s->timing = on != 0;
+  return 0;
+}
+
+int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches) {
+  NEED(s);
+  if (avg_ms) *avg_ms = s->kernel_launches ? s->kernel_ms / (double)s->kernel_launches : 0.0;
+  if (launches) *launches = s->kernel_launches;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+"""Seeded synthetic workloads (grids, materials, plane waves, sources, probes).
+
+The reference fixes no seeds; these generators use SplitMix64 with seed 0x5EED0000 + workload
+number so that every run and every backend (HIP library, CPU oracle) sees the same bytes.
+Workloads: 1 = 64^3 vacuum, axis plane wave; 2 = 256^3 with 40 dielectric spheres and an oblique
+plane wave; 3 = 384x384x256 lossy layered ground under a vertical plane wave.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+C0 = 299792458.0
+MU0 = 4 * 3.1415926535897 * 1e-7
+EPS0 = 1 / C0 / C0 / MU0
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.s = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+    def log_uniform(self, lo, hi) -> float:
+        return math.exp(self.uniform(math.log(lo), math.log(hi)))
+
+
+# ---------------------------------------------------------------------------------------------
+# source waveforms (inputs to Init1D / SetSource)
+# ---------------------------------------------------------------------------------------------
+def gauss_sine(T, dt, f0, tau=None, t0=None):
+    """Gaussian-modulated sinusoid exp(-((t-t0)/tau)^2) sin(2 pi f0 (t-t0)), t = n dt."""
+    tau = 1.0 / f0 if tau is None else tau
+    t0 = 4.0 * tau if t0 is None else t0
+    t = np.arange(T, dtype=np.float64) * dt
+    return (np.exp(-(((t - t0) / tau) ** 2)) * np.sin(2 * np.pi * f0 * (t - t0))).astype(np.float32)
+
+
+def gauss_pulse(T, dt, fc):
+    """Gaussian pulse exp(-((t-t0)/tau)^2) with tau = 1/(pi fc) (spectrum centred at 0, -3 dB near fc)."""
+    tau = 1.0 / (math.pi * fc)
+    t0 = 4.0 * tau
+    t = np.arange(T, dtype=np.float64) * dt
+    return np.exp(-(((t - t0) / tau) ** 2)).astype(np.float32)
+
+
+def sinc_source(fctr, dt, bw, lobes, T):
+    """The reference's windowed-sinc waveform (src/cpucode/FunLib.cpp:19-44)."""
+    fmaxx = fctr * bw / 2.0
+    spread = 1.0 / (2.0 * dt * fmaxx)
+    tto = lobes * spread
+    out = np.zeros(T)
+    for n in range(T):
+        nn = n + 1.0
+        v = 0.0
+        if nn < 2.0 * tto:
+            arg = 2.0 * math.pi * fmaxx * (nn - tto) * dt
+            den = 1e-14 if abs(arg) < 1e-14 else arg
+            v = math.sin(arg) / den * math.sin(2.0 * math.pi * fctr * (nn - tto) * dt)
+        v *= math.cos(((nn - 1.0) / T) * (math.pi / 2.0))
+        out[n] = v
+    return out.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------------------------
+@dataclasses.dataclass
+class Scenario:
+    name: str
+    I: int
+    J: int
+    K: int
+    T: int
+    dx: float
+    dt: float
+    fctr: float
+    cpml: tuple            # [P, m, kappa_max, alpha_max, alpha_m, sigma_factor]
+    inset: int
+    mode: int              # plane-wave mode 1..5
+    pw_params: tuple       # SetPlaneWave parameters
+    source: np.ndarray     # PW source samples [T]
+    mat_index: np.ndarray | None = None  # uint8 [K, J, I] (x fastest in memory)
+    mat_table: np.ndarray | None = None  # float32 [n, 6] sigma x,y,z eps x,y,z
+    probes: np.ndarray | None = None     # int32 [nAnt, 3] offsets from the CPML
+    probe_comps: tuple = (3,)
+    snap_every: int = 0
+    snap_comp: int = 3
+    seed: int = 0
+
+    @property
+    def cells(self) -> int:
+        return self.I * self.J * self.K
+
+    @property
+    def P(self) -> int:
+        return int(self.cpml[0])
+
+    def material_arrays(self):
+        """The six UpdateMaterial arrays (flags 1..6), flat x-fastest."""
+        if self.mat_index is None:
+            n = self.cells
+            return [np.zeros(n, np.float32)] * 3 + [np.ones(n, np.float32)] * 3
+        flat = self.mat_index.reshape(-1)
+        return [self.mat_table[flat, c].astype(np.float32) for c in range(6)]
+
+
+def _dt(dx, courant=0.99):
+    return courant * dx / (math.sqrt(3.0) * C0)
+
+
+CONFIG_CPML = (10, 3.0, 5.0, 0.05, 1.0, 0.8)
+
+
+def _probes(rng, n, I, J, K, P):
+    """n probe cells uniform in the interior (offsets from the CPML, as SetAntennaLocations)."""
+    return np.array([[rng.randint(0, I - 2 * P - 1), rng.randint(0, J - 2 * P - 1), rng.randint(0, K - 2 * P - 1)]
+                     for _ in range(n)], dtype=np.int32)
+
+
+def config1(T: int = 600, N: int = 64) -> Scenario:
+    """64^3 vacuum, 10-cell CPML (m=3, kappa_max=5, alpha_max=0.05), inset 15, x-propagating
+    z-polarised plane wave (reference mode 3), Gaussian-modulated sine f0 = 3 GHz, 8 probes Ez/Hy."""
+    seed = 0x5EED0000 + 1
+    rng = SplitMix64(seed)
+    dx = 0.01
+    dt = _dt(dx)
+    f0 = 3e9
+    P = 10
+    return Scenario("config1", N, N, N, T, dx, dt, f0, CONFIG_CPML, P + 5, 3, (0.0,),
+                    gauss_sine(T, dt, f0), probes=_probes(rng, 8, N, N, N, P), probe_comps=(3, 5), seed=seed)
+
+
+def spheres(rng, I, J, K, box, n, rmin, rmax, eps_rng, sig_rng):
+    """n dielectric spheres, centres uniform inside the TF box (no material crosses the TF/SF
+    surface); later spheres overwrite earlier ones.  Returns uint8 index [K,J,I] and table."""
+    i0, i1, j0, j1, k0, k1 = box
+    idx = np.zeros((K, J, I), dtype=np.uint8)
+    table = [[0, 0, 0, 1, 1, 1]]
+    z, y, x = np.ogrid[0:K, 0:J, 0:I]
+    for m in range(n):
+        r = rng.uniform(rmin, rmax)
+        ri = int(math.ceil(r))
+        cx = rng.uniform(i0 + ri + 1, i1 - ri - 1)
+        cy = rng.uniform(j0 + ri + 1, j1 - ri - 1)
+        cz = rng.uniform(k0 + ri + 1, k1 - ri - 1)
+        er = rng.uniform(*eps_rng)
+        sg = rng.uniform(*sig_rng)
+        table.append([sg, sg, sg, er, er, er])
+        zl, zh = max(0, int(cz - r) - 1), min(K, int(cz + r) + 2)
+        yl, yh = max(0, int(cy - r) - 1), min(J, int(cy + r) + 2)
+        xl, xh = max(0, int(cx - r) - 1), min(I, int(cx + r) + 2)
+        sub = ((x[:, :, xl:xh] - cx) ** 2 + (y[:, yl:yh, :] - cy) ** 2 + (z[zl:zh] - cz) ** 2) <= r * r
+        idx[zl:zh, yl:yh, xl:xh][sub] = m + 1
+    return idx, np.asarray(table, dtype=np.float32)
+
+
+def config2(T: int = 2000, N: int = 256, n_spheres: int = 40, n_probes: int = 64) -> Scenario:
+    """256^3, 40 spheres (r in [4,20], eps_r in [2,12], sigma in [0,0.05] S/m) as uint8 index,
+    oblique plane wave at seeded angles, 2000 steps, 64 probes (Ez, Hy), Ez snapshot every 250."""
+    seed = 0x5EED0000 + 2
+    rng = SplitMix64(seed)
+    dx = 0.01
+    dt = _dt(dx)
+    f0 = 3e9
+    P = 10
+    inset = P + 5
+    box = (inset, N - inset - 1) * 3
+    rmax = min(20.0, (N - 2 * inset) / 2 - 3)
+    rmin = min(4.0, rmax)
+    idx, tab = spheres(rng, N, N, N, box, n_spheres, rmin, rmax, (2.0, 12.0), (0.0, 0.05))
+    theta, phi, psi = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
+    return Scenario("config2", N, N, N, T, dx, dt, f0, CONFIG_CPML, inset, 5, (theta, phi, psi),
+                    gauss_sine(T, dt, f0), mat_index=idx, mat_table=tab,
+                    probes=_probes(rng, n_probes, N, N, N, P), probe_comps=(3, 5), snap_every=250 if T >= 250 else 0,
+                    snap_comp=3, seed=seed)
+
+
+def config3(T: int = 4000, I: int = 384, J: int = 384, K: int = 256, lattice: int = 16) -> Scenario:
+    """Lossy layered ground: five horizontal layers (20-80 cells, sigma log-uniform 1e-4..1 S/m,
+    eps_r 4..30) under air, vertical x-polarised Gaussian pulse (reference mode 2, layered 1D line),
+    dx for >= 10 cells per wavelength in the most conductive layer, 12-cell CPML, 16x16 surface
+    probe lattice recording Ex and Hy."""
+    seed = 0x5EED0000 + 3
+    rng = SplitMix64(seed)
+    P = 12
+    fc = rng.uniform(1e6, 100e6)
+    layers = []
+    for _ in range(5):
+        layers.append((rng.randint(20, 80), rng.log_uniform(1e-4, 1.0), rng.uniform(4.0, 30.0)))
+    # wavelength in the most conductive layer at 2 fc (good-conductor skin-depth estimate vs
+    # dielectric wavelength, whichever is shorter)
+    f_hi = 2 * fc
+    lam = []
+    for _, sg, er in layers:
+        w = 2 * math.pi * f_hi
+        eps_c = complex(er * EPS0, -sg / w)
+        kk = w * (MU0 * eps_c) ** 0.5
+        lam.append(2 * math.pi / abs(kk.real))
+    dx = min(lam) / 10.0
+    dt = _dt(dx)
+    inset = P + 5
+    # layers from the bottom CPML upward, clipped to leave an air half-space above
+    idx = np.zeros((K, J, I), dtype=np.uint8)
+    table = [[0, 0, 0, 1, 1, 1]]
+    ztop = K // 2
+    z = ztop
+    for m, (th, sg, er) in enumerate(layers):
+        zl = max(0, z - th)
+        idx[zl:z] = m + 1
+        table.append([sg, sg, sg, er, er, er])
+        z = zl
+        if z == 0:
+            break
+    idx[:z] = idx[z] if z < K else 0
+    tab = np.asarray(table, dtype=np.float32)
+    # 16x16 probe lattice one cell above the ground surface
+    xs = np.linspace(0, I - 2 * P - 1, lattice).astype(int)
+    ys = np.linspace(0, J - 2 * P - 1, lattice).astype(int)
+    zp = ztop - P + 1
+    probes = np.array([[x, y, zp] for y in ys for x in xs], dtype=np.int32)
+    return Scenario("config3", I, J, K, T, dx, dt, fc, (P, 3.0, 5.0, 0.05, 1.0, 0.8), inset, 2, (1.0,),
+                    gauss_pulse(T, dt, fc), mat_index=idx, mat_table=tab, probes=probes, probe_comps=(1, 5),
+                    seed=seed)
+
+
+def small_variant(sc: Scenario, N: int, T: int) -> Scenario:
+    """The same physics on an N^3 grid (materials regenerated, probes clipped) for parity tests."""
+    d = dataclasses.replace(sc, I=N, J=N, K=N, T=T, source=sc.source[:T].copy())
+    P = sc.P
+    if sc.mat_index is not None:
+        rng = SplitMix64(sc.seed ^ 0xABCDEF)
+        box = (sc.inset, N - sc.inset - 1) * 3
+        rmax = max(1.5, min(6.0, (N - 2 * sc.inset) / 2 - 3))
+        idx, tab = spheres(rng, N, N, N, box, 6, 1.0, rmax, (2.0, 12.0), (0.0, 0.05))
+        d.mat_index, d.mat_table = idx, tab
+    if sc.probes is not None:
+        rng = SplitMix64(sc.seed ^ 0x1234)
+        d.probes = _probes(rng, min(8, len(sc.probes)), N, N, N, P)
+    d.snap_every = 0
+    return d
+
+
+def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = True):
+    """Apply the op sequence of highfrequency.m:63-84 (plus the extensions a workload needs) to a
+    backend exposing the op methods (HIP Simulation or CPU oracle)."""
+    nAnt = 0 if sc.probes is None else len(sc.probes)
+    backend.set_const([sc.fctr, sc.dx, sc.dt], [sc.T, sc.I, sc.J, sc.K, nAnt])
+    backend.set_cpml(list(sc.cpml))
+    backend.set_tfsf(sc.inset)
+    backend.set_source(sc.source)
+    if nAnt:
+        backend.set_probe_components(list(sc.probe_comps))
+        backend.set_antenna_locations(np.asarray(sc.probes, dtype=np.int32).T.reshape(-1))
+    if sc.mat_index is not None:
+        if use_arrays or not hasattr(backend, "set_material_index"):
+            arrs = sc.material_arrays()
+            for f in range(6):
+                backend.update_material(f + 1, arrs[f])
+        else:
+            backend.set_material_index(sc.mat_index, sc.mat_table)
+    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4) and sc.pw_params and sc.pw_params[0]):
+        backend.set_plane_wave(sc.mode, sc.pw_params)
+    backend.ready()
+    if init_line and sc.mode:
+        backend.init_1d(1.0, sc.source)
+    backend.update_coeffs()
+    backend.reset()

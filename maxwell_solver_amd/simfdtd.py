@@ -1,0 +1,373 @@
+"""Python mirror of the reference's MATLAB MEX operator API over the C ABI of libfdtd_hip.so.
+
+`simFDTD(op, handle, ...)` keeps the gateway's op names (case-insensitive), argument order and
+error identifiers (src/simFDTD.cu:19-187); `Simulation` is the object form used by the tests and
+the bench.  The library is loaded from this package directory; there is no fallback: a missing
+library raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
+_lib = None
+
+# Every symbol the header declares (include/fdtd.h).
+EXPORTS = [
+    "fdtd_last_error", "fdtd_last_error_id", "fdtd_version", "fdtd_create", "fdtd_destroy",
+    "fdtd_set_const", "fdtd_set_source", "fdtd_set_antenna_locations", "fdtd_update_material",
+    "fdtd_ready", "fdtd_init_1d", "fdtd_update_coeffs", "fdtd_reset_fields", "fdtd_update_h_all",
+    "fdtd_update_e_all", "fdtd_update_h_pml", "fdtd_update_e_pml", "fdtd_update_h_planewave",
+    "fdtd_update_e_planewave", "fdtd_add_currents", "fdtd_output_ant", "fdtd_output_vis",
+    "fdtd_set_cpml", "fdtd_set_tfsf", "fdtd_set_plane_wave", "fdtd_set_material_index",
+    "fdtd_set_probe_components", "fdtd_step", "fdtd_run", "fdtd_snapshot_count",
+    "fdtd_output_snapshots", "fdtd_synchronize", "fdtd_read_array", "fdtd_prepare",
+    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_set_timing", "fdtd_kernel_time",
+]
+
+
+class SimFDTDError(RuntimeError):
+    """Raised with the MEX error identifier of the reference gateway."""
+
+    def __init__(self, identifier: str, message: str):
+        super().__init__(f"{identifier}: {message}")
+        self.identifier = identifier
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run maxwell_solver_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, u32, u64 = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64
+        fp, ip, dp, u8p, u64p = (C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_uint8), C.POINTER(C.c_uint64))
+        sig = {
+            "fdtd_create": [C.POINTER(vp)], "fdtd_destroy": [vp],
+            "fdtd_set_const": [vp, fp, ip], "fdtd_set_source": [vp, fp, i32],
+            "fdtd_set_antenna_locations": [vp, ip], "fdtd_update_material": [vp, i32, fp],
+            "fdtd_ready": [vp], "fdtd_init_1d": [vp, C.c_float, fp], "fdtd_update_coeffs": [vp],
+            "fdtd_reset_fields": [vp], "fdtd_update_h_all": [vp], "fdtd_update_e_all": [vp],
+            "fdtd_update_h_pml": [vp], "fdtd_update_e_pml": [vp],
+            "fdtd_update_h_planewave": [vp, ip], "fdtd_update_e_planewave": [vp, ip],
+            "fdtd_add_currents": [vp, ip, ip], "fdtd_output_ant": [vp, fp, u32],
+            "fdtd_output_vis": [vp, fp, u32, i32], "fdtd_set_cpml": [vp, dp], "fdtd_set_tfsf": [vp, i32],
+            "fdtd_set_plane_wave": [vp, i32, dp], "fdtd_set_material_index": [vp, u8p, fp, i32],
+            "fdtd_set_probe_components": [vp, ip, i32], "fdtd_step": [vp, i32, i32],
+            "fdtd_run": [vp, i32, i32, i32, i32, i32], "fdtd_snapshot_count": [vp, ip],
+            "fdtd_output_snapshots": [vp, fp, u64], "fdtd_synchronize": [vp],
+            "fdtd_read_array": [vp, i32, fp, u64p], "fdtd_prepare": [vp],
+            "fdtd_read_setup": [vp, i32, fp, u64p], "fdtd_set_tuning": [vp, i32, i32],
+            "fdtd_set_timing": [vp, i32], "fdtd_kernel_time": [vp, dp, C.POINTER(C.c_int64)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        for name in ("fdtd_last_error", "fdtd_last_error_id", "fdtd_version"):
+            getattr(L, name).restype = C.c_char_p
+            getattr(L, name).argtypes = []
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+
+
+FIELD = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+
+
+class Simulation:
+    """One GPU simulation (reference: class simulation, src/simulation.h:24)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        self._chk(lib().fdtd_create(C.byref(h)))
+        self._h = h
+        self.dims = None
+        self.T = 0
+        self.nAnt = 0
+        self.ncomp = 1
+
+    @staticmethod
+    def _chk(rc):
+        if rc:
+            L = lib()
+            raise SimFDTDError(L.fdtd_last_error_id().decode(), L.fdtd_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- ops of src/simFDTD.cu ----
+    def set_const(self, freqRes, timeDim):
+        fr, td = _f32(freqRes), _i32(timeDim)
+        self._chk(lib().fdtd_set_const(self._h, _p(fr, C.c_float), _p(td, C.c_int32)))
+        self.T = int(td[0])
+        self.dims = (int(td[1]), int(td[2]), int(td[3]))
+        self.nAnt = int(td[4])
+
+    def set_cpml(self, params):
+        p = _f64(params)
+        self._chk(lib().fdtd_set_cpml(self._h, _p(p, C.c_double)))
+
+    def set_tfsf(self, inset):
+        self._chk(lib().fdtd_set_tfsf(self._h, int(inset)))
+
+    def set_source(self, src, update=False):
+        s = _f32(src)
+        self._chk(lib().fdtd_set_source(self._h, _p(s, C.c_float), 1 if update else 0))
+
+    def set_antenna_locations(self, loc):
+        a = _i32(loc)
+        self._chk(lib().fdtd_set_antenna_locations(self._h, _p(a, C.c_int32)))
+
+    def set_probe_components(self, comps):
+        c = _i32(comps)
+        self._chk(lib().fdtd_set_probe_components(self._h, _p(c, C.c_int32), len(c)))
+        self.ncomp = len(c)
+
+    def update_material(self, flag, values):
+        v = _f32(values)
+        self._chk(lib().fdtd_update_material(self._h, int(flag), _p(v, C.c_float)))
+
+    def set_material_index(self, index, table):
+        idx = np.ascontiguousarray(np.asarray(index, dtype=np.uint8).reshape(-1))
+        tab = np.ascontiguousarray(np.asarray(table, dtype=np.float32).reshape(-1, 6))
+        self._chk(lib().fdtd_set_material_index(self._h, _p(idx, C.c_uint8), _p(tab, C.c_float), tab.shape[0]))
+
+    def set_plane_wave(self, mode, params=(0.0, 0.0, 0.0)):
+        p = _f64(list(params) + [0.0] * (4 - len(params)))
+        self._chk(lib().fdtd_set_plane_wave(self._h, int(mode), _p(p, C.c_double)))
+
+    def prepare(self):
+        self._chk(lib().fdtd_prepare(self._h))
+
+    def ready(self):
+        self._chk(lib().fdtd_ready(self._h))
+
+    def init_1d(self, rate, pw_source):
+        s = _f32(pw_source)
+        self._chk(lib().fdtd_init_1d(self._h, float(rate), _p(s, C.c_float)))
+
+    def update_coeffs(self):
+        self._chk(lib().fdtd_update_coeffs(self._h))
+
+    def reset(self):
+        self._chk(lib().fdtd_reset_fields(self._h))
+
+    def update_h_all(self):
+        self._chk(lib().fdtd_update_h_all(self._h))
+
+    def update_e_all(self):
+        self._chk(lib().fdtd_update_e_all(self._h))
+
+    def update_h_pml(self):
+        self._chk(lib().fdtd_update_h_pml(self._h))
+
+    def update_e_pml(self):
+        self._chk(lib().fdtd_update_e_pml(self._h))
+
+    def update_h_planewave(self, n, mode):
+        t = _i32([n, mode])
+        self._chk(lib().fdtd_update_h_planewave(self._h, _p(t, C.c_int32)))
+
+    def update_e_planewave(self, n, mode):
+        t = _i32([n, mode])
+        self._chk(lib().fdtd_update_e_planewave(self._h, _p(t, C.c_int32)))
+
+    def add_currents(self, n, pol, box):
+        t, b = _i32([n, pol]), _i32(box)
+        self._chk(lib().fdtd_add_currents(self._h, _p(t, C.c_int32), _p(b, C.c_int32)))
+
+    def step(self, n, mode):
+        self._chk(lib().fdtd_step(self._h, int(n), int(mode)))
+
+    def run(self, n0, count, mode, snap_every=0, snap_comp="Ez"):
+        comp = FIELD[snap_comp] + 1 if isinstance(snap_comp, str) else int(snap_comp)
+        self._chk(lib().fdtd_run(self._h, int(n0), int(count), int(mode), int(snap_every), comp))
+
+    def synchronize(self):
+        self._chk(lib().fdtd_synchronize(self._h))
+
+    def set_tuning(self, tile_rows=8, z_chunk=0):
+        self._chk(lib().fdtd_set_tuning(self._h, int(tile_rows), int(z_chunk)))
+
+    def set_timing(self, on=True):
+        self._chk(lib().fdtd_set_timing(self._h, 1 if on else 0))
+
+    def kernel_time(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(lib().fdtd_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # ---- readback ----
+    def output_vis(self, flag):
+        I, J, K = self.dims
+        out = np.zeros(I * J * K, dtype=np.float32)
+        self._chk(lib().fdtd_output_vis(self._h, _p(out, C.c_float), out.size, int(flag)))
+        return out
+
+    def output_ant(self, size=None):
+        n = self.ncomp * self.T * self.nAnt if size is None else int(size)
+        out = np.zeros(n, dtype=np.float32)
+        self._chk(lib().fdtd_output_ant(self._h, _p(out, C.c_float), n))
+        return out
+
+    def probes(self):
+        return self.output_ant().reshape(self.ncomp, self.T, self.nAnt)
+
+    def field(self, name):
+        I, J, K = self.dims
+        return self.read_array(FIELD[name]).reshape(K, J, I)
+
+    def read_array(self, what):
+        n = C.c_uint64()
+        self._chk(lib().fdtd_read_array(self._h, int(what), None, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        self._chk(lib().fdtd_read_array(self._h, int(what), _p(out, C.c_float), C.byref(n)))
+        return out
+
+    def psi(self, q):
+        return self.read_array(20 + q)
+
+    def line(self, which):
+        arr = self.read_array(40 + which)
+        return arr.reshape(self.T, -1)
+
+    def read_setup(self, what):
+        n = C.c_uint64()
+        self._chk(lib().fdtd_read_setup(self._h, int(what), None, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        self._chk(lib().fdtd_read_setup(self._h, int(what), _p(out, C.c_float), C.byref(n)))
+        return out
+
+    def snapshots(self):
+        n = C.c_int32()
+        self._chk(lib().fdtd_snapshot_count(self._h, C.byref(n)))
+        I, J, K = self.dims
+        out = np.empty((n.value, I * J * K), dtype=np.float32)
+        self._chk(lib().fdtd_output_snapshots(self._h, _p(out, C.c_float), out.size))
+        return out
+
+
+# ---------------------------------------------------------------------------------------------
+# MEX-style gateway: simFDTD(op, handle, ...)   (src/simFDTD.cu:19-187)
+# ---------------------------------------------------------------------------------------------
+_handles: dict[int, Simulation] = {}
+
+
+def simFDTD(op: str, *args):
+    if not isinstance(op, str):
+        raise SimFDTDError("simFDTD:InvalidInput", "At least one input required: operation string.")
+    o = op.lower()
+    if o == "createsimulation":
+        s = Simulation()
+        key = id(s)
+        _handles[key] = s
+        return np.uint64(key)
+    if len(args) < 1:
+        raise SimFDTDError("simFDTD:InvalidInput", "Simulation pointer required for this operation.")
+    s = _handles.get(int(args[0]))
+    if s is None:
+        raise SimFDTDError("simFDTD:InvalidInput", "unknown simulation handle")
+    a = args[1:]
+
+    def need(n, msg):
+        if len(a) < n:
+            raise SimFDTDError("simFDTD:InvalidInput", msg)
+
+    if o == "deletesimulation":
+        _handles.pop(int(args[0])).close()
+    elif o == "updatee_all":
+        s.update_e_all()
+    elif o == "updateh_all":
+        s.update_h_all()
+    elif o == "updatee_pml":
+        s.update_e_pml()
+    elif o == "updateh_pml":
+        s.update_h_pml()
+    elif o == "updatecuda":
+        s.update_coeffs()
+    elif o == "addcurrents":
+        need(2, "time and indices required.")
+        t = np.asarray(a[0]).reshape(-1)
+        s.add_currents(int(t[0]), int(t[1]), a[1])
+    elif o == "readysimulation":
+        s.ready()
+    elif o == "resetfieldall":
+        s.reset()
+    elif o in ("updatee_planewave", "updateh_planewave"):
+        need(1, "time required.")
+        t = np.asarray(a[0]).reshape(-1)
+        (s.update_e_planewave if o == "updatee_planewave" else s.update_h_planewave)(int(t[0]), int(t[1]))
+    elif o == "setconst":
+        need(2, "freqRes and TimeDim required.")
+        s.set_const(a[0], a[1])
+    elif o == "setsource":
+        need(1, "SourceIn required.")
+        s.set_source(a[0], update=len(a) > 1)
+    elif o == "setantennalocations":
+        need(1, "Locations required.")
+        s.set_antenna_locations(a[0])
+    elif o == "init1d":
+        need(2, "rate and PW_source required.")
+        s.init_1d(float(np.asarray(a[0]).reshape(-1)[0]), a[1])
+    elif o == "updatematerial":
+        need(2, "flag, values required.")
+        s.update_material(int(np.asarray(a[0]).reshape(-1)[0]), a[1])
+    elif o == "outputant":
+        need(1, "OutAntSize required.")
+        return s.output_ant(int(np.asarray(a[0]).reshape(-1)[0])).reshape(1, -1)
+    elif o == "outputvis":
+        need(2, "OutAntSize and flag required.")
+        size = int(np.asarray(a[0]).reshape(-1)[0])
+        out = s.output_vis(int(np.asarray(a[1]).reshape(-1)[0]))
+        res = np.zeros(size, dtype=np.float32)
+        res[: min(size, out.size)] = out[: min(size, out.size)]
+        return res.reshape(1, -1)
+    # extensions
+    elif o == "setcpml":
+        s.set_cpml(a[0])
+    elif o == "settfsf":
+        s.set_tfsf(int(np.asarray(a[0]).reshape(-1)[0]))
+    elif o == "setplanewave":
+        s.set_plane_wave(int(np.asarray(a[0]).reshape(-1)[0]), a[1] if len(a) > 1 else (0.0,))
+    elif o == "setmaterialindex":
+        s.set_material_index(a[0], a[1])
+    elif o == "setprobecomponents":
+        s.set_probe_components(a[0])
+    elif o == "step":
+        t = np.asarray(a[0]).reshape(-1)
+        s.step(int(t[0]), int(t[1]))
+    elif o == "run":
+        t = np.asarray(a[0]).reshape(-1)
+        s.run(int(t[0]), int(t[1]), int(t[2]), int(t[3]) if t.size > 3 else 0, int(t[4]) if t.size > 4 else 3)
+    else:
+        raise SimFDTDError("simFDTD:UnknownOperation", f"Unknown operation: {o}")
+    return None

@@ -319,6 +319,7 @@ struct Sim : Base {
     if (!init_ready) { g_err = "simFDTD:InvalidInput|ResetFieldAll before ReadySimulation"; return 2; }
     for (int c = 0; c < 6; ++c) std::fill(F[c].begin(), F[c].end(), (R)0);
     for (int q = 0; q < 12; ++q) std::fill(psi[q].begin(), psi[q].end(), (R)0);
+    std::fill(port.begin(), port.end(), (R)0);
     rec_step = 0;
     return 0;
   }

@@ -27,7 +27,8 @@ class OracleError(RuntimeError):
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the committed Makefile."""
     if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+        args = ["make", "-s", "-C", _HERE] + (["-B"] if force else [])
+        subprocess.run(args, check=True)
     return _LIB_PATH
 
 

@@ -1,0 +1,250 @@
+"""GPU parity: the HIP library against the CPU oracle on identical seeded inputs.
+
+All device arithmetic is IEEE float with no contraction, as the oracle's, so the expected result
+is equality; the acceptance bar is relative L2 <= 1e-5 (max-abs and the count of unequal
+elements are reported in the failure message).
+"""
+import numpy as np
+import pytest
+
+from tests.fdtd_compare import FIELDS, assert_close, compare_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+REF_CPML = (10, 3.0, 1.0, 0.08, 1.0, 0.8)
+CFG_CPML = (10, 3.0, 5.0, 0.05, 1.0, 0.8)
+
+
+def _backends():
+    from maxwell_solver_amd import Simulation
+    from oracle.oracle import OracleSim
+    return Simulation(), OracleSim(np.float32)
+
+
+def _scenario(I=40, J=44, K=48, T=60, mode=3, cpml=REF_CPML, inset=15, spheres=True, layered=False,
+              oblique=None, seed=7):
+    from maxwell_solver_amd import scenarios as S
+    dx = 0.01
+    dt = 0.99 * dx / (np.sqrt(3) * S.C0)
+    rng = S.SplitMix64(seed)
+    idx = tab = None
+    if spheres:
+        box = (inset, I - inset - 1, inset, J - inset - 1, inset, K - inset - 1)
+        idx, tab = S.spheres(rng, I, J, K, box, 5, 1.5, 3.5, (2.0, 12.0), (0.0, 0.05))
+    if layered:
+        idx = np.zeros((K, J, I), np.uint8)
+        idx[: K // 2] = 1
+        idx[: K // 3] = 2
+        tab = np.array([[0, 0, 0, 1, 1, 1], [0.01, 0.01, 0.01, 6, 6, 6], [0.2, 0.2, 0.2, 15, 15, 15]], np.float32)
+    params = (1.0,) if layered else (oblique if oblique is not None else (0.0,))
+    probes = np.array([[3, 4, 5], [I - 2 * cpml[0] - 2, 7, 9], [10, J - 2 * cpml[0] - 3, K - 2 * cpml[0] - 1]], np.int32)
+    return S.Scenario("t", I, J, K, T, dx, dt, 3e9, cpml, inset, mode, params, S.gauss_sine(T, dt, 3e9),
+                      mat_index=idx, mat_table=tab, probes=probes, probe_comps=(3, 5, 1))
+
+
+def _per_op_steps(b, T, mode, currents=None):
+    for n in range(T):
+        b.update_h_all()
+        if mode:
+            b.update_h_planewave(n, mode)
+        b.update_h_pml()
+        b.update_e_all()
+        if mode:
+            b.update_e_planewave(n, mode)
+        b.update_e_pml()
+        if currents is not None:
+            b.add_currents(n, currents[0], currents[1])
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+def test_per_op_reference_sequence(mode):
+    """highfrequency.m:85-91 op sequence, reference CPML preset, modes 1..4."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=mode)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    _per_op_steps(g, sc.T, mode)
+    _per_op_steps(o, sc.T, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def test_per_op_gic_currents():
+    """GICsimulation.m:119-124 order (no plane wave, AddCurrents after UpdateE_PML)."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=0, cpml=CFG_CPML)
+    g, o = _backends()
+    S.setup(g, sc, init_line=False)
+    S.setup(o, sc, init_line=False)
+    box = [1, 30, 20, 20, 12, 12]
+    _per_op_steps(g, sc.T, 0, currents=(1, box))
+    _per_op_steps(o, sc.T, 0, currents=(1, box))
+    assert_close(compare_state(g, o), exact=True)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("cpml", [REF_CPML, CFG_CPML], ids=["refcpml", "cfgcpml"])
+def test_fused_step_axis_modes(mode, cpml):
+    """One fused kernel per step equals the per-op sequence of the oracle."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=mode, cpml=cpml)
+    g, o = _backends()
+    S.setup(g, sc, init_line=bool(mode))
+    S.setup(o, sc, init_line=bool(mode))
+    for n in range(sc.T):
+        g.step(n, mode)
+    o.run(0, sc.T, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+@pytest.mark.parametrize("tile_rows,z_chunk", [(4, 0), (8, 0), (8, 5), (4, 13)])
+def test_fused_tilings_oblique(tile_rows, z_chunk):
+    """Oblique plane wave through every tiling: halo rows/planes recomputed from old data."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=52, J=37, K=45, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_tuning(tile_rows, z_chunk)
+    g.run(0, sc.T, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def test_oblique_per_op_and_lines():
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=5, cpml=CFG_CPML, oblique=(2.2, 4.0, 1.1))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    np.testing.assert_array_equal(g.line(0), o.line(0))
+    np.testing.assert_array_equal(g.line(1), o.line(1))
+    _per_op_steps(g, sc.T, 5)
+    _per_op_steps(o, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+
+
+def test_reference_lines_equal():
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=3, cpml=CFG_CPML)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    for w in range(4):
+        np.testing.assert_array_equal(g.line(w), o.line(w))
+
+
+def test_layered_line_mode2():
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=40, J=40, K=56, mode=2, cpml=(12, 3.0, 5.0, 0.05, 1.0, 0.8), inset=17, spheres=False, layered=True)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    np.testing.assert_array_equal(g.line(0), o.line(0))
+    g.run(0, sc.T, 2)
+    o.run(0, sc.T, 2)
+    assert_close(compare_state(g, o), exact=True)
+
+
+def test_full_coefficient_mode_matches_index_mode():
+    """More than 256 distinct materials switch to per-cell coefficient arrays; same result."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=3, cpml=CFG_CPML, spheres=False)
+    I, J, K = sc.I, sc.J, sc.K
+    rng = np.random.default_rng(3)
+    eps = (1 + rng.integers(0, 300, size=I * J * K) * 0.01).astype(np.float32)
+    g, o = _backends()
+    for b in (g, o):
+        S.setup(b, sc)
+        b.update_material(4, eps)
+        b.update_material(5, eps)
+        b.update_coeffs()
+    for n in range(sc.T):
+        g.step(n, 3)
+    o.run(0, sc.T, 3)
+    assert_close(compare_state(g, o), exact=True)
+
+
+def test_snapshots_and_output_vis():
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=3, cpml=CFG_CPML)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, 3, snap_every=20, snap_comp="Ez")
+    snap_o = o.run(0, sc.T, 3, snap_every=20, snap_comp="Ez")
+    snaps = g.snapshots()
+    assert snaps.shape[0] == 3
+    np.testing.assert_array_equal(snaps, snap_o[:3].astype(np.float32))
+    for flag, name in [(1, "Ex"), (2, "Ey"), (3, "Ez"), (4, "Hx"), (5, "Hy"), (6, "Hz")]:
+        np.testing.assert_array_equal(g.output_vis(flag), o.field(name).reshape(-1))
+
+
+def test_workload1_full():
+    """64^3 vacuum workload, 600 steps: probe series and final fields vs the oracle."""
+    from maxwell_solver_amd import scenarios as S
+    sc = S.config1()
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, sc.mode)
+    o.run(0, sc.T, sc.mode)
+    rep = compare_state(g, o)
+    assert_close(rep)
+    assert rel_l2(g.probes(), o.probes()) <= 1e-5
+
+
+def test_workload2_reduced():
+    """Workload-2 physics (spheres, oblique wave) on 64^3, 200 steps."""
+    from maxwell_solver_amd import scenarios as S
+    sc = S.small_variant(S.config2(T=200), 64, 200)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, sc.mode)
+    o.run(0, sc.T, sc.mode)
+    assert_close(compare_state(g, o))
+    assert rel_l2(g.probes(), o.probes()) <= 1e-5
+
+
+def test_workload2_full_size_first_steps():
+    """256^3 workload 2 at its real size: the first 12 steps vs the oracle."""
+    from maxwell_solver_amd import scenarios as S
+    sc = S.config2(T=12)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, sc.mode)
+    o.run(0, sc.T, sc.mode)
+    assert_close(compare_state(g, o, psi=False))
+
+
+def test_workload3_reduced():
+    from maxwell_solver_amd import scenarios as S
+    sc = S.config3(T=200, I=64, J=64, K=96, lattice=4)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, sc.mode)
+    o.run(0, sc.T, sc.mode)
+    assert_close(compare_state(g, o))
+    assert rel_l2(g.probes(), o.probes()) <= 1e-5
+
+
+def test_error_paths():
+    from maxwell_solver_amd import SimFDTDError, Simulation
+    s = Simulation()
+    with pytest.raises(SimFDTDError) as e:
+        s.update_h_all()
+    assert e.value.identifier == "simFDTD:InvalidInput"
+    s.set_const([3e9, 0.01, 1.9e-11], [10, 40, 40, 40, 0])
+    with pytest.raises(SimFDTDError) as e:
+        s.set_const([3e9, 0.01, 1.9e-11], [10, 40, 40, 40, 0])
+    assert e.value.identifier == "simFDTD:AlreadyInitialized"
+    s.ready()
+    with pytest.raises(SimFDTDError):
+        s.update_h_planewave(0, 3)  # before Init1D
+    s.update_h_planewave(0, 9)  # unknown modes are ignored, as the reference
