@@ -1111,7 +1111,18 @@ int orc_get_setup(void *h, int32_t what, double *out) {
         std::memcpy(out, v, sizeof(v));
         break;
       }
-      default: g_err = "simFDTD:InvalidInput|unknown setup array"; return 1;
+      case 32: {
+        for (int c = 0; c < 3; ++c) { out[c] = (double)s->polE[c]; out[3 + c] = (double)s->polH[c]; }
+        break;
+      }
+      default:
+        if (what >= 20 && what < 32) {  // oblique tables: 20 + 4*axis + {E int, E half, H int, H half}
+          const int u = (what - 20) / 4, q = (what - 20) % 4;
+          widen_out(q == 0 ? s->tabE_i[u] : q == 1 ? s->tabE_h[u] : q == 2 ? s->tabH_i[u] : s->tabH_h[u], out);
+          break;
+        }
+        g_err = "simFDTD:InvalidInput|unknown setup array";
+        return 1;
     }
     return 0;
   };

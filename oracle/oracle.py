@@ -235,7 +235,8 @@ class OracleSim:
     def setup(self, what):
         I, J, K = self.dims
         P = int(self.scalars()["P"]) if what != 12 else 0
-        n = {0: 2 * P, 1: 2 * P, 2: 2 * P, 3: 2 * P, 4: P, 5: P, 6: I, 7: J, 8: K, 9: I, 10: J, 11: K, 12: 13}[what]
+        n = {0: 2 * P, 1: 2 * P, 2: 2 * P, 3: 2 * P, 4: P, 5: P, 6: I, 7: J, 8: K, 9: I, 10: J, 11: K, 12: 13,
+             32: 6}.get(what) or (I, J, K)[(what - 20) // 4]
         out = np.empty(n, dtype=np.float64)
         self._chk(lib().orc_get_setup(self._h, int(what), _p(out, C.c_double)))
         return out
