@@ -1,0 +1,149 @@
+"""Benchmark: 3D Yee FDTD step throughput on one MI355X.
+
+Workload (default): 256^3 FP32 grid, 10-cell CPML (m=3, kappa_max=5, alpha_max=0.05), 40 seeded
+dielectric spheres stored as a uint8 material index, oblique TF/SF plane wave at seeded angles,
+64 probes recording Ez and Hy every step, Ez snapshot every 250 steps.  A step is one full time
+step of the grid (H update, TF/SF, CPML, E update, TF/SF, CPML, probes) = one fused kernel.
+
+Prints one JSON line: Gcell-updates/s over exactly --steps timed steps (inputs resident in HBM),
+the fused kernel's HBM roofline fraction from HIP-event kernel times, and the CPU oracle timed on
+a bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_cell(sc, material_bytes=1):
+    """Compulsory HBM bytes per cell-update: 6 components read + written, material, 12 CPML psi
+    arrays read + written over the face slabs."""
+    P = sc.P
+    return 48.0 + material_bytes + 64.0 * P * (1.0 / sc.I + 1.0 / sc.J + 1.0 / sc.K)
+
+
+def cpu_baseline(sc, budget_s=15.0, max_steps=200):
+    """The CPU oracle (float32, OpenMP over k planes) on the same workload, bounded to ~budget_s."""
+    import numpy as np
+
+    from oracle.oracle import OracleSim
+    from maxwell_solver_amd import scenarios as S
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = OracleSim(np.float32)
+    S.setup(o, sc)
+    steps, t_tot = 0, 0.0
+    chunk = 2
+    while steps < max_steps and t_tot < budget_s:
+        t0 = time.perf_counter()
+        o.run(steps, chunk, sc.mode)
+        t_tot += time.perf_counter() - t0
+        steps += chunk
+    o.close()
+    return {
+        "value": round(sc.cells * steps / t_tot / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the {sc.I}x{sc.J}x{sc.K} workload ({sc.cells * steps / 1e9:.2f} G cell-updates, "
+                  f"{t_tot:.1f} s, float32, {threads} OpenMP threads)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--tile-rows", type=int, default=8)
+    ap.add_argument("--z-chunk", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU z-slab "
+              "decomposition is a design, not a build, in this version.", file=sys.stderr)
+        sys.exit(2)
+
+    from maxwell_solver_amd import Simulation
+    from maxwell_solver_amd import scenarios as S
+
+    T = args.warmup + args.steps
+    sc = S.config2(T=max(T, 1))
+    if args.size != 256:
+        sc = S.small_variant(sc, args.size, T)
+        sc.snap_every = 250
+    g = Simulation()
+    S.setup(g, sc)
+    g.set_tuning(args.tile_rows, args.z_chunk)
+
+    # warmup (untimed), then exactly --steps timed steps
+    if args.warmup:
+        g.run(0, args.warmup, sc.mode)
+    g.synchronize()
+    g.set_timing(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(args.warmup, args.steps, sc.mode, snap_every=sc.snap_every, snap_comp=sc.snap_comp)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    kern_ms, launches = g.kernel_time()
+
+    cell_updates = sc.cells * args.steps
+    value = cell_updates / dt / 1e9
+    bpc = bytes_per_cell(sc)
+    achieved = bpc * sc.cells / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_fused_step_{sc.I}.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    rec = {
+        "metric": "Gcell-updates/s",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded spheres, seeded plane-wave angles)",
+        "config": {
+            "workload": f"{sc.I}^3 FP32 Yee FDTD, 10-cell CPML, 40 dielectric spheres (uint8 index), oblique TF/SF "
+                        f"plane wave, 64 probes Ez/Hy, Ez snapshot every {sc.snap_every} steps",
+            "grid": [sc.I, sc.J, sc.K],
+            "cells": sc.cells,
+            "time_steps": args.steps,
+            "kernel": "k_step_fused (one launch per time step)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_cell": round(bpc, 2),
+            "kernel_ms_avg": round(kern_ms, 5),
+            "kernel_launches": launches,
+        },
+    }
+    g.close()
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(S.config2(T=400) if args.size == 256 else sc, budget_s=args.cpu_budget)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()
