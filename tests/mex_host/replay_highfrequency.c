@@ -1,7 +1,7 @@
 /* Test driver: replays the op sequence of highfrequency.m:63-128 through the C MEX gateway
  * (simFDTD_gateway.c) with the fake mx layer, on a small vacuum grid.
  *   replay_highfrequency errors                         -> error-identifier checks (no GPU needed)
- *   replay_highfrequency run MODE N T src.bin out.bin   -> writes Ez after T steps (float32)  */
+ *   replay_highfrequency run MODE N T src.bin out.bin   -> writes the polarised E component after T steps  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -120,7 +120,8 @@ int main(int argc, char **argv) {
   {
     mxArray *sz = vec(mxUINT32_CLASS, 1);
     *(unsigned *)mxGetData(sz) = (unsigned)IJK;
-    const int flag = 3;
+    /* the polarised component: mode 1/4 Ey, mode 2 Ex, mode 3 Ez */
+    const int flag = (mode == 2) ? 1 : (mode == 3) ? 3 : 2;
     mxArray *a[2] = {sz, i32s(1, &flag)};
     OK(op("OutputVis", h, 2, a, &out));
   }
