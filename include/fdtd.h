@@ -126,7 +126,7 @@ int fdtd_prepare(fdtd_sim *s);
  * 19 material id per cell; 20 + 4*axis + {0,1,2,3} oblique tables (E integer, E half, H integer,
  * H half); 32 polarisation [eE x,y,z, eH x,y,z].  *count receives the element count. */
 int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
-/* Launch tuning for the fused kernel: rows per tile (4 or 8) and z-chunk depth (0 = automatic). */
+/* Launch tuning for the fused kernel: rows per tile (4, 8 or 13) and z-chunk depth (0 = automatic). */
 int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk);
 /* Average device time of the fused step kernel over the last fdtd_run with timing on
  * (HIP events around each launch on the simulation's stream). */

@@ -88,9 +88,11 @@ struct fdtd_sim {
 
   // ---- device ----
   hipStream_t stream = nullptr;
+  float *dFb[2] = {}, *dPsiB[2] = {};  // packed field / psi sets
   float *dF[2][6] = {};
   float *dPsi[2][12] = {};
   size_t psi_len[12] = {};
+  size_t psi_total = 0;
   int cur = 0;
   bool second = false;
   uint8_t *dMat = nullptr;
@@ -99,8 +101,9 @@ struct fdtd_sim {
   float *dIkE[3] = {}, *dIkH[3] = {};
   float *dbE = nullptr, *dcE = nullptr, *dbH = nullptr, *dcH = nullptr;
   float *dE1h = nullptr, *dH1h = nullptr, *dEinv = nullptr, *dHinv = nullptr;
-  float *dZero = nullptr, *dTabX = nullptr, *dTabZ = nullptr;
-  float *dObl[3][2][2] = {};  // [axis][E/H][int/half]
+  float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
+  float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
+  float *dPwTab[6] = {};      // per plane-wave mode: E tables x,y,z (int, half), then H tables
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
@@ -108,6 +111,15 @@ struct fdtd_sim {
   int nsnap = 0;
   PWDev pwdev[6];
   StepTuning tune;
+  // device argument blocks of the fused kernel per (state parity, plane-wave mode); rebuilt when
+  // args_gen moves (any buffer or tuning change)
+  struct ArgSlot {
+    int gen = -1, nwg = 0;
+    void *dptr = nullptr;
+    std::vector<char> host;
+  };
+  ArgSlot argslots[2][6];
+  int args_gen = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;
   double kernel_ms = 0;
@@ -303,13 +315,17 @@ struct fdtd_sim {
   }
 
   int upload_coefficients() {
+    args_gen++;
     if (index_mode) {
       if (int rc = upload(&dMat, cell_mat)) return rc;
       if (int rc = upload(&dTab, coef_tab)) return rc;
     } else {
+      if (int rc = dalloc(&dCoefFull, 6 * IJK)) return rc;
       for (int c = 0; c < 3; ++c) {
-        if (int rc = upload(&dCe[c], cell_Ce[c])) return rc;
-        if (int rc = upload(&dCh[c], cell_Ch[c])) return rc;
+        dCe[c] = dCoefFull + c * IJK;
+        dCh[c] = dCoefFull + (3 + c) * IJK;
+        HIPCHK(hipMemcpyAsync(dCe[c], cell_Ce[c].data(), IJK * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dCh[c], cell_Ch[c].data(), IJK * sizeof(float), hipMemcpyHostToDevice, stream));
       }
     }
     return 0;
@@ -321,44 +337,54 @@ struct fdtd_sim {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FDTD_ERR_DEVICE, "no HIP device");
     HIPCHK(hipStreamCreate(&stream));
-    for (int c = 0; c < 6; ++c)
-      if (int rc = dalloc(&dF[0][c], IJK)) return rc;
-    const size_t P = (size_t)cp.P;
-    const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
-    const size_t sizes[12] = {sy, sz, sx, sz, sx, sy, sy, sz, sx, sz, sx, sy};
-    for (int q = 0; q < 12; ++q) {
-      psi_len[q] = sizes[q];
-      if (int rc = dalloc(&dPsi[0][q], sizes[q])) return rc;
-    }
+    if (int rc = alloc_set(0)) return rc;
     if (int rc = upload_coefficients()) return rc;
-    for (int c = 0; c < 3; ++c) {
-      if (int rc = upload(&dIkE[c], ikE[c])) return rc;
-      if (int rc = upload(&dIkH[c], ikH[c])) return rc;
-    }
-    if (int rc = upload(&dbE, prof.bE)) return rc;
-    if (int rc = upload(&dcE, prof.cE)) return rc;
-    if (int rc = upload(&dbH, prof.bH)) return rc;
-    if (int rc = upload(&dcH, prof.cH)) return rc;
+    // parameter table: bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
+    std::vector<float> prm;
+    for (const auto *v : {&prof.bE, &prof.cE, &prof.bH, &prof.cH, &ikE[0], &ikE[1], &ikE[2], &ikH[0], &ikH[1], &ikH[2]})
+      prm.insert(prm.end(), v->begin(), v->end());
+    if (int rc = upload(&dPrm, prm)) return rc;
+    const size_t P2 = 2 * (size_t)cp.P;
+    dbE = dPrm; dcE = dPrm + P2; dbH = dPrm + 2 * P2; dcH = dPrm + 3 * P2;
+    dIkE[0] = dPrm + 4 * P2; dIkE[1] = dIkE[0] + I; dIkE[2] = dIkE[1] + J;
+    dIkH[0] = dIkE[2] + K; dIkH[1] = dIkH[0] + I; dIkH[2] = dIkH[1] + J;
     init_ready = true;
     if (int rc = alloc_probes()) return rc;
     return zero_state();
   }
 
   int zero_state() {
-    for (int c = 0; c < 6; ++c) HIPCHK(hipMemsetAsync(dF[cur][c], 0, IJK * sizeof(float), stream));
-    for (int q = 0; q < 12; ++q) HIPCHK(hipMemsetAsync(dPsi[cur][q], 0, psi_len[q] * sizeof(float), stream));
+    HIPCHK(hipMemsetAsync(dFb[cur], 0, 6 * IJK * sizeof(float), stream));
+    HIPCHK(hipMemsetAsync(dPsiB[cur], 0, psi_total * sizeof(float), stream));
     if (dPort) HIPCHK(hipMemsetAsync(dPort, 0, port_len * sizeof(float), stream));
     rec_step = 0;
     return 0;
   }
 
+  // One field set (six components at stride I*J*K) and one psi set, packed by slab orientation:
+  // x slabs [Eyx Ezx Hyx Hzx], y slabs [Exy Ezy Hxy Hzy], z slabs [Exz Eyz Hxz Hyz].
+  int alloc_set(int which) {
+    const size_t P = (size_t)cp.P;
+    const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
+    if (int rc = dalloc(&dFb[which], 6 * IJK)) return rc;
+    for (int c = 0; c < 6; ++c) dF[which][c] = dFb[which] + c * IJK;
+    if (int rc = dalloc(&dPsiB[which], 4 * (sx + sy + sz))) return rc;
+    float *px = dPsiB[which], *py = px + 4 * sx, *pz = py + 4 * sy;
+    float **q = dPsi[which];
+    q[P_EYX] = px; q[P_EZX] = px + sx; q[P_HYX] = px + 2 * sx; q[P_HZX] = px + 3 * sx;
+    q[P_EXY] = py; q[P_EZY] = py + sy; q[P_HXY] = py + 2 * sy; q[P_HZY] = py + 3 * sy;
+    q[P_EXZ] = pz; q[P_EYZ] = pz + sz; q[P_HXZ] = pz + 2 * sz; q[P_HYZ] = pz + 3 * sz;
+    const size_t len[12] = {sy, sz, sx, sz, sx, sy, sy, sz, sx, sz, sx, sy};
+    for (int k = 0; k < 12; ++k) psi_len[k] = len[k];
+    psi_total = 4 * (sx + sy + sz);
+    return 0;
+  }
+
   int ensure_second() {
     if (second) return 0;
-    for (int c = 0; c < 6; ++c)
-      if (int rc = dalloc(&dF[1][c], IJK)) return rc;
-    for (int q = 0; q < 12; ++q)
-      if (int rc = dalloc(&dPsi[1][q], psi_len[q])) return rc;
+    if (int rc = alloc_set(1)) return rc;
     second = true;
+    args_gen++;
     return 0;
   }
 
@@ -391,11 +417,6 @@ struct fdtd_sim {
       N1 = obl.N1;
       line_src = obl.src;
       if (int rc = run_line(N1, obl.ds, obl.Ch1, std::vector<float>(N1, 1.0f), std::vector<float>(N1, obl.Cb1), obl.src, pw, &dE1h, &dH1h)) return rc;
-      for (int u = 0; u < 3; ++u)
-        for (int t = 0; t < 2; ++t) {
-          if (int rc = upload(&dObl[u][0][t], obl.tabE[u][t])) return rc;
-          if (int rc = upload(&dObl[u][1][t], obl.tabH[u][t])) return rc;
-        }
     } else if (pw_layered && pw_mode >= 1 && pw_mode <= 4) {
       // exact restriction of the centre column along the propagation axis
       const int ax = pw_mode <= 2 ? 2 : 0;
@@ -440,45 +461,48 @@ struct fdtd_sim {
     }
   }
 
+  // Per-mode incident-field tables, one buffer each: E tables (x int, x half, y int, y half, z int,
+  // z half) then the same for H.  Axis modes 1..4 map the propagation coordinate q to line node
+  // q + N1 - Q (src/simulation.h:330-369) and hold zeros on the other axes.
   int build_pw() {
-    const int Qm = std::max(std::max(I, J), K);
-    std::vector<float> zero((size_t)Qm, 0.0f), tx((size_t)I), tz((size_t)K);
-    for (int q = 0; q < I; ++q) tx[q] = (float)(q + N1 - I);
-    for (int q = 0; q < K; ++q) tz[q] = (float)(q + N1 - K);
-    if (!dZero && upload(&dZero, zero)) return FDTD_ERR_NOMEM;
-    if (upload(&dTabX, tx) || upload(&dTabZ, tz)) return FDTD_ERR_NOMEM;
+    args_gen++;
+    const size_t tl = 2 * ((size_t)I + J + K);
     for (int m = 0; m < 6; ++m) pwdev[m] = PWDev();
-    for (int m = 1; m <= 4; ++m) {
+    for (int m = 1; m <= 5; ++m) {
+      if (m == 5 && pw_mode != 5) continue;
+      std::vector<float> t(2 * tl, 0.0f);
+      const int Q[3] = {I, J, K};
+      size_t off[3] = {0, 2 * (size_t)I, 2 * (size_t)I + 2 * (size_t)J};
+      for (int eh = 0; eh < 2; ++eh)
+        for (int u = 0; u < 3; ++u)
+          for (int h = 0; h < 2; ++h)
+            for (int q = 0; q < Q[u]; ++q) {
+              float v = 0.0f;
+              if (m == 5) v = eh == 0 ? obl.tabE[u][h][q] : obl.tabH[u][h][q];
+              else if ((m <= 2 && u == 2) || (m >= 3 && u == 0)) v = (float)(q + N1 - Q[u]);
+              t[eh * tl + off[u] + (size_t)h * Q[u] + q] = v;
+            }
+      if (int rc = upload(&dPwTab[m], t)) return rc;
       PWDev &p = pwdev[m];
-      const bool zax = m <= 2;
-      const float *ax = zax ? dTabZ : dTabX;
       for (int u = 0; u < 3; ++u)
-        for (int t = 0; t < 2; ++t) {
-          const bool on = (zax && u == 2) || (!zax && u == 0);
-          p.tabE[u][t] = on ? ax : dZero;
-          p.tabH[u][t] = on ? ax : dZero;
+        for (int h = 0; h < 2; ++h) {
+          p.tabE[u][h] = dPwTab[m] + off[u] + (size_t)h * Q[u];
+          p.tabH[u][h] = dPwTab[m] + tl + off[u] + (size_t)h * Q[u];
         }
       p.N1 = N1;
       p.Eh = m == 4 ? dEinv : dE1h;
       p.Hh = m == 4 ? dHinv : dH1h;
-      if (!p.Eh || !p.Hh) continue;  // line of this mode not computed
+      if (!p.Eh || !p.Hh) continue;  // the line of this mode was not computed
+      const int all = (1 << 1) | (1 << 2) | (1 << 3) | (1 << 5) | (1 << 6) | (1 << 7);
       if (m == 1) { p.polE[1] = 1; p.polH[0] = 1; p.active_h = (1 << 6) | (1 << 2); p.active_e = (1 << 7) | (1 << 5); }
       if (m == 2) { p.polE[0] = 1; p.polH[1] = -1; p.active_h = (1 << 7) | (1 << 5); p.active_e = (1 << 6) | (1 << 2); }
       if (m == 3) { p.polE[2] = 1; p.polH[1] = 1; p.active_h = (1 << 3) | (1 << 1); p.active_e = (1 << 6) | (1 << 2); }
       if (m == 4) { p.polE[1] = 1; p.polH[2] = -1; p.active_h = (1 << 6) | (1 << 2); p.active_e = (1 << 3) | (1 << 1); }
-    }
-    if (pw_mode == 5) {
-      PWDev &p = pwdev[5];
-      for (int u = 0; u < 3; ++u)
-        for (int t = 0; t < 2; ++t) {
-          p.tabE[u][t] = dObl[u][0][t];
-          p.tabH[u][t] = dObl[u][1][t];
-        }
-      for (int c = 0; c < 3; ++c) { p.polE[c] = obl.polE[c]; p.polH[c] = obl.polH[c]; }
-      p.N1 = N1; p.Eh = dE1h; p.Hh = dH1h;
-      const int all = (1 << 1) | (1 << 2) | (1 << 3) | (1 << 5) | (1 << 6) | (1 << 7);
-      p.active_h = all;
-      p.active_e = all;
+      if (m == 5) {
+        for (int c = 0; c < 3; ++c) { p.polE[c] = obl.polE[c]; p.polH[c] = obl.polH[c]; }
+        p.active_h = all;
+        p.active_e = all;
+      }
     }
     return 0;
   }
@@ -505,9 +529,31 @@ struct fdtd_sim {
     const PWDev *pw = nullptr;
     if (int rc = check_pw(n, mode, &pw)) return rc;
     if (int rc = ensure_second()) return rc;
-    const PWDev none;
+    const int slot_mode = pw ? mode : 0;
+    ArgSlot &slot = argslots[cur][slot_mode];
+    if (slot.gen != args_gen) {
+      FusedDev f;
+      f.g = g;
+      f.Fc = dFb[cur]; f.Fn = dFb[1 - cur]; f.Pc = dPsiB[cur]; f.Pn = dPsiB[1 - cur];
+      f.mat = index_mode ? dMat : nullptr;
+      f.coef = index_mode ? dTab : dCoefFull;
+      f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
+      f.prm = dPrm;
+      if (pw) {
+        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1;
+        f.act_h = pw->active_h; f.act_e = pw->active_e;
+        for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
+      }
+      f.Chh = Chh; f.Che = Che; f.dx = delX;
+      slot.host.assign(step_args_size(), 0);
+      slot.nwg = make_step_args(f, tune, slot.host.data());
+      if (slot.nwg < 0) return fail(FDTD_ERR_INVALID, "grid not supported by the fused kernel");
+      if (!slot.dptr && dalloc((char **)&slot.dptr, slot.host.size())) return FDTD_ERR_NOMEM;
+      HIPCHK(hipMemcpyAsync(slot.dptr, slot.host.data(), slot.host.size(), hipMemcpyHostToDevice, stream));
+      slot.gen = args_gen;
+    }
     if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
-    HIPCHK(launch_step_fused(stream, g, fields(cur), fields(1 - cur), psis(cur), psis(1 - cur), coefs(), pw ? *pw : none, n, tune));
+    HIPCHK(launch_step_fused(stream, slot.dptr, slot.host.data(), slot.nwg, n));
     if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
     cur = 1 - cur;
     return record_probes(cur);
@@ -925,10 +971,11 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
 
 int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk) {
   NEED(s);
-  if (tile_rows != 4 && tile_rows != 8) return fail(FDTD_ERR_INVALID, "tile_rows must be 4 or 8");
+  if (tile_rows != 4 && tile_rows != 8 && tile_rows != 13) return fail(FDTD_ERR_INVALID, "tile_rows must be 4, 8 or 13");
   if (z_chunk < 0) return fail(FDTD_ERR_INVALID, "z_chunk must be >= 0");
   s->tune.tile_rows = tile_rows;
   s->tune.z_chunk = z_chunk;
+  s->args_gen++;
   return 0;
 }
 

@@ -15,6 +15,8 @@
 // reference's association order so results equal the CPU oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "fdtd_kernels.hpp"
 
 namespace fdtd {
@@ -556,157 +558,445 @@ __global__ void k_probes(FieldSet F, const int *idx, int nAnt, const int *comps,
 
 // ------------------------------------------------------------------------------------------
 // fused step kernel
+//
+// Tiles of 64 columns x TY rows x KC planes.  Waves 0..TY+1 each own one grid row (row jb-1 is
+// the low halo row, jb+TY the high halo row); wave TY+2 covers the tile's halo columns ib-1 and
+// ib+64 for all rows.  Every thread thus stands for one cell (i, j) of a (TY+2) x 66 plane tile
+// and marches k up the chunk:
+//   load E(k+1), H(k) (one plane ahead), stage E(k) in LDS, barrier,
+//   H(k) at n+1/2 for rows jb-1..jb+TY-1 and columns ib-1..ib+63 -> LDS (own cells also -> HBM),
+//   barrier, E(k) at n+1 for the own cells -> HBM.
+// H of halo cells and of the halo plane kb-1 is recomputed from the old (cur) state, so tiles never
+// wait for each other; the new state goes to the other buffer (nxt).  Buffers are packed (one base
+// pointer per field set, psi set and parameter table) to keep the argument block in few SGPRs.
 // ------------------------------------------------------------------------------------------
 struct StepArgs {
-  GridDev g;
-  FieldSet cur, nxt;
-  PsiSet pcur, pnxt;
-  CoefDev co;
-  PWDev pw;
-  int n;
-  int KC;   // z chunk depth
-  int nTy;  // row tiles
+  int I, J, K, P;
+  int i0, i1, j0, j1, k0, k1;
+  int nTx, nTy, KC, ntab;
+  int act_h, act_e, N1, pad;
+  size_t IJK, sx, sy, sz;
+  const float *Fc;
+  float *Fn;
+  const float *Pc;
+  float *Pn;
+  const uint8_t *mat;
+  const float *coef;   // index mode: [ntab][6]; full mode: [6][IJK]
+  const float *prm;    // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
+  const float *pwtab;  // E tables x(int,half) y(int,half) z(int,half), then the same for H
+  const float *Eh, *Hh;  // 1D line histories [T][N1]
+  float Chh, Che, dx;
+  float polE[3], polH[3];
 };
 
+// psi groups of one set: x slabs [Eyx Ezx Hyx Hzx], y slabs [Exy Ezy Hxy Hzy], z slabs [Exz Eyz Hxz Hyz]
+struct PsiView {
+  const float *c;
+  float *n;
+  size_t sx, sy, sz;
+  __device__ __forceinline__ size_t off_x(int q) const { return (size_t)q * sx; }
+  __device__ __forceinline__ size_t off_y(int q) const { return 4 * sx + (size_t)q * sy; }
+  __device__ __forceinline__ size_t off_z(int q) const { return 4 * sx + 4 * sy + (size_t)q * sz; }
+};
+
+__device__ __forceinline__ float tab_e(const StepArgs &a, const float *row, int c, int x, int y, int z) {
+  const float *t = a.pwtab;
+  const float sx = t[(c == 0 ? a.I : 0) + x];
+  const float sy = t[2 * a.I + (c == 1 ? a.J : 0) + y];
+  const float sz = t[2 * a.I + 2 * a.J + (c == 2 ? a.K : 0) + z];
+  return a.polE[c] * line_at(row, (sx + sy) + sz);
+}
+__device__ __forceinline__ float tab_h(const StepArgs &a, const float *row, int c, int x, int y, int z) {
+  const float *t = a.pwtab + 2 * (a.I + a.J + a.K);
+  const float sx = t[(c != 0 ? a.I : 0) + x];
+  const float sy = t[2 * a.I + (c != 1 ? a.J : 0) + y];
+  const float sz = t[2 * a.I + 2 * a.J + (c != 2 ? a.K : 0) + z];
+  return a.polH[c] * line_at(row, (sx + sy) + sz);
+}
+
+struct F3 {
+  float x, y, z;
+};
+
+// TF/SF corrections of an H cell (faces x, y, z in order), generic form of src/gpu.h:152-573.
+// Out of line: only TF/SF surface cells take this path.
+__device__ __noinline__ F3 fused_pw_h(const StepArgs *__restrict__ pa, int n, int i, int j, int k, F3 hv) {
+  const StepArgs &a = *pa;
+  const float *row = a.Eh + (size_t)n * a.N1;
+  float hx = hv.x, hy = hv.y, hz = hv.z;
+  const int m = a.act_h;
+  const float Che = a.Che;
+  if (i == a.i0 - 1 || i == a.i1) {
+    const bool lo = i == a.i0 - 1;
+    const int xe = lo ? a.i0 : a.i1;
+    if ((m & (1 << 1)) && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1 - 1) {
+      const float v = Che * tab_e(a, row, 2, xe, j, k);
+      hy = lo ? hy - v : hy + v;
+    }
+    if ((m & (1 << 2)) && j >= a.j0 && j <= a.j1 - 1 && k >= a.k0 && k <= a.k1) {
+      const float v = Che * tab_e(a, row, 1, xe, j, k);
+      hz = lo ? hz + v : hz - v;
+    }
+  }
+  if (j == a.j0 - 1 || j == a.j1) {
+    const bool lo = j == a.j0 - 1;
+    const int ye = lo ? a.j0 : a.j1;
+    if ((m & (1 << 3)) && i >= a.i0 && i <= a.i1 && k >= a.k0 && k <= a.k1 - 1) {
+      const float v = Che * tab_e(a, row, 2, i, ye, k);
+      hx = lo ? hx + v : hx - v;
+    }
+    if ((m & (1 << 5)) && i >= a.i0 && i <= a.i1 - 1 && k >= a.k0 && k <= a.k1) {
+      const float v = Che * tab_e(a, row, 0, i, ye, k);
+      hz = lo ? hz - v : hz + v;
+    }
+  }
+  if (k == a.k0 - 1 || k == a.k1) {
+    const bool lo = k == a.k0 - 1;
+    const int ze = lo ? a.k0 : a.k1;
+    if ((m & (1 << 6)) && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 - 1) {
+      const float v = Che * tab_e(a, row, 1, i, j, ze);
+      hx = lo ? hx - v : hx + v;
+    }
+    if ((m & (1 << 7)) && i >= a.i0 && i <= a.i1 - 1 && j >= a.j0 && j <= a.j1) {
+      const float v = Che * tab_e(a, row, 0, i, j, ze);
+      hy = lo ? hy + v : hy - v;
+    }
+  }
+  return F3{hx, hy, hz};
+}
+
+__device__ __noinline__ F3 fused_pw_e(const StepArgs *__restrict__ pa, int n, F3 chv, int i, int j, int k, F3 ev) {
+  const StepArgs &a = *pa;
+  const float *row = a.Hh + (size_t)n * a.N1;
+  const float ch[3] = {chv.x, chv.y, chv.z};
+  float ex = ev.x, ey = ev.y, ez = ev.z;
+  const int m = a.act_e;
+  if (i == a.i0 || i == a.i1) {
+    const bool lo = i == a.i0;
+    const int xh = lo ? a.i0 - 1 : a.i1;
+    if ((m & (1 << 2)) && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1 - 1) {
+      const float v = ch[2] * tab_h(a, row, 1, xh, j, k);
+      ez = lo ? ez - v : ez + v;
+    }
+    if ((m & (1 << 1)) && j >= a.j0 && j <= a.j1 - 1 && k >= a.k0 && k <= a.k1) {
+      const float v = ch[1] * tab_h(a, row, 2, xh, j, k);
+      ey = lo ? ey + v : ey - v;
+    }
+  }
+  if (j == a.j0 || j == a.j1) {
+    const bool lo = j == a.j0;
+    const int yh = lo ? a.j0 - 1 : a.j1;
+    if ((m & (1 << 3)) && i >= a.i0 && i <= a.i1 - 1 && k >= a.k0 && k <= a.k1) {
+      const float v = ch[0] * tab_h(a, row, 2, i, yh, k);
+      ex = lo ? ex - v : ex + v;
+    }
+    if ((m & (1 << 5)) && i >= a.i0 && i <= a.i1 && k >= a.k0 && k <= a.k1 - 1) {
+      const float v = ch[2] * tab_h(a, row, 0, i, yh, k);
+      ez = lo ? ez + v : ez - v;
+    }
+  }
+  if (k == a.k0 || k == a.k1) {
+    const bool lo = k == a.k0;
+    const int zh = lo ? a.k0 - 1 : a.k1;
+    if ((m & (1 << 6)) && i >= a.i0 && i <= a.i1 - 1 && j >= a.j0 && j <= a.j1) {
+      const float v = ch[0] * tab_h(a, row, 1, i, j, zh);
+      ex = lo ? ex + v : ex - v;
+    }
+    if ((m & (1 << 7)) && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 - 1) {
+      const float v = ch[1] * tab_h(a, row, 0, i, j, zh);
+      ey = lo ? ey - v : ey + v;
+    }
+  }
+  return F3{ex, ey, ez};
+}
+
+// CPML of an H cell: slabs Y, X, Z in the order of src/simulation.h:392-403.  Out of line: only
+// slab cells take this path.  d = {dEz/dy, dEx/dy, dEz/dx, dEy/dx, dEy/dz, dEx/dz} differences.
+struct D6 {
+  float a, b, c, d, e, f;
+};
+__device__ __noinline__ F3 fused_pml_h(const StepArgs *__restrict__ pa, bool own, int i, int j, int k, D6 dd, F3 hv) {
+  const StepArgs &a = *pa;
+  float hx = hv.x, hy = hv.y, hz = hv.z;
+  const int I = a.I, J = a.J, K = a.K, P = a.P;
+  const float Che = a.Che, dx = a.dx;
+  const float *bH = a.prm + 4 * P, *cH = a.prm + 6 * P;
+  const PsiView ps{a.Pc, a.Pn, a.sx, a.sy, a.sz};
+  const size_t IJ = (size_t)I * J;
+  if (j < P || j >= J - P) {
+    const int s = j < P ? j : j - (J - 2 * P);
+    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
+    const size_t oxy = ps.off_y(2) + t, ozy = ps.off_y(3) + t;
+    if (s < 2 * P - 1 && i < I - 1 && k < K - 1) {
+      const float pxy = bH[s] * ps.c[oxy] + cH[s] * dd.a / dx;
+      hx = hx - Che * pxy * dx;
+      const float pzy = bH[s] * ps.c[ozy] + cH[s] * dd.b / dx;
+      hz = hz + Che * pzy * dx;
+      if (own) { ps.n[oxy] = pxy; ps.n[ozy] = pzy; }
+    } else if (own) {
+      ps.n[oxy] = ps.c[oxy];
+      ps.n[ozy] = ps.c[ozy];
+    }
+  }
+  if (i < P || i >= I - P) {
+    const int s = i < P ? i : i - (I - 2 * P);
+    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
+    const size_t oyx = ps.off_x(2) + t, ozx = ps.off_x(3) + t;
+    if (s < 2 * P - 1 && j < J - 1 && k < K - 1) {
+      const float pyx = bH[s] * ps.c[oyx] + cH[s] * dd.c / dx;
+      hy = hy + Che * pyx * dx;
+      const float pzx = bH[s] * ps.c[ozx] + cH[s] * dd.d / dx;
+      hz = hz - Che * pzx * dx;
+      if (own) { ps.n[oyx] = pyx; ps.n[ozx] = pzx; }
+    } else if (own) {
+      ps.n[oyx] = ps.c[oyx];
+      ps.n[ozx] = ps.c[ozx];
+    }
+  }
+  if (k < P || k >= K - P) {
+    const int s = k < P ? k : k - (K - 2 * P);
+    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
+    const size_t oxz = ps.off_z(2) + t, oyz = ps.off_z(3) + t;
+    if (s < 2 * P - 1 && i < I - 1 && j < J - 1) {
+      const float pxz = bH[s] * ps.c[oxz] + cH[s] * dd.e / dx;
+      hx = hx + Che * pxz * dx;
+      const float pyz = bH[s] * ps.c[oyz] + cH[s] * dd.f / dx;
+      hy = hy - Che * pyz * dx;
+      if (own) { ps.n[oxz] = pxz; ps.n[oyz] = pyz; }
+    } else if (own) {
+      ps.n[oxz] = ps.c[oxz];
+      ps.n[oyz] = ps.c[oyz];
+    }
+  }
+  return F3{hx, hy, hz};
+}
+
+// CPML of an E cell: slabs Y, Z, X in the order of src/simulation.h:404-418.
+// d = {dHz/dy, dHx/dy, dHy/dz, dHx/dz, dHz/dx, dHy/dx} backward differences.
+__device__ __noinline__ F3 fused_pml_e(const StepArgs *__restrict__ pa, int i, int j, int k, F3 chv, D6 dd, F3 ev) {
+  const StepArgs &a = *pa;
+  const float ch[3] = {chv.x, chv.y, chv.z};
+  float ex = ev.x, ey = ev.y, ez = ev.z;
+  const int I = a.I, J = a.J, K = a.K, P = a.P;
+  const float dx = a.dx;
+  const float *bE = a.prm, *cE = a.prm + 2 * P;
+  const PsiView ps{a.Pc, a.Pn, a.sx, a.sy, a.sz};
+  const size_t IJ = (size_t)I * J;
+  if (j < P || j >= J - P) {
+    const int s = j < P ? j : j - (J - 2 * P);
+    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
+    const size_t oxy = ps.off_y(0) + t, ozy = ps.off_y(1) + t;
+    if (s >= 1 && i >= 1 && i < I - 1 && k >= 1 && k < K - 1) {
+      const float pxy = bE[s] * ps.c[oxy] + cE[s] * dd.a / dx;
+      ex = ex + ch[0] * pxy * dx;
+      const float pzy = bE[s] * ps.c[ozy] + cE[s] * dd.b / dx;
+      ez = ez - ch[2] * pzy * dx;
+      ps.n[oxy] = pxy;
+      ps.n[ozy] = pzy;
+    } else {
+      ps.n[oxy] = ps.c[oxy];
+      ps.n[ozy] = ps.c[ozy];
+    }
+  }
+  if (k < P || k >= K - P) {
+    const int s = k < P ? k : k - (K - 2 * P);
+    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
+    const size_t oxz = ps.off_z(0) + t, oyz = ps.off_z(1) + t;
+    if (s >= 1 && i >= 1 && i < I - 1 && j >= 1 && j < J - 1) {
+      const float pxz = bE[s] * ps.c[oxz] + cE[s] * dd.c / dx;
+      ex = ex - ch[0] * pxz * dx;
+      const float pyz = bE[s] * ps.c[oyz] + cE[s] * dd.d / dx;
+      ey = ey + ch[1] * pyz * dx;
+      ps.n[oxz] = pxz;
+      ps.n[oyz] = pyz;
+    } else {
+      ps.n[oxz] = ps.c[oxz];
+      ps.n[oyz] = ps.c[oyz];
+    }
+  }
+  if (i < P || i >= I - P) {
+    const int s = i < P ? i : i - (I - 2 * P);
+    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
+    const size_t oyx = ps.off_x(0) + t, ozx = ps.off_x(1) + t;
+    if (s >= 1 && j >= 1 && j < J - 1 && k >= 1 && k < K - 1) {
+      const float pyx = bE[s] * ps.c[oyx] + cE[s] * dd.e / dx;
+      ey = ey - ch[1] * pyx * dx;
+      const float pzx = bE[s] * ps.c[ozx] + cE[s] * dd.f / dx;
+      ez = ez + ch[2] * pzx * dx;
+      ps.n[oyx] = pyx;
+      ps.n[ozx] = pzx;
+    } else {
+      ps.n[oyx] = ps.c[oyx];
+      ps.n[ozx] = ps.c[ozx];
+    }
+  }
+  return F3{ex, ey, ez};
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range
-// of tiles (neighbouring row tiles of one z chunk share halo rows in that XCD's L2).
+// of tiles (neighbouring tiles of one z chunk share halo rows and columns in that XCD's L2).
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int per = nb / 8, rem = nb % 8;
   const int x = b % 8, q = b / 8;
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
-template <bool IDX, int NC, int TY>
-__global__ void __launch_bounds__(64 * (TY + 2)) k_step_fused(const StepArgs a) {
-  constexpr int W = TY + 2;   // waves: halo row, TY own rows, halo row
-  constexpr int IP = NC * 64; // LDS row length
-  __shared__ float sE[3][W][IP];
-  __shared__ float sH[3][TY + 1][IP];
+template <bool IDX, int TY>
+__global__ void __launch_bounds__(64 * (TY + 3)) k_step_fused(const StepArgs *__restrict__ pa, int n) {
+  const StepArgs &a = *pa;
+  constexpr int RW = TY + 2;  // row waves (with the two halo rows)
+  constexpr int SX = 66;      // LDS columns ib-1 .. ib+64
+  __shared__ float sE[3][RW][SX];
+  __shared__ float sH[3][TY + 1][SX];
   __shared__ float sCoef[IDX ? 256 * 6 : 6];
 
-  const GridDev &g = a.g;
-  const int I = g.I, J = g.J, K = g.K;
-  const size_t IJ = (size_t)I * J;
+  const int I = a.I, J = a.J, K = a.K;
+  const size_t IJ = (size_t)I * J, IJK = a.IJK;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int ty = L % a.nTy, tz = L / a.nTy;
-  const int jb = ty * TY, kb = tz * a.KC, ke = min(kb + a.KC, K);
-  const int j = jb - 1 + w;
-  const bool row_ok = (j >= 0) && (j < J);
-  const bool h_row = row_ok && (w <= TY);
-  const bool own_row = row_ok && (w >= 1) && (w <= TY);
-  const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
-  const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
+  const int tx = L % a.nTx, t2 = L / a.nTx;
+  const int ty = t2 % a.nTy, tz = t2 / a.nTy;
+  const int ib = tx * 64, jb = ty * TY, kb = tz * a.KC, ke = min(kb + a.KC, K);
 
-  if (IDX) {
-    for (int q = threadIdx.x; q < a.co.ntab * 6; q += 64 * W) sCoef[q] = a.co.tab[q];
+  int i = -1000, r = 0;
+  if (w < RW) {
+    i = ib + lane;
+    r = w;
+  } else if (lane < RW) {
+    i = ib - 1;
+    r = lane;
+  } else if (lane >= 32 && lane < 32 + RW) {
+    i = ib + 64;
+    r = lane - 32;
   }
+  const int j = jb - 1 + r;
+  const int slot = i - ib + 1;
+  const bool has_slot = i != -1000;  // idle lanes of the halo wave own no LDS slot
+  const bool valid = has_slot && (i >= 0) && (i < I) && (j >= 0) && (j < J);
+  const bool doH = valid && (r <= TY) && (slot <= 64);
+  const bool own = valid && (r >= 1) && (r <= TY) && (slot >= 1) && (slot <= 64);
 
-  float ecx[NC], ecy[NC], ecz[NC];
-  float enx[NC], eny[NC], enz[NC];
-  float hpx[NC], hpy[NC];
-  float nhx[NC], nhy[NC], nhz[NC];
+  if (IDX)
+    for (int q = threadIdx.x; q < a.ntab * 6; q += 64 * (TY + 3)) sCoef[q] = a.coef[q];
+
+  const float *Ex = a.Fc, *Ey = a.Fc + IJK, *Ez = a.Fc + 2 * IJK;
+  const float *Hx = a.Fc + 3 * IJK, *Hy = a.Fc + 4 * IJK, *Hz = a.Fc + 5 * IJK;
+  const float *ikEx = a.prm + 8 * a.P, *ikEy = ikEx + I, *ikEz = ikEy + J;
+  const float *ikHx = ikEz + K, *ikHy = ikHx + I, *ikHz = ikHy + J;
+  const float kHx = doH ? ikHx[i] : 1.0f, kHy = doH ? ikHy[j] : 1.0f;
+  const float kEx = own ? ikEx[i] : 1.0f, kEy = own ? ikEy[j] : 1.0f;
 
   int k = kb > 0 ? kb - 1 : 0;
-  {
-    const size_t base = (size_t)j * I + (size_t)k * IJ;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int i = lane + 64 * c;
-      const bool ok = row_ok && i < I;
-      ecx[c] = ok ? a.cur.f[EX][base + i] : 0.0f;
-      ecy[c] = ok ? a.cur.f[EY][base + i] : 0.0f;
-      ecz[c] = ok ? a.cur.f[EZ][base + i] : 0.0f;
-      hpx[c] = 0.0f;
-      hpy[c] = 0.0f;
-    }
+  size_t id = valid ? (size_t)i + (size_t)j * I + (size_t)k * IJ : 0;
+  float ecx = 0, ecy = 0, ecz = 0, enx = 0, eny = 0, enz = 0, hox = 0, hoy = 0, hoz = 0;
+  float hpx = 0, hpy = 0;
+  int mt = 0;
+  if (valid) {
+    ecx = Ex[id]; ecy = Ey[id]; ecz = Ez[id];
+    if (k + 1 < K) { enx = Ex[id + IJ]; eny = Ey[id + IJ]; enz = Ez[id + IJ]; }
   }
+  if (doH) { hox = Hx[id]; hoy = Hy[id]; hoz = Hz[id]; }
+  if (IDX && own) mt = a.mat[id];
 
-  for (; k < ke; ++k) {
-    const bool kp1 = (k + 1 < K);
-    const size_t base = (size_t)j * I + (size_t)k * IJ;
-    // E(k+1) of this row (feeds the z differences of H(k) and next plane's LDS)
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int i = lane + 64 * c;
-      const bool ok = row_ok && kp1 && i < I;
-      enx[c] = ok ? a.cur.f[EX][base + IJ + i] : 0.0f;
-      eny[c] = ok ? a.cur.f[EY][base + IJ + i] : 0.0f;
-      enz[c] = ok ? a.cur.f[EZ][base + IJ + i] : 0.0f;
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int i = lane + 64 * c;
-      sE[0][w][i] = ecx[c];
-      sE[1][w][i] = ecy[c];
-      sE[2][w][i] = ecz[c];
+  for (; k < ke; ++k, id += IJ) {
+    // one plane ahead: E(k+2), H(k+1), material(k+1)
+    float fnx = 0, fny = 0, fnz = 0, gox = 0, goy = 0, goz = 0;
+    int mn = 0;
+    if (valid && k + 2 < K) { fnx = Ex[id + 2 * IJ]; fny = Ey[id + 2 * IJ]; fnz = Ez[id + 2 * IJ]; }
+    if (doH && k + 1 < ke) { gox = Hx[id + IJ]; goy = Hy[id + IJ]; goz = Hz[id + IJ]; }
+    if (IDX && own && k + 1 < ke) mn = a.mat[id + IJ];
+
+    if (has_slot) {
+      sE[0][r][slot] = ecx;
+      sE[1][r][slot] = ecy;
+      sE[2][r][slot] = ecz;
     }
     __syncthreads();
 
-    // ---- H(k) for rows jb-1 .. jb+TY-1 ----
-    const bool own_plane = (k >= kb);
-    if (h_row) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int i = lane + 64 * c;
-        if (i < I) {
-          const size_t id = base + i;
-          float hx = a.cur.f[HX][id], hy = a.cur.f[HY][id], hz = a.cur.f[HZ][id];
-          HNb e;
-          e.ex = ecx[c]; e.ey = ecy[c]; e.ez = ecz[c];
-          const bool jn = (j + 1 < J), in = (i + 1 < I);
-          e.ex_j1 = jn ? sE[0][w + 1][i] : 0.0f;
-          e.ez_j1 = jn ? sE[2][w + 1][i] : 0.0f;
-          e.ey_i1 = in ? sE[1][w][i + 1] : 0.0f;
-          e.ez_i1 = in ? sE[2][w][i + 1] : 0.0f;
-          e.ex_k1 = enx[c]; e.ey_k1 = eny[c];
-          const bool own = own_row && own_plane;
-          h_cell(g, a.co, a.pw, rowE, a.pcur, a.pnxt, own, i, j, k, IJ, e, hx, hy, hz);
-          sH[0][w][i] = hx;
-          sH[1][w][i] = hy;
-          sH[2][w][i] = hz;
-          if (own) {
-            a.nxt.f[HX][id] = hx;
-            a.nxt.f[HY][id] = hy;
-            a.nxt.f[HZ][id] = hz;
-          }
-          nhx[c] = hx; nhy[c] = hy; nhz[c] = hz;
-        } else {
-          nhx[c] = 0.0f; nhy[c] = 0.0f; nhz[c] = 0.0f;
-        }
+    const bool own_plane = k >= kb;
+    float hx = hox, hy = hoy, hz = hoz;
+    if (doH) {
+      HNb e;
+      e.ex = ecx; e.ey = ecy; e.ez = ecz;
+      const bool jn = j + 1 < J, in = i + 1 < I;
+      e.ex_j1 = jn ? sE[0][r + 1][slot] : 0.0f;
+      e.ez_j1 = jn ? sE[2][r + 1][slot] : 0.0f;
+      e.ey_i1 = in ? sE[1][r][slot + 1] : 0.0f;
+      e.ez_i1 = in ? sE[2][r][slot + 1] : 0.0f;
+      e.ex_k1 = enx; e.ey_k1 = eny;
+      // K1 GPU_updateH, src/gpu.h:592-601
+      if (i < I - 1 && j < J - 1 && k < K - 1) {
+        const float kz = ikHz[k];
+        hx = a.Chh * hx + a.Che * (-(e.ez_j1 - e.ez) * kHy + (e.ey_k1 - e.ey) * kz);
+        hy = a.Chh * hy + a.Che * (-(e.ex_k1 - e.ex) * kz + (e.ez_i1 - e.ez) * kHx);
+        hz = a.Chh * hz + a.Che * (-(e.ey_i1 - e.ey) * kHx + (e.ex_j1 - e.ex) * kHy);
+      }
+      if (a.act_h && i >= a.i0 - 1 && i <= a.i1 && j >= a.j0 - 1 && j <= a.j1 && k >= a.k0 - 1 && k <= a.k1) {
+        const F3 o = fused_pw_h(pa, n, i, j, k, F3{hx, hy, hz});
+        hx = o.x; hy = o.y; hz = o.z;
+      }
+      const int P = a.P;
+      if (i < P || i >= I - P || j < P || j >= J - P || k < P || k >= K - P) {
+        const D6 dd{e.ez_j1 - e.ez, e.ex_j1 - e.ex, e.ez_i1 - e.ez, e.ey_i1 - e.ey, e.ey_k1 - e.ey, e.ex_k1 - e.ex};
+        const F3 o = fused_pml_h(pa, own && own_plane, i, j, k, dd, F3{hx, hy, hz});
+        hx = o.x; hy = o.y; hz = o.z;
+      }
+      sH[0][r][slot] = hx;
+      sH[1][r][slot] = hy;
+      sH[2][r][slot] = hz;
+      if (own && own_plane) {
+        a.Fn[3 * IJK + id] = hx;
+        a.Fn[4 * IJK + id] = hy;
+        a.Fn[5 * IJK + id] = hz;
       }
     }
     __syncthreads();
 
-    // ---- E(k) for own rows ----
-    if (own_row && own_plane) {
+    if (own && own_plane) {
+      ENb h;
+      h.hx = hx; h.hy = hy; h.hz = hz;
+      h.hx_jm = sH[0][r - 1][slot];
+      h.hz_jm = sH[2][r - 1][slot];
+      h.hy_im = sH[1][r][slot - 1];
+      h.hz_im = sH[2][r][slot - 1];
+      h.hx_km = hpx; h.hy_km = hpy;
+      float ce[3], ch[3];
+      if (IDX) {
+        const float *row = sCoef + 6 * mt;
+        ce[0] = row[0]; ce[1] = row[1]; ce[2] = row[2];
+        const bool last = i == I - 1;
+        ch[0] = last ? 0.0f : row[3];
+        ch[1] = last ? 0.0f : row[4];
+        ch[2] = last ? 0.0f : row[5];
+      } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int i = lane + 64 * c;
-        if (i < I) {
-          const size_t id = base + i;
-          ENb h;
-          h.hx = nhx[c]; h.hy = nhy[c]; h.hz = nhz[c];
-          h.hx_jm = sH[0][w - 1][i];
-          h.hz_jm = sH[2][w - 1][i];
-          h.hy_im = i > 0 ? sH[1][w][i - 1] : 0.0f;
-          h.hz_im = i > 0 ? sH[2][w][i - 1] : 0.0f;
-          h.hx_km = hpx[c]; h.hy_km = hpy[c];
-          float ce[3], ch[3];
-          cell_coef<IDX>(a.co, sCoef, id, i, I, ce, ch);
-          float ex = ecx[c], ey = ecy[c], ez = ecz[c];
-          e_cell(g, a.co, a.pw, rowH, a.pcur, a.pnxt, i, j, k, IJ, ce, ch, h, ex, ey, ez);
-          a.nxt.f[EX][id] = ex;
-          a.nxt.f[EY][id] = ey;
-          a.nxt.f[EZ][id] = ez;
-        }
+        for (int c = 0; c < 3; ++c) { ce[c] = a.coef[c * IJK + id]; ch[c] = a.coef[(3 + c) * IJK + id]; }
       }
+      float ex = ecx, ey = ecy, ez = ecz;
+      // K3 GPU_updateE, src/gpu.h:745-762
+      const float kz = ikEz[k];
+      if (j != 0 && k != 0) ex = ce[0] * ex + ch[0] * ((h.hz - h.hz_jm) * kEy - (h.hy - h.hy_km) * kz);
+      if (i != 0 && k != 0) ey = ce[1] * ey + ch[1] * ((h.hx - h.hx_km) * kz - (h.hz - h.hz_im) * kEx);
+      if (i != 0 && j != 0) ez = ce[2] * ez + ch[2] * ((h.hy - h.hy_im) * kEx - (h.hx - h.hx_jm) * kEy);
+      if (a.act_e && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1) {
+        const F3 o = fused_pw_e(pa, n, F3{ch[0], ch[1], ch[2]}, i, j, k, F3{ex, ey, ez});
+        ex = o.x; ey = o.y; ez = o.z;
+      }
+      const int P = a.P;
+      if (i < P || i >= I - P || j < P || j >= J - P || k < P || k >= K - P) {
+        const D6 dd{h.hz - h.hz_jm, h.hx - h.hx_jm, h.hy - h.hy_km, h.hx - h.hx_km, h.hz - h.hz_im, h.hy - h.hy_im};
+        const F3 o = fused_pml_e(pa, i, j, k, F3{ch[0], ch[1], ch[2]}, dd, F3{ex, ey, ez});
+        ex = o.x; ey = o.y; ez = o.z;
+      }
+      a.Fn[id] = ex;
+      a.Fn[IJK + id] = ey;
+      a.Fn[2 * IJK + id] = ez;
     }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      hpx[c] = nhx[c]; hpy[c] = nhy[c];
-      ecx[c] = enx[c]; ecy[c] = eny[c]; ecz[c] = enz[c];
-    }
+    hpx = hx; hpy = hy;
+    ecx = enx; ecy = eny; ecz = enz;
+    enx = fnx; eny = fny; enz = fnz;
+    hox = gox; hoy = goy; hoz = goz;
+    mt = mn;
   }
 }
 
@@ -805,55 +1095,64 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
   return hipGetLastError();
 }
 
-bool fused_supported(const GridDev &g) { return g.I <= 512 && g.I >= 2 * g.P + 2; }
+bool fused_supported(const GridDev &g) { return g.I >= 2 * g.P + 2 && g.J >= 2 * g.P + 2 && g.K >= 2 * g.P + 2; }
 
-template <bool IDX, int NC, int TY>
-static hipError_t launch_fused_t(hipStream_t st, const StepArgs &a, int nwg) {
-  hipLaunchKernelGGL((k_step_fused<IDX, NC, TY>), dim3(nwg), dim3(64 * (TY + 2)), 0, st, a);
+template <bool IDX, int TY>
+static hipError_t launch_fused_t(hipStream_t st, const StepArgs *dargs, int nwg, int n) {
+  hipLaunchKernelGGL((k_step_fused<IDX, TY>), dim3(nwg), dim3(64 * (TY + 3)), 0, st, dargs, n);
   return hipGetLastError();
 }
 
-template <bool IDX, int TY>
-static hipError_t launch_fused_nc(hipStream_t st, const StepArgs &a, int nwg, int nc) {
-  switch (nc) {
-    case 1: return launch_fused_t<IDX, 1, TY>(st, a, nwg);
-    case 2: return launch_fused_t<IDX, 2, TY>(st, a, nwg);
-    case 3: return launch_fused_t<IDX, 3, TY>(st, a, nwg);
-    case 4: return launch_fused_t<IDX, 4, TY>(st, a, nwg);
-    case 5: return launch_fused_t<IDX, 5, TY>(st, a, nwg);
-    case 6: return launch_fused_t<IDX, 6, TY>(st, a, nwg);
-    case 7: return launch_fused_t<IDX, 7, TY>(st, a, nwg);
-    case 8: return launch_fused_t<IDX, 8, TY>(st, a, nwg);
-    default: return hipErrorInvalidValue;
-  }
-}
+size_t step_args_size() { return sizeof(StepArgs); }
 
-hipError_t launch_step_fused(hipStream_t st, const GridDev &g, const FieldSet &cur, const FieldSet &nxt,
-                             const PsiSet &pcur, const PsiSet &pnxt, const CoefDev &co, const PWDev &pw, int n,
-                             const StepTuning &tune) {
-  if (!fused_supported(g)) return hipErrorInvalidValue;
-  const int nc = (g.I + 63) / 64;
-  // LDS budget: rows x components x row length; wide rows use 4-row tiles.
-  int ty = tune.tile_rows == 4 ? 4 : 8;
-  if (nc > 5) ty = 4;
-  const int nTy = (g.J + ty - 1) / ty;
+// Fills the argument block of the fused kernel (host copy, to be uploaded once per state parity /
+// plane-wave mode); returns the number of workgroups, or -1 if the grid is not supported.
+int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
+  const GridDev &g = f.g;
+  if (!fused_supported(g)) return -1;
+  if (f.mat && f.ntab > 256) return -1;
+  const int ty = (tune.tile_rows == 4 || tune.tile_rows == 13) ? tune.tile_rows : 8;
+  const int nTx = (g.I + 63) / 64, nTy = (g.J + ty - 1) / ty;
   int kc = tune.z_chunk;
   if (kc <= 0) {
-    const int want = 1024;  // workgroups to launch (a few per CU)
-    int nkc = (want + nTy - 1) / nTy;
+    const int want = 2048;  // workgroups per launch (several per CU)
+    int nkc = (want + nTx * nTy - 1) / (nTx * nTy);
     if (nkc < 1) nkc = 1;
     kc = (g.K + nkc - 1) / nkc;
-    if (kc < 8) kc = 8;
+    if (kc < 16) kc = 16;
   }
   const int nKc = (g.K + kc - 1) / kc;
   StepArgs a;
-  a.g = g; a.cur = cur; a.nxt = nxt; a.pcur = pcur; a.pnxt = pnxt; a.co = co; a.pw = pw; a.n = n;
-  a.KC = kc; a.nTy = nTy;
-  const int nwg = nTy * nKc;
-  const bool idx = co.mat != nullptr;
-  if (idx && co.ntab > 256) return hipErrorInvalidValue;
-  if (idx) return ty == 4 ? launch_fused_nc<true, 4>(st, a, nwg, nc) : launch_fused_nc<true, 8>(st, a, nwg, nc);
-  return ty == 4 ? launch_fused_nc<false, 4>(st, a, nwg, nc) : launch_fused_nc<false, 8>(st, a, nwg, nc);
+  std::memset(&a, 0, sizeof(a));
+  a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
+  a.i0 = g.i0; a.i1 = g.i1; a.j0 = g.j0; a.j1 = g.j1; a.k0 = g.k0; a.k1 = g.k1;
+  a.nTx = nTx; a.nTy = nTy; a.KC = kc; a.ntab = f.ntab;
+  a.act_h = f.act_h; a.act_e = f.act_e; a.N1 = f.N1; a.pad = ty;
+  a.IJK = (size_t)g.I * g.J * g.K;
+  a.sx = (size_t)2 * g.P * g.J * g.K; a.sy = (size_t)2 * g.I * g.P * g.K; a.sz = (size_t)2 * g.I * g.J * g.P;
+  a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
+  a.mat = f.mat; a.coef = f.coef; a.prm = f.prm; a.pwtab = f.pwtab;
+  a.Eh = f.Eh; a.Hh = f.Hh;
+  a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
+  for (int c = 0; c < 3; ++c) { a.polE[c] = f.polE[c]; a.polH[c] = f.polH[c]; }
+  std::memcpy(out, &a, sizeof(a));
+  return nTx * nTy * nKc;
+}
+
+// Launches the fused step with an argument block already in device memory (dargs) whose host
+// copy is hargs.
+hipError_t launch_step_fused(hipStream_t st, const void *dargs, const void *hargs, int nwg, int n) {
+  const StepArgs &a = *(const StepArgs *)hargs;
+  const StepArgs *d = (const StepArgs *)dargs;
+  const int ty = a.pad;
+  if (a.mat) {
+    if (ty == 4) return launch_fused_t<true, 4>(st, d, nwg, n);
+    if (ty == 13) return launch_fused_t<true, 13>(st, d, nwg, n);
+    return launch_fused_t<true, 8>(st, d, nwg, n);
+  }
+  if (ty == 4) return launch_fused_t<false, 4>(st, d, nwg, n);
+  if (ty == 13) return launch_fused_t<false, 13>(st, d, nwg, n);
+  return launch_fused_t<false, 8>(st, d, nwg, n);
 }
 
 }  // namespace fdtd

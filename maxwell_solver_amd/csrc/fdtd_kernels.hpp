@@ -52,15 +52,32 @@ hipError_t launch_line(hipStream_t st, const LineDev &L);
 hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int nAnt, const int *comps, int ncomp,
                          float *port, int row, int T);
 
-// Fused step: reads (cur, pcur), writes (nxt, pnxt).  Returns hipErrorInvalidValue when the grid
-// is wider than the fused kernel supports (I > 512).
+// Fused step: reads the cur state, writes the nxt state (packed buffers).
 struct StepTuning {
-  int tile_rows = 8;  // 4 or 8
+  int tile_rows = 8;  // 4, 8 or 13 rows per tile
   int z_chunk = 0;    // 0: automatic
 };
-hipError_t launch_step_fused(hipStream_t st, const GridDev &g, const FieldSet &cur, const FieldSet &nxt,
-                             const PsiSet &pcur, const PsiSet &pnxt, const CoefDev &co, const PWDev &pw, int n,
-                             const StepTuning &tune);
+struct FusedDev {
+  GridDev g;
+  const float *Fc = nullptr;  // six fields, component c at c*I*J*K
+  float *Fn = nullptr;
+  const float *Pc = nullptr;  // psi groups: x slabs [Eyx Ezx Hyx Hzx], y [Exy Ezy Hxy Hzy], z [Exz Eyz Hxz Hyz]
+  float *Pn = nullptr;
+  const uint8_t *mat = nullptr;
+  const float *coef = nullptr;  // index mode: table [ntab][6]; full mode: [6][I*J*K]
+  int ntab = 0;
+  const float *prm = nullptr;   // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
+  const float *pwtab = nullptr; // incident-field projection tables (E then H; x,y,z; integer, half)
+  const float *Eh = nullptr, *Hh = nullptr;
+  int N1 = 0, act_h = 0, act_e = 0;
+  float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
+  float Chh = 1, Che = 0, dx = 0;
+};
+// Argument block of the fused kernel: built on the host (make_step_args returns the workgroup
+// count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.
+size_t step_args_size();
+int make_step_args(const FusedDev &f, const StepTuning &tune, void *out);
+hipError_t launch_step_fused(hipStream_t st, const void *dargs, const void *hargs, int nwg, int n);
 bool fused_supported(const GridDev &g);
 
 }  // namespace fdtd
