@@ -531,8 +531,8 @@ struct fdtd_sim {
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
     ArgSlot &slot = argslots[cur][slot_mode];
-    if (slot.gen != args_gen) {
-      FusedDev f;
+    FusedDev f;
+    {
       f.g = g;
       f.Fc = dFb[cur]; f.Fn = dFb[1 - cur]; f.Pc = dPsiB[cur]; f.Pn = dPsiB[1 - cur];
       f.mat = index_mode ? dMat : nullptr;
@@ -545,6 +545,8 @@ struct fdtd_sim {
         for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
       }
       f.Chh = Chh; f.Che = Che; f.dx = delX;
+    }
+    if (slot.gen != args_gen) {
       slot.host.assign(step_args_size(), 0);
       slot.nwg = make_step_args(f, tune, slot.host.data());
       if (slot.nwg < 0) return fail(FDTD_ERR_INVALID, "grid not supported by the fused kernel");
@@ -553,7 +555,7 @@ struct fdtd_sim {
       slot.gen = args_gen;
     }
     if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
-    HIPCHK(launch_step_fused(stream, slot.dptr, slot.host.data(), slot.nwg, n));
+    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n));
     if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
     cur = 1 - cur;
     return record_probes(cur);

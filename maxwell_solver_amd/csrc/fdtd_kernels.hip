@@ -557,18 +557,22 @@ __global__ void k_probes(FieldSet F, const int *idx, int nAnt, const int *comps,
 }
 
 // ------------------------------------------------------------------------------------------
-// fused step kernel
+// fused time step: three launches per step, reading the cur state and writing the nxt state
 //
-// Tiles of 64 columns x TY rows x KC planes.  Waves 0..TY+1 each own one grid row (row jb-1 is
-// the low halo row, jb+TY the high halo row); wave TY+2 covers the tile's halo columns ib-1 and
-// ib+64 for all rows.  Every thread thus stands for one cell (i, j) of a (TY+2) x 66 plane tile
-// and marches k up the chunk:
-//   load E(k+1), H(k) (one plane ahead), stage E(k) in LDS, barrier,
-//   H(k) at n+1/2 for rows jb-1..jb+TY-1 and columns ib-1..ib+63 -> LDS (own cells also -> HBM),
-//   barrier, E(k) at n+1 for the own cells -> HBM.
-// H of halo cells and of the halo plane kb-1 is recomputed from the old (cur) state, so tiles never
-// wait for each other; the new state goes to the other buffer (nxt).  Buffers are packed (one base
-// pointer per field set, psi set and parameter table) to keep the argument block in few SGPRs.
+//  A  k_slab_h   one thread per H cell of the CPML slabs S (the cells within P of a face): main
+//                update, TF/SF, CPML Y-X-Z with psi, in the reference order.
+//  B  k_interior the interior box C = [P, I-P) x [P, J-P) x [P, K-P), fused H and E.  Tiles of
+//                64 columns x TY rows x KC planes; waves 0..TY+1 own one row each (rows jb-1 and
+//                jb+TY are halo rows), wave TY+2 covers the halo columns ib-1 and ib+64.  Each
+//                thread stands for one cell (i, j) and marches k up the chunk: E(k) staged in LDS,
+//                barrier, H(k) at n+1/2 for rows jb-1..jb+TY-1 and columns ib-1..ib+63 into LDS
+//                (own cells also to HBM), barrier, E(k) at n+1 for own cells.  H of halo cells in C
+//                is recomputed from the old state; halo cells in S take the new H written by A.
+//                No CPML code in this kernel: its loop only streams fields, the material byte and
+//                the next plane's prefetch.
+//  C  k_slab_e   one thread per E cell of S: main update, TF/SF, CPML Y-Z-X with psi, reading the
+//                new H of A and B.
+// All three apply per cell exactly the operations of the per-op kernels, in the same order.
 // ------------------------------------------------------------------------------------------
 struct StepArgs {
   int I, J, K, P;
@@ -589,29 +593,54 @@ struct StepArgs {
   float polE[3], polH[3];
 };
 
+// Pointers taken from the argument block are generic; casting them to the global address space
+// lets the compiler emit global_load/store (counted by vmcnt only) instead of flat accesses,
+// which also count against lgkmcnt and would drain the prefetched loads at every LDS wait.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
+// Workgroup barrier ordering LDS only: global loads in flight (the next plane's prefetch) are not
+// waited for, unlike __syncthreads(), whose fence also covers global memory.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // psi groups of one set: x slabs [Eyx Ezx Hyx Hzx], y slabs [Exy Ezy Hxy Hzy], z slabs [Exz Eyz Hxz Hyz]
 struct PsiView {
-  const float *c;
-  float *n;
+  __attribute__((address_space(1))) const float *c;
+  __attribute__((address_space(1))) float *n;
   size_t sx, sy, sz;
   __device__ __forceinline__ size_t off_x(int q) const { return (size_t)q * sx; }
   __device__ __forceinline__ size_t off_y(int q) const { return 4 * sx + (size_t)q * sy; }
   __device__ __forceinline__ size_t off_z(int q) const { return 4 * sx + 4 * sy + (size_t)q * sz; }
 };
 
+__device__ __forceinline__ float line_at_g(const float *rowp, float s) {
+  const auto row = gptr(rowp);
+  const float fl = floorf(s);
+  const int m = (int)fl;
+  const float w = s - fl;
+  if (w == 0.0f) return row[m];
+  return row[m] + w * (row[m + 1] - row[m]);
+}
+
 __device__ __forceinline__ float tab_e(const StepArgs &a, const float *row, int c, int x, int y, int z) {
-  const float *t = a.pwtab;
+  const auto t = gptr(a.pwtab);
   const float sx = t[(c == 0 ? a.I : 0) + x];
   const float sy = t[2 * a.I + (c == 1 ? a.J : 0) + y];
   const float sz = t[2 * a.I + 2 * a.J + (c == 2 ? a.K : 0) + z];
-  return a.polE[c] * line_at(row, (sx + sy) + sz);
+  return a.polE[c] * line_at_g(row, (sx + sy) + sz);
 }
 __device__ __forceinline__ float tab_h(const StepArgs &a, const float *row, int c, int x, int y, int z) {
-  const float *t = a.pwtab + 2 * (a.I + a.J + a.K);
+  const auto t = gptr(a.pwtab + 2 * (a.I + a.J + a.K));
   const float sx = t[(c != 0 ? a.I : 0) + x];
   const float sy = t[2 * a.I + (c != 1 ? a.J : 0) + y];
   const float sz = t[2 * a.I + 2 * a.J + (c != 2 ? a.K : 0) + z];
-  return a.polH[c] * line_at(row, (sx + sy) + sz);
+  return a.polH[c] * line_at_g(row, (sx + sy) + sz);
 }
 
 struct F3 {
@@ -710,129 +739,6 @@ __device__ __noinline__ F3 fused_pw_e(const StepArgs *__restrict__ pa, int n, F3
   return F3{ex, ey, ez};
 }
 
-// CPML of an H cell: slabs Y, X, Z in the order of src/simulation.h:392-403.  Out of line: only
-// slab cells take this path.  d = {dEz/dy, dEx/dy, dEz/dx, dEy/dx, dEy/dz, dEx/dz} differences.
-struct D6 {
-  float a, b, c, d, e, f;
-};
-__device__ __noinline__ F3 fused_pml_h(const StepArgs *__restrict__ pa, bool own, int i, int j, int k, D6 dd, F3 hv) {
-  const StepArgs &a = *pa;
-  float hx = hv.x, hy = hv.y, hz = hv.z;
-  const int I = a.I, J = a.J, K = a.K, P = a.P;
-  const float Che = a.Che, dx = a.dx;
-  const float *bH = a.prm + 4 * P, *cH = a.prm + 6 * P;
-  const PsiView ps{a.Pc, a.Pn, a.sx, a.sy, a.sz};
-  const size_t IJ = (size_t)I * J;
-  if (j < P || j >= J - P) {
-    const int s = j < P ? j : j - (J - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
-    const size_t oxy = ps.off_y(2) + t, ozy = ps.off_y(3) + t;
-    if (s < 2 * P - 1 && i < I - 1 && k < K - 1) {
-      const float pxy = bH[s] * ps.c[oxy] + cH[s] * dd.a / dx;
-      hx = hx - Che * pxy * dx;
-      const float pzy = bH[s] * ps.c[ozy] + cH[s] * dd.b / dx;
-      hz = hz + Che * pzy * dx;
-      if (own) { ps.n[oxy] = pxy; ps.n[ozy] = pzy; }
-    } else if (own) {
-      ps.n[oxy] = ps.c[oxy];
-      ps.n[ozy] = ps.c[ozy];
-    }
-  }
-  if (i < P || i >= I - P) {
-    const int s = i < P ? i : i - (I - 2 * P);
-    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
-    const size_t oyx = ps.off_x(2) + t, ozx = ps.off_x(3) + t;
-    if (s < 2 * P - 1 && j < J - 1 && k < K - 1) {
-      const float pyx = bH[s] * ps.c[oyx] + cH[s] * dd.c / dx;
-      hy = hy + Che * pyx * dx;
-      const float pzx = bH[s] * ps.c[ozx] + cH[s] * dd.d / dx;
-      hz = hz - Che * pzx * dx;
-      if (own) { ps.n[oyx] = pyx; ps.n[ozx] = pzx; }
-    } else if (own) {
-      ps.n[oyx] = ps.c[oyx];
-      ps.n[ozx] = ps.c[ozx];
-    }
-  }
-  if (k < P || k >= K - P) {
-    const int s = k < P ? k : k - (K - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
-    const size_t oxz = ps.off_z(2) + t, oyz = ps.off_z(3) + t;
-    if (s < 2 * P - 1 && i < I - 1 && j < J - 1) {
-      const float pxz = bH[s] * ps.c[oxz] + cH[s] * dd.e / dx;
-      hx = hx + Che * pxz * dx;
-      const float pyz = bH[s] * ps.c[oyz] + cH[s] * dd.f / dx;
-      hy = hy - Che * pyz * dx;
-      if (own) { ps.n[oxz] = pxz; ps.n[oyz] = pyz; }
-    } else if (own) {
-      ps.n[oxz] = ps.c[oxz];
-      ps.n[oyz] = ps.c[oyz];
-    }
-  }
-  return F3{hx, hy, hz};
-}
-
-// CPML of an E cell: slabs Y, Z, X in the order of src/simulation.h:404-418.
-// d = {dHz/dy, dHx/dy, dHy/dz, dHx/dz, dHz/dx, dHy/dx} backward differences.
-__device__ __noinline__ F3 fused_pml_e(const StepArgs *__restrict__ pa, int i, int j, int k, F3 chv, D6 dd, F3 ev) {
-  const StepArgs &a = *pa;
-  const float ch[3] = {chv.x, chv.y, chv.z};
-  float ex = ev.x, ey = ev.y, ez = ev.z;
-  const int I = a.I, J = a.J, K = a.K, P = a.P;
-  const float dx = a.dx;
-  const float *bE = a.prm, *cE = a.prm + 2 * P;
-  const PsiView ps{a.Pc, a.Pn, a.sx, a.sy, a.sz};
-  const size_t IJ = (size_t)I * J;
-  if (j < P || j >= J - P) {
-    const int s = j < P ? j : j - (J - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
-    const size_t oxy = ps.off_y(0) + t, ozy = ps.off_y(1) + t;
-    if (s >= 1 && i >= 1 && i < I - 1 && k >= 1 && k < K - 1) {
-      const float pxy = bE[s] * ps.c[oxy] + cE[s] * dd.a / dx;
-      ex = ex + ch[0] * pxy * dx;
-      const float pzy = bE[s] * ps.c[ozy] + cE[s] * dd.b / dx;
-      ez = ez - ch[2] * pzy * dx;
-      ps.n[oxy] = pxy;
-      ps.n[ozy] = pzy;
-    } else {
-      ps.n[oxy] = ps.c[oxy];
-      ps.n[ozy] = ps.c[ozy];
-    }
-  }
-  if (k < P || k >= K - P) {
-    const int s = k < P ? k : k - (K - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
-    const size_t oxz = ps.off_z(0) + t, oyz = ps.off_z(1) + t;
-    if (s >= 1 && i >= 1 && i < I - 1 && j >= 1 && j < J - 1) {
-      const float pxz = bE[s] * ps.c[oxz] + cE[s] * dd.c / dx;
-      ex = ex - ch[0] * pxz * dx;
-      const float pyz = bE[s] * ps.c[oyz] + cE[s] * dd.d / dx;
-      ey = ey + ch[1] * pyz * dx;
-      ps.n[oxz] = pxz;
-      ps.n[oyz] = pyz;
-    } else {
-      ps.n[oxz] = ps.c[oxz];
-      ps.n[oyz] = ps.c[oyz];
-    }
-  }
-  if (i < P || i >= I - P) {
-    const int s = i < P ? i : i - (I - 2 * P);
-    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
-    const size_t oyx = ps.off_x(0) + t, ozx = ps.off_x(1) + t;
-    if (s >= 1 && j >= 1 && j < J - 1 && k >= 1 && k < K - 1) {
-      const float pyx = bE[s] * ps.c[oyx] + cE[s] * dd.e / dx;
-      ey = ey - ch[1] * pyx * dx;
-      const float pzx = bE[s] * ps.c[ozx] + cE[s] * dd.f / dx;
-      ez = ez + ch[2] * pzx * dx;
-      ps.n[oyx] = pyx;
-      ps.n[ozx] = pzx;
-    } else {
-      ps.n[oyx] = ps.c[oyx];
-      ps.n[ozx] = ps.c[ozx];
-    }
-  }
-  return F3{ex, ey, ez};
-}
-
 // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range
 // of tiles (neighbouring tiles of one z chunk share halo rows and columns in that XCD's L2).
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -841,22 +747,33 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
+
 template <bool IDX, int TY>
-__global__ void __launch_bounds__(64 * (TY + 3)) k_step_fused(const StepArgs *__restrict__ pa, int n) {
-  const StepArgs &a = *pa;
+__global__ void __launch_bounds__(64 * (TY + 3)) k_interior(const StepArgs *__restrict__ pa, int n) {
+  // Every field the loop needs is read once here (scalar loads, before any store).
+  const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
+  const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
+  const int nTx = pa->nTx, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab, act_h = pa->act_h, act_e = pa->act_e;
+  const size_t IJK = pa->IJK;
+  const auto Fc = gptr(pa->Fc);
+  const auto Fn = gptr(pa->Fn);
+  const auto mat = gptr(pa->mat);
+  const auto coef = gptr(pa->coef);
+  const auto prm = gptr(pa->prm);
+  const float Chh = pa->Chh, Che = pa->Che;
   constexpr int RW = TY + 2;  // row waves (with the two halo rows)
   constexpr int SX = 66;      // LDS columns ib-1 .. ib+64
   __shared__ float sE[3][RW][SX];
   __shared__ float sH[3][TY + 1][SX];
   __shared__ float sCoef[IDX ? 256 * 6 : 6];
 
-  const int I = a.I, J = a.J, K = a.K;
-  const size_t IJ = (size_t)I * J, IJK = a.IJK;
+  const size_t IJ = (size_t)I * J;
+  const int cx1 = I - P, cy1 = J - P, cz1 = K - P;  // C = [P, cx1) x [P, cy1) x [P, cz1)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int tx = L % a.nTx, t2 = L / a.nTx;
-  const int ty = t2 % a.nTy, tz = t2 / a.nTy;
-  const int ib = tx * 64, jb = ty * TY, kb = tz * a.KC, ke = min(kb + a.KC, K);
+  const int tx = L % nTx, t2 = L / nTx;
+  const int ty = t2 % nTy, tz = t2 / nTy;
+  const int ib = P + tx * 64, jb = P + ty * TY, kb = P + tz * KC, ke = min(kb + KC, cz1);
 
   int i = -1000, r = 0;
   if (w < RW) {
@@ -873,131 +790,220 @@ __global__ void __launch_bounds__(64 * (TY + 3)) k_step_fused(const StepArgs *__
   const int slot = i - ib + 1;
   const bool has_slot = i != -1000;  // idle lanes of the halo wave own no LDS slot
   const bool valid = has_slot && (i >= 0) && (i < I) && (j >= 0) && (j < J);
-  const bool doH = valid && (r <= TY) && (slot <= 64);
-  const bool own = valid && (r >= 1) && (r <= TY) && (slot >= 1) && (slot <= 64);
+  const bool inC = (i >= P) && (i < cx1) && (j >= P) && (j < cy1);
+  const bool needH = valid && (r <= TY) && (slot <= 64);
+  const bool doH = needH && inC;    // H recomputed here
+  const bool getH = needH && !inC;  // H of a slab cell: already new in HBM (kernel A)
+  const bool own = valid && inC && (r >= 1) && (r <= TY) && (slot >= 1) && (slot <= 64);
 
   if (IDX)
-    for (int q = threadIdx.x; q < a.ntab * 6; q += 64 * (TY + 3)) sCoef[q] = a.coef[q];
+    for (int q = threadIdx.x; q < ntab * 6; q += 64 * (TY + 3)) sCoef[q] = coef[q];
 
-  const float *Ex = a.Fc, *Ey = a.Fc + IJK, *Ez = a.Fc + 2 * IJK;
-  const float *Hx = a.Fc + 3 * IJK, *Hy = a.Fc + 4 * IJK, *Hz = a.Fc + 5 * IJK;
-  const float *ikEx = a.prm + 8 * a.P, *ikEy = ikEx + I, *ikEz = ikEy + J;
-  const float *ikHx = ikEz + K, *ikHy = ikHx + I, *ikHz = ikHy + J;
+  const auto Ex = Fc, Ey = Fc + IJK, Ez = Fc + 2 * IJK;
+  const auto Hx = Fc + 3 * IJK, Hy = Fc + 4 * IJK, Hz = Fc + 5 * IJK;
+  const auto Nx = Fn + 3 * IJK, Ny = Fn + 4 * IJK, Nz = Fn + 5 * IJK;
+  const auto ikEx = prm + 8 * P, ikEy = ikEx + I, ikEz = ikEy + J;
+  const auto ikHx = ikEz + K, ikHy = ikHx + I, ikHz = ikHy + J;
   const float kHx = doH ? ikHx[i] : 1.0f, kHy = doH ? ikHy[j] : 1.0f;
   const float kEx = own ? ikEx[i] : 1.0f, kEy = own ? ikEy[j] : 1.0f;
+  const size_t col = valid ? (size_t)i + (size_t)j * I : 0;
 
-  int k = kb > 0 ? kb - 1 : 0;
-  size_t id = valid ? (size_t)i + (size_t)j * I + (size_t)k * IJ : 0;
-  float ecx = 0, ecy = 0, ecz = 0, enx = 0, eny = 0, enz = 0, hox = 0, hoy = 0, hoz = 0;
-  float hpx = 0, hpy = 0;
-  int mt = 0;
-  if (valid) {
-    ecx = Ex[id]; ecy = Ey[id]; ecz = Ez[id];
-    if (k + 1 < K) { enx = Ex[id + IJ]; eny = Ey[id + IJ]; enz = Ez[id + IJ]; }
+  // plane kb-1: recomputed when it lies in C, else read from the new state (slab)
+  int k = kb > P ? kb - 1 : kb;
+  float hpx = 0.0f, hpy = 0.0f;
+  if (own && kb == P && kb > 0) {
+    hpx = Nx[col + (size_t)(kb - 1) * IJ];
+    hpy = Ny[col + (size_t)(kb - 1) * IJ];
   }
-  if (doH) { hox = Hx[id]; hoy = Hy[id]; hoz = Hz[id]; }
-  if (IDX && own) mt = a.mat[id];
-
-  for (; k < ke; ++k, id += IJ) {
-    // one plane ahead: E(k+2), H(k+1), material(k+1)
-    float fnx = 0, fny = 0, fnz = 0, gox = 0, goy = 0, goz = 0;
-    int mn = 0;
-    if (valid && k + 2 < K) { fnx = Ex[id + 2 * IJ]; fny = Ey[id + 2 * IJ]; fnz = Ez[id + 2 * IJ]; }
-    if (doH && k + 1 < ke) { gox = Hx[id + IJ]; goy = Hy[id + IJ]; goz = Hz[id + IJ]; }
-    if (IDX && own && k + 1 < ke) mn = a.mat[id + IJ];
-
-    if (has_slot) {
-      sE[0][r][slot] = ecx;
-      sE[1][r][slot] = ecy;
-      sE[2][r][slot] = ecz;
+  // register planes: E(k), E(k+1); H source (old for doH lanes, new for getH lanes) at k; material
+  float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, ha[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
+  int ma = 0, mb = 0;
+  {
+    const size_t id = col + (size_t)k * IJ;
+    if (valid) {
+      ea[0] = Ex[id]; ea[1] = Ey[id]; ea[2] = Ez[id];
+      if (k + 1 < K) { eb[0] = Ex[id + IJ]; eb[1] = Ey[id + IJ]; eb[2] = Ez[id + IJ]; }
     }
-    __syncthreads();
+    if (doH) { ha[0] = Hx[id]; ha[1] = Hy[id]; ha[2] = Hz[id]; }
+    if (getH) { ha[0] = Nx[id]; ha[1] = Ny[id]; ha[2] = Nz[id]; }
+    if (IDX && own) ma = mat[id];
+  }
+
+  // One plane: ec holds E(k) on entry and receives E(k+2); en holds E(k+1); hs holds the H source
+  // of plane k, hn receives plane k+1's; mc / mn likewise for the material byte.
+  auto plane = [&](float (&ec)[3], float (&en)[3], float (&hs)[3], float (&hn)[3], int &mc, int &mn) {
+    const size_t id = col + (size_t)k * IJ;
+    if (has_slot) {
+      sE[0][r][slot] = ec[0];
+      sE[1][r][slot] = ec[1];
+      sE[2][r][slot] = ec[2];
+    }
+    if (valid && k + 2 < K) { ec[0] = Ex[id + 2 * IJ]; ec[1] = Ey[id + 2 * IJ]; ec[2] = Ez[id + 2 * IJ]; }
+    if (k + 1 < ke) {
+      if (doH) { hn[0] = Hx[id + IJ]; hn[1] = Hy[id + IJ]; hn[2] = Hz[id + IJ]; }
+      if (getH) { hn[0] = Nx[id + IJ]; hn[1] = Ny[id + IJ]; hn[2] = Nz[id + IJ]; }
+      if (IDX && own) mn = mat[id + IJ];
+    }
+    lds_barrier();
 
     const bool own_plane = k >= kb;
-    float hx = hox, hy = hoy, hz = hoz;
+    float hx = hs[0], hy = hs[1], hz = hs[2];
     if (doH) {
-      HNb e;
-      e.ex = ecx; e.ey = ecy; e.ez = ecz;
-      const bool jn = j + 1 < J, in = i + 1 < I;
-      e.ex_j1 = jn ? sE[0][r + 1][slot] : 0.0f;
-      e.ez_j1 = jn ? sE[2][r + 1][slot] : 0.0f;
-      e.ey_i1 = in ? sE[1][r][slot + 1] : 0.0f;
-      e.ez_i1 = in ? sE[2][r][slot + 1] : 0.0f;
-      e.ex_k1 = enx; e.ey_k1 = eny;
-      // K1 GPU_updateH, src/gpu.h:592-601
-      if (i < I - 1 && j < J - 1 && k < K - 1) {
-        const float kz = ikHz[k];
-        hx = a.Chh * hx + a.Che * (-(e.ez_j1 - e.ez) * kHy + (e.ey_k1 - e.ey) * kz);
-        hy = a.Chh * hy + a.Che * (-(e.ex_k1 - e.ex) * kz + (e.ez_i1 - e.ez) * kHx);
-        hz = a.Chh * hz + a.Che * (-(e.ey_i1 - e.ey) * kHx + (e.ex_j1 - e.ex) * kHy);
-      }
-      if (a.act_h && i >= a.i0 - 1 && i <= a.i1 && j >= a.j0 - 1 && j <= a.j1 && k >= a.k0 - 1 && k <= a.k1) {
+      const float ex = sE[0][r][slot], ey = sE[1][r][slot], ez = sE[2][r][slot];
+      const float ex_j1 = sE[0][r + 1][slot], ez_j1 = sE[2][r + 1][slot];
+      const float ey_i1 = sE[1][r][slot + 1], ez_i1 = sE[2][r][slot + 1];
+      // K1 GPU_updateH, src/gpu.h:592-601 (C cells satisfy its mask)
+      const float kz = ikHz[k];
+      hx = Chh * hx + Che * (-(ez_j1 - ez) * kHy + (en[1] - ey) * kz);
+      hy = Chh * hy + Che * (-(en[0] - ex) * kz + (ez_i1 - ez) * kHx);
+      hz = Chh * hz + Che * (-(ey_i1 - ey) * kHx + (ex_j1 - ex) * kHy);
+      if (act_h && i >= bi0 - 1 && i <= bi1 && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1) {
         const F3 o = fused_pw_h(pa, n, i, j, k, F3{hx, hy, hz});
         hx = o.x; hy = o.y; hz = o.z;
       }
-      const int P = a.P;
-      if (i < P || i >= I - P || j < P || j >= J - P || k < P || k >= K - P) {
-        const D6 dd{e.ez_j1 - e.ez, e.ex_j1 - e.ex, e.ez_i1 - e.ez, e.ey_i1 - e.ey, e.ey_k1 - e.ey, e.ex_k1 - e.ex};
-        const F3 o = fused_pml_h(pa, own && own_plane, i, j, k, dd, F3{hx, hy, hz});
-        hx = o.x; hy = o.y; hz = o.z;
-      }
+    }
+    if (needH) {
       sH[0][r][slot] = hx;
       sH[1][r][slot] = hy;
       sH[2][r][slot] = hz;
-      if (own && own_plane) {
-        a.Fn[3 * IJK + id] = hx;
-        a.Fn[4 * IJK + id] = hy;
-        a.Fn[5 * IJK + id] = hz;
-      }
     }
-    __syncthreads();
+    if (own && own_plane) {
+      Nx[id] = hx;
+      Ny[id] = hy;
+      Nz[id] = hz;
+    }
+    lds_barrier();
 
     if (own && own_plane) {
-      ENb h;
-      h.hx = hx; h.hy = hy; h.hz = hz;
-      h.hx_jm = sH[0][r - 1][slot];
-      h.hz_jm = sH[2][r - 1][slot];
-      h.hy_im = sH[1][r][slot - 1];
-      h.hz_im = sH[2][r][slot - 1];
-      h.hx_km = hpx; h.hy_km = hpy;
+      const float hx_jm = sH[0][r - 1][slot], hz_jm = sH[2][r - 1][slot];
+      const float hy_im = sH[1][r][slot - 1], hz_im = sH[2][r][slot - 1];
       float ce[3], ch[3];
       if (IDX) {
-        const float *row = sCoef + 6 * mt;
+        const float *row = sCoef + 6 * mc;
         ce[0] = row[0]; ce[1] = row[1]; ce[2] = row[2];
-        const bool last = i == I - 1;
-        ch[0] = last ? 0.0f : row[3];
-        ch[1] = last ? 0.0f : row[4];
-        ch[2] = last ? 0.0f : row[5];
+        ch[0] = row[3]; ch[1] = row[4]; ch[2] = row[5];  // C cells are never on the last x plane
       } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { ce[c] = a.coef[c * IJK + id]; ch[c] = a.coef[(3 + c) * IJK + id]; }
+        for (int c = 0; c < 3; ++c) { ce[c] = coef[c * IJK + id]; ch[c] = coef[(3 + c) * IJK + id]; }
       }
-      float ex = ecx, ey = ecy, ez = ecz;
-      // K3 GPU_updateE, src/gpu.h:745-762
+      float ex = sE[0][r][slot], ey = sE[1][r][slot], ez = sE[2][r][slot];
+      // K3 GPU_updateE, src/gpu.h:745-762 (C cells satisfy its masks)
       const float kz = ikEz[k];
-      if (j != 0 && k != 0) ex = ce[0] * ex + ch[0] * ((h.hz - h.hz_jm) * kEy - (h.hy - h.hy_km) * kz);
-      if (i != 0 && k != 0) ey = ce[1] * ey + ch[1] * ((h.hx - h.hx_km) * kz - (h.hz - h.hz_im) * kEx);
-      if (i != 0 && j != 0) ez = ce[2] * ez + ch[2] * ((h.hy - h.hy_im) * kEx - (h.hx - h.hx_jm) * kEy);
-      if (a.act_e && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1) {
+      ex = ce[0] * ex + ch[0] * ((hz - hz_jm) * kEy - (hy - hpy) * kz);
+      ey = ce[1] * ey + ch[1] * ((hx - hpx) * kz - (hz - hz_im) * kEx);
+      ez = ce[2] * ez + ch[2] * ((hy - hy_im) * kEx - (hx - hx_jm) * kEy);
+      if (act_e && i >= bi0 && i <= bi1 && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1) {
         const F3 o = fused_pw_e(pa, n, F3{ch[0], ch[1], ch[2]}, i, j, k, F3{ex, ey, ez});
         ex = o.x; ey = o.y; ez = o.z;
       }
-      const int P = a.P;
-      if (i < P || i >= I - P || j < P || j >= J - P || k < P || k >= K - P) {
-        const D6 dd{h.hz - h.hz_jm, h.hx - h.hx_jm, h.hy - h.hy_km, h.hx - h.hx_km, h.hz - h.hz_im, h.hy - h.hy_im};
-        const F3 o = fused_pml_e(pa, i, j, k, F3{ch[0], ch[1], ch[2]}, dd, F3{ex, ey, ez});
-        ex = o.x; ey = o.y; ez = o.z;
-      }
-      a.Fn[id] = ex;
-      a.Fn[IJK + id] = ey;
-      a.Fn[2 * IJK + id] = ez;
+      Fn[id] = ex;
+      Fn[IJK + id] = ey;
+      Fn[2 * IJK + id] = ez;
     }
-    hpx = hx; hpy = hy;
-    ecx = enx; ecy = eny; ecz = enz;
-    enx = fnx; eny = fny; enz = fnz;
-    hox = gox; hoy = goy; hoz = goz;
-    mt = mn;
+    hpx = hx;
+    hpy = hy;
+    ++k;
+  };
+  // two planes per trip with swapped register roles: no copies of in-flight loads
+  while (k < ke) {
+    plane(ea, eb, ha, hb, ma, mb);
+    if (k >= ke) break;
+    plane(eb, ea, hb, ha, mb, ma);
   }
+}
+
+// Decode thread t of the slab kernels into a cell of S (z slabs, then y slabs of the remaining
+// planes, then x slabs of the remaining rows).
+__device__ __forceinline__ bool slab_cell(long t, int I, int J, int K, int P, int &i, int &j, int &k) {
+  const long nz = 2L * P * I * J, ny = 2L * P * I * (K - 2 * P), nx = 2L * P * (J - 2 * P) * (K - 2 * P);
+  if (t < nz) {
+    const long pl = t / ((long)I * J), rem = t % ((long)I * J);
+    k = pl < P ? (int)pl : K - 2 * P + (int)pl;
+    j = (int)(rem / I);
+    i = (int)(rem % I);
+    return true;
+  }
+  t -= nz;
+  if (t < ny) {
+    const long q = t / I;
+    i = (int)(t % I);
+    const int js = (int)(q % (2 * P));
+    k = P + (int)(q / (2 * P));
+    j = js < P ? js : J - 2 * P + js;
+    return true;
+  }
+  t -= ny;
+  if (t < nx) {
+    const long q = t / (2 * P);
+    const int is = (int)(t % (2 * P));
+    i = is < P ? is : I - 2 * P + is;
+    j = P + (int)(q % (J - 2 * P));
+    k = P + (int)(q / (J - 2 * P));
+    return true;
+  }
+  return false;
+}
+
+struct SlabArgs {
+  GridDev g;
+  FieldSet cur, nxt;
+  PsiSet pcur, pnxt;
+  CoefDev co;
+  PWDev pw;
+  int n;
+};
+
+// A: H cells of the CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
+__global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
+  int i, j, k;
+  if (!slab_cell((long)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  const int I = a.g.I, J = a.g.J, K = a.g.K;
+  const size_t IJ = (size_t)I * J;
+  const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
+  const float *Ex = a.cur.f[EX], *Ey = a.cur.f[EY], *Ez = a.cur.f[EZ];
+  HNb e;
+  e.ex = Ex[id]; e.ey = Ey[id]; e.ez = Ez[id];
+  const bool in = i + 1 < I, jn = j + 1 < J, kn = k + 1 < K;
+  e.ey_i1 = in ? Ey[id + 1] : 0.0f;
+  e.ez_i1 = in ? Ez[id + 1] : 0.0f;
+  e.ex_j1 = jn ? Ex[id + I] : 0.0f;
+  e.ez_j1 = jn ? Ez[id + I] : 0.0f;
+  e.ex_k1 = kn ? Ex[id + IJ] : 0.0f;
+  e.ey_k1 = kn ? Ey[id + IJ] : 0.0f;
+  float hx = a.cur.f[HX][id], hy = a.cur.f[HY][id], hz = a.cur.f[HZ][id];
+  const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
+  h_cell(a.g, a.co, a.pw, rowE, a.pcur, a.pnxt, true, i, j, k, IJ, e, hx, hy, hz);
+  a.nxt.f[HX][id] = hx;
+  a.nxt.f[HY][id] = hy;
+  a.nxt.f[HZ][id] = hz;
+}
+
+// C: E cells of the CPML slabs (src/gpu.h:727-763 + plane wave + src/gpu.h:765-868), reading the
+// new H of kernels A and B.
+template <bool IDX>
+__global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
+  int i, j, k;
+  if (!slab_cell((long)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  const int I = a.g.I, J = a.g.J;
+  const size_t IJ = (size_t)I * J;
+  const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
+  const float *Hx = a.nxt.f[HX], *Hy = a.nxt.f[HY], *Hz = a.nxt.f[HZ];
+  ENb h;
+  h.hx = Hx[id]; h.hy = Hy[id]; h.hz = Hz[id];
+  h.hy_im = i > 0 ? Hy[id - 1] : 0.0f;
+  h.hz_im = i > 0 ? Hz[id - 1] : 0.0f;
+  h.hx_jm = j > 0 ? Hx[id - I] : 0.0f;
+  h.hz_jm = j > 0 ? Hz[id - I] : 0.0f;
+  h.hx_km = k > 0 ? Hx[id - IJ] : 0.0f;
+  h.hy_km = k > 0 ? Hy[id - IJ] : 0.0f;
+  float ce[3], ch[3];
+  cell_coef<IDX>(a.co, a.co.tab, id, i, I, ce, ch);
+  float ex = a.cur.f[EX][id], ey = a.cur.f[EY][id], ez = a.cur.f[EZ][id];
+  const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
+  e_cell(a.g, a.co, a.pw, rowH, a.pcur, a.pnxt, i, j, k, IJ, ce, ch, h, ex, ey, ez);
+  a.nxt.f[EX][id] = ex;
+  a.nxt.f[EY][id] = ey;
+  a.nxt.f[EZ][id] = ez;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1098,30 +1104,31 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
 bool fused_supported(const GridDev &g) { return g.I >= 2 * g.P + 2 && g.J >= 2 * g.P + 2 && g.K >= 2 * g.P + 2; }
 
 template <bool IDX, int TY>
-static hipError_t launch_fused_t(hipStream_t st, const StepArgs *dargs, int nwg, int n) {
-  hipLaunchKernelGGL((k_step_fused<IDX, TY>), dim3(nwg), dim3(64 * (TY + 3)), 0, st, dargs, n);
+static hipError_t launch_interior_t(hipStream_t st, const StepArgs *dargs, int nwg, int n) {
+  hipLaunchKernelGGL((k_interior<IDX, TY>), dim3(nwg), dim3(64 * (TY + 3)), 0, st, dargs, n);
   return hipGetLastError();
 }
 
 size_t step_args_size() { return sizeof(StepArgs); }
 
-// Fills the argument block of the fused kernel (host copy, to be uploaded once per state parity /
-// plane-wave mode); returns the number of workgroups, or -1 if the grid is not supported.
+// Argument block of the interior kernel (host copy, uploaded once per state parity / plane-wave
+// mode); returns the number of interior workgroups (0 if the interior is empty), -1 if unsupported.
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const GridDev &g = f.g;
   if (!fused_supported(g)) return -1;
   if (f.mat && f.ntab > 256) return -1;
   const int ty = (tune.tile_rows == 4 || tune.tile_rows == 13) ? tune.tile_rows : 8;
-  const int nTx = (g.I + 63) / 64, nTy = (g.J + ty - 1) / ty;
+  const int cI = g.I - 2 * g.P, cJ = g.J - 2 * g.P, cK = g.K - 2 * g.P;
+  const int nTx = (cI + 63) / 64, nTy = (cJ + ty - 1) / ty;
   int kc = tune.z_chunk;
   if (kc <= 0) {
     const int want = 2048;  // workgroups per launch (several per CU)
     int nkc = (want + nTx * nTy - 1) / (nTx * nTy);
     if (nkc < 1) nkc = 1;
-    kc = (g.K + nkc - 1) / nkc;
+    kc = (cK + nkc - 1) / nkc;
     if (kc < 16) kc = 16;
   }
-  const int nKc = (g.K + kc - 1) / kc;
+  const int nKc = (cK + kc - 1) / kc;
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
@@ -1136,23 +1143,78 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
   for (int c = 0; c < 3; ++c) { a.polE[c] = f.polE[c]; a.polH[c] = f.polH[c]; }
   std::memcpy(out, &a, sizeof(a));
-  return nTx * nTy * nKc;
+  return (cI > 0 && cJ > 0 && cK > 0) ? nTx * nTy * nKc : 0;
 }
 
-// Launches the fused step with an argument block already in device memory (dargs) whose host
-// copy is hargs.
-hipError_t launch_step_fused(hipStream_t st, const void *dargs, const void *hargs, int nwg, int n) {
-  const StepArgs &a = *(const StepArgs *)hargs;
-  const StepArgs *d = (const StepArgs *)dargs;
-  const int ty = a.pad;
-  if (a.mat) {
-    if (ty == 4) return launch_fused_t<true, 4>(st, d, nwg, n);
-    if (ty == 13) return launch_fused_t<true, 13>(st, d, nwg, n);
-    return launch_fused_t<true, 8>(st, d, nwg, n);
+static SlabArgs slab_args(const FusedDev &f, int n) {
+  SlabArgs s;
+  const GridDev &g = f.g;
+  const size_t IJK = (size_t)g.I * g.J * g.K;
+  const size_t sx = (size_t)2 * g.P * g.J * g.K, sy = (size_t)2 * g.I * g.P * g.K, sz = (size_t)2 * g.I * g.J * g.P;
+  s.g = g;
+  for (int c = 0; c < 6; ++c) {
+    s.cur.f[c] = const_cast<float *>(f.Fc) + c * IJK;
+    s.nxt.f[c] = f.Fn + c * IJK;
   }
-  if (ty == 4) return launch_fused_t<false, 4>(st, d, nwg, n);
-  if (ty == 13) return launch_fused_t<false, 13>(st, d, nwg, n);
-  return launch_fused_t<false, 8>(st, d, nwg, n);
+  auto psi = [&](float *base, PsiSet &ps) {
+    float *px = base, *py = px + 4 * sx, *pz = py + 4 * sy;
+    ps.p[P_EYX] = px; ps.p[P_EZX] = px + sx; ps.p[P_HYX] = px + 2 * sx; ps.p[P_HZX] = px + 3 * sx;
+    ps.p[P_EXY] = py; ps.p[P_EZY] = py + sy; ps.p[P_HXY] = py + 2 * sy; ps.p[P_HZY] = py + 3 * sy;
+    ps.p[P_EXZ] = pz; ps.p[P_EYZ] = pz + sz; ps.p[P_HXZ] = pz + 2 * sz; ps.p[P_HYZ] = pz + 3 * sz;
+  };
+  psi(const_cast<float *>(f.Pc), s.pcur);
+  psi(f.Pn, s.pnxt);
+  CoefDev &co = s.co;
+  co.mat = f.mat;
+  co.tab = f.mat ? f.coef : nullptr;
+  co.ntab = f.ntab;
+  for (int c = 0; c < 3; ++c) {
+    co.Ce[c] = f.mat ? nullptr : f.coef + c * IJK;
+    co.Ch[c] = f.mat ? nullptr : f.coef + (3 + c) * IJK;
+  }
+  const size_t P2 = 2 * (size_t)g.P;
+  co.bE = f.prm; co.cE = f.prm + P2; co.bH = f.prm + 2 * P2; co.cH = f.prm + 3 * P2;
+  co.ikE[0] = f.prm + 4 * P2; co.ikE[1] = co.ikE[0] + g.I; co.ikE[2] = co.ikE[1] + g.J;
+  co.ikH[0] = co.ikE[2] + g.K; co.ikH[1] = co.ikH[0] + g.I; co.ikH[2] = co.ikH[1] + g.J;
+  co.Chh = f.Chh; co.Che = f.Che; co.dx = f.dx;
+  PWDev &pw = s.pw;
+  pw.active_h = f.act_h; pw.active_e = f.act_e; pw.N1 = f.N1; pw.Eh = f.Eh; pw.Hh = f.Hh;
+  if (f.pwtab) {
+    const size_t tl = 2 * ((size_t)g.I + g.J + g.K);
+    const size_t off[3] = {0, 2 * (size_t)g.I, 2 * (size_t)g.I + 2 * (size_t)g.J};
+    const int Q[3] = {g.I, g.J, g.K};
+    for (int u = 0; u < 3; ++u)
+      for (int h = 0; h < 2; ++h) {
+        pw.tabE[u][h] = f.pwtab + off[u] + (size_t)h * Q[u];
+        pw.tabH[u][h] = f.pwtab + tl + off[u] + (size_t)h * Q[u];
+      }
+  }
+  for (int c = 0; c < 3; ++c) { pw.polE[c] = f.polE[c]; pw.polH[c] = f.polH[c]; }
+  s.n = n;
+  return s;
+}
+
+// One time step: A (slab H), B (interior H and E), C (slab E).
+hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n) {
+  const GridDev &g = f.g;
+  const SlabArgs sa = slab_args(f, n);
+  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P) + 2L * g.P * (g.J - 2 * g.P) * (g.K - 2 * g.P);
+  const unsigned nb = cdiv(nslab, 256);
+  hipLaunchKernelGGL(k_slab_h, dim3(nb), dim3(256), 0, st, sa);
+  if (nwg > 0) {
+    const StepArgs &a = *(const StepArgs *)hargs;
+    const StepArgs *d = (const StepArgs *)dargs;
+    const int ty = a.pad;
+    hipError_t e;
+    if (a.mat) e = ty == 4 ? launch_interior_t<true, 4>(st, d, nwg, n) : ty == 13 ? launch_interior_t<true, 13>(st, d, nwg, n)
+                                                                       : launch_interior_t<true, 8>(st, d, nwg, n);
+    else e = ty == 4 ? launch_interior_t<false, 4>(st, d, nwg, n) : ty == 13 ? launch_interior_t<false, 13>(st, d, nwg, n)
+                                                                   : launch_interior_t<false, 8>(st, d, nwg, n);
+    if (e != hipSuccess) return e;
+  }
+  if (f.mat) hipLaunchKernelGGL(k_slab_e<true>, dim3(nb), dim3(256), 0, st, sa);
+  else hipLaunchKernelGGL(k_slab_e<false>, dim3(nb), dim3(256), 0, st, sa);
+  return hipGetLastError();
 }
 
 }  // namespace fdtd

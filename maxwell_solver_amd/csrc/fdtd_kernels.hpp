@@ -77,7 +77,7 @@ struct FusedDev {
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.
 size_t step_args_size();
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out);
-hipError_t launch_step_fused(hipStream_t st, const void *dargs, const void *hargs, int nwg, int n);
+hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n);
 bool fused_supported(const GridDev &g);
 
 }  // namespace fdtd
