@@ -748,9 +748,17 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 }
 
 
-template <bool IDX, int TY>
-__global__ void __launch_bounds__(64 * (TY + 3)) k_interior(const StepArgs *__restrict__ pa, int n) {
-  // Every field the loop needs is read once here (scalar loads, before any store).
+// Interior kernel: one wave per grid row segment, barrier-free.  Lane l of a wave stands for column
+// i = P - 1 + 62 tx + l of row j; lanes 1..62 own their cells, lane 0 supplies H at i-1 and lane 63
+// supplies E at i+1 through cross-lane shuffles.  Each lane marches k up the chunk and per plane
+// computes H at (i, j) and Hx, Hz at (i, j-1) (the row below belongs to another wave, so those two
+// components are recomputed from the old state), then E at (i, j).  H at (i, j, k-1) comes from
+// the previous plane in registers.  H of cells in the CPML slabs (outside C) is not recomputed: it
+// is read from the new state written by the slab-H kernel.
+constexpr int kOwnCols = 62;
+
+template <bool IDX>
+__global__ void __launch_bounds__(256) k_interior(const StepArgs *__restrict__ pa, int n) {
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
   const int nTx = pa->nTx, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab, act_h = pa->act_h, act_e = pa->act_e;
@@ -759,13 +767,11 @@ __global__ void __launch_bounds__(64 * (TY + 3)) k_interior(const StepArgs *__re
   const auto Fn = gptr(pa->Fn);
   const auto mat = gptr(pa->mat);
   const auto coef = gptr(pa->coef);
-  const auto prm = gptr(pa->prm);
   const float Chh = pa->Chh, Che = pa->Che;
-  constexpr int RW = TY + 2;  // row waves (with the two halo rows)
-  constexpr int SX = 66;      // LDS columns ib-1 .. ib+64
-  __shared__ float sE[3][RW][SX];
-  __shared__ float sH[3][TY + 1][SX];
   __shared__ float sCoef[IDX ? 256 * 6 : 6];
+  if (IDX)
+    for (int q = threadIdx.x; q < ntab * 6; q += 256) sCoef[q] = coef[q];
+  __syncthreads();
 
   const size_t IJ = (size_t)I * J;
   const int cx1 = I - P, cy1 = J - P, cz1 = K - P;  // C = [P, cx1) x [P, cy1) x [P, cz1)
@@ -773,142 +779,110 @@ __global__ void __launch_bounds__(64 * (TY + 3)) k_interior(const StepArgs *__re
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = L % nTx, t2 = L / nTx;
   const int ty = t2 % nTy, tz = t2 / nTy;
-  const int ib = P + tx * 64, jb = P + ty * TY, kb = P + tz * KC, ke = min(kb + KC, cz1);
-
-  int i = -1000, r = 0;
-  if (w < RW) {
-    i = ib + lane;
-    r = w;
-  } else if (lane < RW) {
-    i = ib - 1;
-    r = lane;
-  } else if (lane >= 32 && lane < 32 + RW) {
-    i = ib + 64;
-    r = lane - 32;
-  }
-  const int j = jb - 1 + r;
-  const int slot = i - ib + 1;
-  const bool has_slot = i != -1000;  // idle lanes of the halo wave own no LDS slot
-  const bool valid = has_slot && (i >= 0) && (i < I) && (j >= 0) && (j < J);
-  const bool inC = (i >= P) && (i < cx1) && (j >= P) && (j < cy1);
-  const bool needH = valid && (r <= TY) && (slot <= 64);
-  const bool doH = needH && inC;    // H recomputed here
-  const bool getH = needH && !inC;  // H of a slab cell: already new in HBM (kernel A)
-  const bool own = valid && inC && (r >= 1) && (r <= TY) && (slot >= 1) && (slot <= 64);
-
-  if (IDX)
-    for (int q = threadIdx.x; q < ntab * 6; q += 64 * (TY + 3)) sCoef[q] = coef[q];
+  const int i = P - 1 + kOwnCols * tx + lane;
+  const int j = P + 4 * ty + w;
+  const int kb = P + tz * KC, ke = min(kb + KC, cz1);
+  if (j >= cy1) return;  // whole wave: no barrier follows
+  const bool inI = i < I;
+  const bool iC = (i >= P) && (i < cx1);          // column inside C
+  const bool own = iC && lane >= 1 && lane <= kOwnCols;
+  const bool hlane = lane <= kOwnCols && inI;      // lanes whose H feeds an own E
+  const bool hC = hlane && iC;                     // H recomputed here
+  const bool jm_C = (j - 1) >= P;                  // row j-1 inside C
 
   const auto Ex = Fc, Ey = Fc + IJK, Ez = Fc + 2 * IJK;
   const auto Hx = Fc + 3 * IJK, Hy = Fc + 4 * IJK, Hz = Fc + 5 * IJK;
   const auto Nx = Fn + 3 * IJK, Ny = Fn + 4 * IJK, Nz = Fn + 5 * IJK;
-  const auto ikEx = prm + 8 * P, ikEy = ikEx + I, ikEz = ikEy + J;
-  const auto ikHx = ikEz + K, ikHy = ikHx + I, ikHz = ikHy + J;
-  const float kHx = doH ? ikHx[i] : 1.0f, kHy = doH ? ikHy[j] : 1.0f;
-  const float kEx = own ? ikEx[i] : 1.0f, kEy = own ? ikEy[j] : 1.0f;
-  const size_t col = valid ? (size_t)i + (size_t)j * I : 0;
+  const size_t col = inI ? (size_t)i + (size_t)j * I : (size_t)j * I;
 
-  // plane kb-1: recomputed when it lies in C, else read from the new state (slab)
   int k = kb > P ? kb - 1 : kb;
-  float hpx = 0.0f, hpy = 0.0f;
-  if (own && kb == P && kb > 0) {
-    hpx = Nx[col + (size_t)(kb - 1) * IJ];
-    hpy = Ny[col + (size_t)(kb - 1) * IJ];
-  }
-  // register planes: E(k), E(k+1); H source (old for doH lanes, new for getH lanes) at k; material
-  float ea[3] = {0, 0, 0}, eb[3] = {0, 0, 0}, ha[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
-  int ma = 0, mb = 0;
+  // E of row j at plane k, and Ey of row j-1 at plane k (carried from plane to plane)
+  float ex = 0, ey = 0, ez = 0, eym = 0;
+  float hpx = 0, hpy = 0;  // H(i, j, k-1) of own cells
   {
     const size_t id = col + (size_t)k * IJ;
-    if (valid) {
-      ea[0] = Ex[id]; ea[1] = Ey[id]; ea[2] = Ez[id];
-      if (k + 1 < K) { eb[0] = Ex[id + IJ]; eb[1] = Ey[id + IJ]; eb[2] = Ez[id + IJ]; }
+    if (inI) {
+      ex = Ex[id]; ey = Ey[id]; ez = Ez[id];
+      eym = Ey[id - I];
     }
-    if (doH) { ha[0] = Hx[id]; ha[1] = Hy[id]; ha[2] = Hz[id]; }
-    if (getH) { ha[0] = Nx[id]; ha[1] = Ny[id]; ha[2] = Nz[id]; }
-    if (IDX && own) ma = mat[id];
+    if (own && kb == P) {  // plane P-1 is a slab plane: its new H is already written
+      hpx = Nx[id - IJ];
+      hpy = Ny[id - IJ];
+    }
   }
-
-  // One plane: ec holds E(k) on entry and receives E(k+2); en holds E(k+1); hs holds the H source
-  // of plane k, hn receives plane k+1's; mc / mn likewise for the material byte.
-  auto plane = [&](float (&ec)[3], float (&en)[3], float (&hs)[3], float (&hn)[3], int &mc, int &mn) {
+  for (; k < ke; ++k) {
     const size_t id = col + (size_t)k * IJ;
-    if (has_slot) {
-      sE[0][r][slot] = ec[0];
-      sE[1][r][slot] = ec[1];
-      sE[2][r][slot] = ec[2];
-    }
-    if (valid && k + 2 < K) { ec[0] = Ex[id + 2 * IJ]; ec[1] = Ey[id + 2 * IJ]; ec[2] = Ez[id + 2 * IJ]; }
-    if (k + 1 < ke) {
-      if (doH) { hn[0] = Hx[id + IJ]; hn[1] = Hy[id + IJ]; hn[2] = Hz[id + IJ]; }
-      if (getH) { hn[0] = Nx[id + IJ]; hn[1] = Ny[id + IJ]; hn[2] = Nz[id + IJ]; }
-      if (IDX && own) mn = mat[id + IJ];
-    }
-    lds_barrier();
-
     const bool own_plane = k >= kb;
-    float hx = hs[0], hy = hs[1], hz = hs[2];
-    if (doH) {
-      const float ex = sE[0][r][slot], ey = sE[1][r][slot], ez = sE[2][r][slot];
-      const float ex_j1 = sE[0][r + 1][slot], ez_j1 = sE[2][r + 1][slot];
-      const float ey_i1 = sE[1][r][slot + 1], ez_i1 = sE[2][r][slot + 1];
-      // K1 GPU_updateH, src/gpu.h:592-601 (C cells satisfy its mask)
-      const float kz = ikHz[k];
-      hx = Chh * hx + Che * (-(ez_j1 - ez) * kHy + (en[1] - ey) * kz);
-      hy = Chh * hy + Che * (-(en[0] - ex) * kz + (ez_i1 - ez) * kHx);
-      hz = Chh * hz + Che * (-(ey_i1 - ey) * kHx + (ex_j1 - ex) * kHy);
-      if (act_h && i >= bi0 - 1 && i <= bi1 && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1) {
+    // ---- loads of this plane (independent, issued together) ----
+    float nx = 0, ny = 0, nz = 0, nym = 0;       // E(k+1) row j, Ey(k+1) row j-1
+    float exm = 0, ezm = 0, exp_ = 0, ezp = 0;   // Ex, Ez of rows j-1 and j+1
+    float hx = 0, hy = 0, hz = 0, hxm = 0, hzm = 0;
+    int mt = 0;
+    if (inI) {
+      nx = Ex[id + IJ]; ny = Ey[id + IJ]; nz = Ez[id + IJ];
+      nym = Ey[id + IJ - I];
+      exm = Ex[id - I]; ezm = Ez[id - I];
+      exp_ = Ex[id + I]; ezp = Ez[id + I];
+    }
+    if (hC) { hx = Hx[id]; hy = Hy[id]; hz = Hz[id]; }
+    else if (hlane) { hx = Nx[id]; hy = Ny[id]; hz = Nz[id]; }
+    if (hC && jm_C) { hxm = Hx[id - I]; hzm = Hz[id - I]; }
+    else if (hlane) { hxm = Nx[id - I]; hzm = Nz[id - I]; }
+    if (IDX && own && own_plane) mt = mat[id];
+    // ---- x neighbours through the wave ----
+    const float ey_i1 = __shfl_down(ey, 1), ez_i1 = __shfl_down(ez, 1), eym_i1 = __shfl_down(eym, 1);
+    // ---- H(i, j, k) and Hx, Hz at (i, j-1, k): K1 GPU_updateH, src/gpu.h:592-601 ----
+    // (C cells satisfy its mask; 1/kappa is exactly 1 in C, so the x * 1.0f factors are omitted)
+    if (hC) {
+      hx = Chh * hx + Che * (-(ezp - ez) + (ny - ey));
+      hy = Chh * hy + Che * (-(nx - ex) + (ez_i1 - ez));
+      hz = Chh * hz + Che * (-(ey_i1 - ey) + (exp_ - ex));
+      if (act_h && i >= bi0 - 1 && i <= bi1 && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1 &&
+          (i == bi0 - 1 || i == bi1 || j == bj0 - 1 || j == bj1 || k == bk0 - 1 || k == bk1)) {
         const F3 o = fused_pw_h(pa, n, i, j, k, F3{hx, hy, hz});
         hx = o.x; hy = o.y; hz = o.z;
       }
+      if (jm_C) {
+        const int jm = j - 1;
+        hxm = Chh * hxm + Che * (-(ez - ezm) + (nym - eym));
+        hzm = Chh * hzm + Che * (-(eym_i1 - eym) + (ex - exm));
+        if (act_h && i >= bi0 - 1 && i <= bi1 && jm >= bj0 - 1 && jm <= bj1 && k >= bk0 - 1 && k <= bk1 &&
+            (i == bi0 - 1 || i == bi1 || jm == bj0 - 1 || jm == bj1 || k == bk0 - 1 || k == bk1)) {
+          const F3 o = fused_pw_h(pa, n, i, jm, k, F3{hxm, 0.0f, hzm});
+          hxm = o.x; hzm = o.z;
+        }
+      }
     }
-    if (needH) {
-      sH[0][r][slot] = hx;
-      sH[1][r][slot] = hy;
-      sH[2][r][slot] = hz;
-    }
+    const float hy_im = __shfl_up(hy, 1), hz_im = __shfl_up(hz, 1);
     if (own && own_plane) {
       Nx[id] = hx;
       Ny[id] = hy;
       Nz[id] = hz;
-    }
-    lds_barrier();
-
-    if (own && own_plane) {
-      const float hx_jm = sH[0][r - 1][slot], hz_jm = sH[2][r - 1][slot];
-      const float hy_im = sH[1][r][slot - 1], hz_im = sH[2][r][slot - 1];
       float ce[3], ch[3];
       if (IDX) {
-        const float *row = sCoef + 6 * mc;
+        const float *row = sCoef + 6 * mt;
         ce[0] = row[0]; ce[1] = row[1]; ce[2] = row[2];
         ch[0] = row[3]; ch[1] = row[4]; ch[2] = row[5];  // C cells are never on the last x plane
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) { ce[c] = coef[c * IJK + id]; ch[c] = coef[(3 + c) * IJK + id]; }
       }
-      float ex = sE[0][r][slot], ey = sE[1][r][slot], ez = sE[2][r][slot];
-      // K3 GPU_updateE, src/gpu.h:745-762 (C cells satisfy its masks)
-      const float kz = ikEz[k];
-      ex = ce[0] * ex + ch[0] * ((hz - hz_jm) * kEy - (hy - hpy) * kz);
-      ey = ce[1] * ey + ch[1] * ((hx - hpx) * kz - (hz - hz_im) * kEx);
-      ez = ce[2] * ez + ch[2] * ((hy - hy_im) * kEx - (hx - hx_jm) * kEy);
-      if (act_e && i >= bi0 && i <= bi1 && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1) {
-        const F3 o = fused_pw_e(pa, n, F3{ch[0], ch[1], ch[2]}, i, j, k, F3{ex, ey, ez});
-        ex = o.x; ey = o.y; ez = o.z;
+      // K3 GPU_updateE, src/gpu.h:745-762 (C cells satisfy its masks; 1/kappa = 1 in C)
+      float ux = ce[0] * ex + ch[0] * ((hz - hzm) - (hy - hpy));
+      float uy = ce[1] * ey + ch[1] * ((hx - hpx) - (hz - hz_im));
+      float uz = ce[2] * ez + ch[2] * ((hy - hy_im) - (hx - hxm));
+      if (act_e && i >= bi0 && i <= bi1 && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1 &&
+          (i == bi0 || i == bi1 || j == bj0 || j == bj1 || k == bk0 || k == bk1)) {
+        const F3 o = fused_pw_e(pa, n, F3{ch[0], ch[1], ch[2]}, i, j, k, F3{ux, uy, uz});
+        ux = o.x; uy = o.y; uz = o.z;
       }
-      Fn[id] = ex;
-      Fn[IJK + id] = ey;
-      Fn[2 * IJK + id] = ez;
+      Fn[id] = ux;
+      Fn[IJK + id] = uy;
+      Fn[2 * IJK + id] = uz;
     }
     hpx = hx;
     hpy = hy;
-    ++k;
-  };
-  // two planes per trip with swapped register roles: no copies of in-flight loads
-  while (k < ke) {
-    plane(ea, eb, ha, hb, ma, mb);
-    if (k >= ke) break;
-    plane(eb, ea, hb, ha, mb, ma);
+    ex = nx; ey = ny; ez = nz; eym = nym;
   }
 }
 
@@ -1103,9 +1077,9 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
 
 bool fused_supported(const GridDev &g) { return g.I >= 2 * g.P + 2 && g.J >= 2 * g.P + 2 && g.K >= 2 * g.P + 2; }
 
-template <bool IDX, int TY>
+template <bool IDX>
 static hipError_t launch_interior_t(hipStream_t st, const StepArgs *dargs, int nwg, int n) {
-  hipLaunchKernelGGL((k_interior<IDX, TY>), dim3(nwg), dim3(64 * (TY + 3)), 0, st, dargs, n);
+  hipLaunchKernelGGL((k_interior<IDX>), dim3(nwg), dim3(256), 0, st, dargs, n);
   return hipGetLastError();
 }
 
@@ -1117,12 +1091,12 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const GridDev &g = f.g;
   if (!fused_supported(g)) return -1;
   if (f.mat && f.ntab > 256) return -1;
-  const int ty = (tune.tile_rows == 4 || tune.tile_rows == 13) ? tune.tile_rows : 8;
+  const int ty = 4;  // rows (waves) per workgroup
   const int cI = g.I - 2 * g.P, cJ = g.J - 2 * g.P, cK = g.K - 2 * g.P;
-  const int nTx = (cI + 63) / 64, nTy = (cJ + ty - 1) / ty;
+  const int nTx = (cI + kOwnCols - 1) / kOwnCols, nTy = (cJ + ty - 1) / ty;
   int kc = tune.z_chunk;
   if (kc <= 0) {
-    const int want = 2048;  // workgroups per launch (several per CU)
+    const int want = 4096;  // workgroups per launch (several per CU)
     int nkc = (want + nTx * nTy - 1) / (nTx * nTy);
     if (nkc < 1) nkc = 1;
     kc = (cK + nkc - 1) / nkc;
@@ -1204,12 +1178,7 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
   if (nwg > 0) {
     const StepArgs &a = *(const StepArgs *)hargs;
     const StepArgs *d = (const StepArgs *)dargs;
-    const int ty = a.pad;
-    hipError_t e;
-    if (a.mat) e = ty == 4 ? launch_interior_t<true, 4>(st, d, nwg, n) : ty == 13 ? launch_interior_t<true, 13>(st, d, nwg, n)
-                                                                       : launch_interior_t<true, 8>(st, d, nwg, n);
-    else e = ty == 4 ? launch_interior_t<false, 4>(st, d, nwg, n) : ty == 13 ? launch_interior_t<false, 13>(st, d, nwg, n)
-                                                                   : launch_interior_t<false, 8>(st, d, nwg, n);
+    const hipError_t e = a.mat ? launch_interior_t<true>(st, d, nwg, n) : launch_interior_t<false>(st, d, nwg, n);
     if (e != hipSuccess) return e;
   }
   if (f.mat) hipLaunchKernelGGL(k_slab_e<true>, dim3(nb), dim3(256), 0, st, sa);
