@@ -93,6 +93,9 @@ struct fdtd_sim {
   float *dPsi[2][12] = {};
   size_t psi_len[12] = {};
   size_t psi_total = 0;
+  size_t cs = 0;                                   // field component stride
+  static constexpr size_t kCompSkew = 2112;        // 8 KiB + 256 B
+  static constexpr size_t kSetSkew = 3136;         // 12 KiB + 256 B
   int cur = 0;
   bool second = false;
   uint8_t *dMat = nullptr;
@@ -354,7 +357,7 @@ struct fdtd_sim {
   }
 
   int zero_state() {
-    HIPCHK(hipMemsetAsync(dFb[cur], 0, 6 * IJK * sizeof(float), stream));
+    HIPCHK(hipMemsetAsync(dFb[cur], 0, 6 * cs * sizeof(float), stream));
     HIPCHK(hipMemsetAsync(dPsiB[cur], 0, psi_total * sizeof(float), stream));
     if (dPort) HIPCHK(hipMemsetAsync(dPort, 0, port_len * sizeof(float), stream));
     rec_step = 0;
@@ -366,8 +369,14 @@ struct fdtd_sim {
   int alloc_set(int which) {
     const size_t P = (size_t)cp.P;
     const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
-    if (int rc = dalloc(&dFb[which], 6 * IJK)) return rc;
-    for (int c = 0; c < 6; ++c) dF[which][c] = dFb[which] + c * IJK;
+    // components cs = I*J*K + skew apart, and the two sets offset differently, so that the twelve
+    // streams a step touches at one cell do not sit at large power-of-two distances (HBM channel
+    // interleaving would otherwise send them to the same channels)
+    cs = IJK + kCompSkew;
+    float *blk = nullptr;
+    if (int rc = dalloc(&blk, 6 * cs + kSetSkew)) return rc;
+    dFb[which] = blk + (which ? kSetSkew : 0);
+    for (int c = 0; c < 6; ++c) dF[which][c] = dFb[which] + c * cs;
     if (int rc = dalloc(&dPsiB[which], 4 * (sx + sy + sz))) return rc;
     float *px = dPsiB[which], *py = px + 4 * sx, *pz = py + 4 * sy;
     float **q = dPsi[which];
@@ -535,6 +544,7 @@ struct fdtd_sim {
     {
       f.g = g;
       f.Fc = dFb[cur]; f.Fn = dFb[1 - cur]; f.Pc = dPsiB[cur]; f.Pn = dPsiB[1 - cur];
+      f.cs = cs;
       f.mat = index_mode ? dMat : nullptr;
       f.coef = index_mode ? dTab : dCoefFull;
       f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;

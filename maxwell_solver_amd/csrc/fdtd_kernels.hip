@@ -580,6 +580,7 @@ struct StepArgs {
   int nTx, nTy, KC, ntab;
   int act_h, act_e, N1, pad;
   size_t IJK, sx, sy, sz;
+  size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
   float *Fn;
   const float *Pc;
@@ -762,7 +763,7 @@ __global__ void __launch_bounds__(256) k_interior(const StepArgs *__restrict__ p
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
   const int nTx = pa->nTx, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab, act_h = pa->act_h, act_e = pa->act_e;
-  const size_t IJK = pa->IJK;
+  const size_t IJK = pa->IJK, cs = pa->cs;
   const auto Fc = gptr(pa->Fc);
   const auto Fn = gptr(pa->Fn);
   const auto mat = gptr(pa->mat);
@@ -790,9 +791,9 @@ __global__ void __launch_bounds__(256) k_interior(const StepArgs *__restrict__ p
   const bool hC = hlane && iC;                     // H recomputed here
   const bool jm_C = (j - 1) >= P;                  // row j-1 inside C
 
-  const auto Ex = Fc, Ey = Fc + IJK, Ez = Fc + 2 * IJK;
-  const auto Hx = Fc + 3 * IJK, Hy = Fc + 4 * IJK, Hz = Fc + 5 * IJK;
-  const auto Nx = Fn + 3 * IJK, Ny = Fn + 4 * IJK, Nz = Fn + 5 * IJK;
+  const auto Ex = Fc, Ey = Fc + cs, Ez = Fc + 2 * cs;
+  const auto Hx = Fc + 3 * cs, Hy = Fc + 4 * cs, Hz = Fc + 5 * cs;
+  const auto Nx = Fn + 3 * cs, Ny = Fn + 4 * cs, Nz = Fn + 5 * cs;
   const size_t col = inI ? (size_t)i + (size_t)j * I : (size_t)j * I;
 
   int k = kb > P ? kb - 1 : kb;
@@ -877,8 +878,8 @@ __global__ void __launch_bounds__(256) k_interior(const StepArgs *__restrict__ p
         ux = o.x; uy = o.y; uz = o.z;
       }
       Fn[id] = ux;
-      Fn[IJK + id] = uy;
-      Fn[2 * IJK + id] = uz;
+      Fn[cs + id] = uy;
+      Fn[2 * cs + id] = uz;
     }
     hpx = hx;
     hpy = hy;
@@ -1110,6 +1111,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.nTx = nTx; a.nTy = nTy; a.KC = kc; a.ntab = f.ntab;
   a.act_h = f.act_h; a.act_e = f.act_e; a.N1 = f.N1; a.pad = ty;
   a.IJK = (size_t)g.I * g.J * g.K;
+  a.cs = f.cs;
   a.sx = (size_t)2 * g.P * g.J * g.K; a.sy = (size_t)2 * g.I * g.P * g.K; a.sz = (size_t)2 * g.I * g.J * g.P;
   a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
   a.mat = f.mat; a.coef = f.coef; a.prm = f.prm; a.pwtab = f.pwtab;
@@ -1127,8 +1129,8 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
   const size_t sx = (size_t)2 * g.P * g.J * g.K, sy = (size_t)2 * g.I * g.P * g.K, sz = (size_t)2 * g.I * g.J * g.P;
   s.g = g;
   for (int c = 0; c < 6; ++c) {
-    s.cur.f[c] = const_cast<float *>(f.Fc) + c * IJK;
-    s.nxt.f[c] = f.Fn + c * IJK;
+    s.cur.f[c] = const_cast<float *>(f.Fc) + c * f.cs;
+    s.nxt.f[c] = f.Fn + c * f.cs;
   }
   auto psi = [&](float *base, PsiSet &ps) {
     float *px = base, *py = px + 4 * sx, *pz = py + 4 * sy;

@@ -59,7 +59,8 @@ struct StepTuning {
 };
 struct FusedDev {
   GridDev g;
-  const float *Fc = nullptr;  // six fields, component c at c*I*J*K
+  const float *Fc = nullptr;  // six fields, component c at c*cs
+  size_t cs = 0;
   float *Fn = nullptr;
   const float *Pc = nullptr;  // psi groups: x slabs [Eyx Ezx Hyx Hzx], y [Exy Ezy Hxy Hzy], z [Exz Eyz Hxz Hyz]
   float *Pn = nullptr;
