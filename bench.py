@@ -3,11 +3,15 @@
 Workload (default): 256^3 FP32 grid, 10-cell CPML (m=3, kappa_max=5, alpha_max=0.05), 40 seeded
 dielectric spheres stored as a uint8 material index, oblique TF/SF plane wave at seeded angles,
 64 probes recording Ez and Hy every step, Ez snapshot every 250 steps.  A step is one full time
-step of the grid (H update, TF/SF, CPML, E update, TF/SF, CPML, probes) = one fused kernel.
+step of the grid (H update, TF/SF, CPML, E update, TF/SF, CPML, probes): three launches, the
+y/z-slab H kernel, the row kernel (every other row, the step's dominant launch) and the y/z-slab
+E kernel, plus the probe gather.
 
 Prints one JSON line: Gcell-updates/s over exactly --steps timed steps (inputs resident in HBM),
-the fused kernel's HBM roofline fraction from HIP-event kernel times, and the CPU oracle timed on
-a bounded sample of the same workload on this host's cores.
+the row kernel's HBM roofline fraction from HIP-event times around its launches, and the CPU
+oracle timed on a bounded sample of the same workload on this host's cores.
+
+--copy-calibration runs a plain device copy of known size (the scale of the HBM byte counters).
 """
 import argparse
 import json
@@ -26,6 +30,26 @@ def bytes_per_cell(sc, material_bytes=1):
     arrays read + written over the face slabs."""
     P = sc.P
     return 48.0 + material_bytes + 64.0 * P * (1.0 / sc.I + 1.0 / sc.J + 1.0 / sc.K)
+
+
+def row_kernel_bytes(sc, material_bytes=1):
+    """Compulsory HBM bytes of one row-kernel launch: the rows j in [P, J-P) of the planes k in
+    [P, K-P), every column: 6 components read + written and the material byte per cell, plus the
+    x-slab psi (Hyx, Hzx, Eyx, Ezx read + written) in the 2P x-slab columns of each row."""
+    P = sc.P
+    rows = (sc.J - 2 * P) * (sc.K - 2 * P)
+    return rows * (sc.I * (48.0 + material_bytes) + 2 * P * 32.0)
+
+
+def copy_calibration(nbytes=1 << 30, reps=10):
+    """A plain device-to-device copy of nbytes, reps times (reads and writes nbytes per launch)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": reps}), flush=True)
 
 
 def cpu_baseline(sc, budget_s=15.0, max_steps=200):
@@ -61,15 +85,20 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--tile-rows", type=int, default=8)
+    ap.add_argument("--rows-per-group", type=int, default=0)
+    ap.add_argument("--cols-per-lane", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU z-slab "
               "decomposition is a design, not a build, in this version.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from maxwell_solver_amd import Simulation
     from maxwell_solver_amd import scenarios as S
@@ -81,7 +110,7 @@ def main():
         sc.snap_every = 250
     g = Simulation()
     S.setup(g, sc)
-    g.set_tuning(args.tile_rows, args.z_chunk)
+    g.set_tuning(args.rows_per_group, args.z_chunk, args.cols_per_lane)
 
     # warmup (untimed), then exactly --steps timed steps
     if args.warmup:
@@ -93,14 +122,17 @@ def main():
     g.run(args.warmup, args.steps, sc.mode, snap_every=sc.snap_every, snap_comp=sc.snap_comp)
     g.synchronize()
     dt = time.perf_counter() - t0
-    kern_ms, launches = g.kernel_time()
+    step_ms, launches = g.kernel_time()
+    rows_ms = g.row_kernel_time()
 
     cell_updates = sc.cells * args.steps
     value = cell_updates / dt / 1e9
     bpc = bytes_per_cell(sc)
-    achieved = bpc * sc.cells / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    rows_bytes = row_kernel_bytes(sc)
+    achieved = rows_bytes / (rows_ms * 1e-3) / 1e9 if rows_ms > 0 else 0.0
+    step_achieved = bpc * sc.cells / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_fused_step_{sc.I}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_rows_{sc.I}.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")
@@ -125,18 +157,24 @@ def main():
             "grid": [sc.I, sc.J, sc.K],
             "cells": sc.cells,
             "time_steps": args.steps,
-            "kernel": "k_step_fused (one launch per time step)",
+            "kernels": "k_slab_h (y/z-slab H + TF/SF x-face sheets), k_rows (all other rows), k_slab_e (y/z-slab E), "
+                       "k_probes",
         },
         "roofline": {
+            "kernel": "k_rows",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "bytes_per_cell": round(bpc, 2),
-            "kernel_ms_avg": round(kern_ms, 5),
+            "algorithmic_bytes_per_launch": int(rows_bytes),
+            "kernel_ms_avg": round(rows_ms, 5),
             "kernel_launches": launches,
+            "step_ms_avg": round(step_ms, 5),
+            "step_bytes_per_cell": round(bpc, 2),
+            "step_achieved": round(step_achieved, 1),
+            "step_frac": round(step_achieved / HBM_PEAK_GBS, 4),
         },
     }
     g.close()

@@ -126,12 +126,18 @@ int fdtd_prepare(fdtd_sim *s);
  * 19 material id per cell; 20 + 4*axis + {0,1,2,3} oblique tables (E integer, E half, H integer,
  * H half); 32 polarisation [eE x,y,z, eH x,y,z].  *count receives the element count. */
 int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
-/* Launch tuning for the fused kernel: rows per tile (4, 8 or 13) and z-chunk depth (0 = automatic). */
-int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk);
+/* Launch tuning of the fused step's row kernel: rows per workgroup (1, 2 or 4; 0 = automatic),
+ * z-chunk depth in planes (0 = automatic) and columns per lane (1, 2 or 4; 0 = automatic; the
+ * grid's I must be a multiple of it and a row at most 256 of it).  An unsupported choice makes
+ * the next step fail with FDTD_ERR_INVALID. */
+int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
 /* Average device time of the fused step kernel over the last fdtd_run with timing on
  * (HIP events around each launch on the simulation's stream). */
 int fdtd_set_timing(fdtd_sim *s, int32_t on);
 int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches);
+/* Average device time of the row kernel alone (the rows j in [P, J-P) of the planes k in
+ * [P, K-P), the step's dominant launch) over the same run. */
+int fdtd_row_kernel_time(fdtd_sim *s, double *avg_ms);
 
 #ifdef __cplusplus
 }

@@ -107,6 +107,7 @@ struct fdtd_sim {
   float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
   float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
   float *dPwTab[6] = {};      // per plane-wave mode: E tables x,y,z (int, half), then H tables
+  float *dXs = nullptr;       // x-face sheets of the TF/SF box (fused step, refilled every step)
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
@@ -125,7 +126,7 @@ struct fdtd_sim {
   int args_gen = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;
-  double kernel_ms = 0;
+  double kernel_ms = 0, rows_ms = 0;  // summed step / row-kernel event times of the last timed run
   int64_t kernel_launches = 0;
   std::vector<void *> allocs;
 
@@ -539,6 +540,11 @@ struct fdtd_sim {
     if (int rc = check_pw(n, mode, &pw)) return rc;
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
+    if (pw && !dXs && g.j1 >= g.j0 && g.k1 >= g.k0) {
+      const size_t nxs = (size_t)8 * (g.j1 - g.j0 + 1) * (g.k1 - g.k0 + 1);
+      if (dalloc(&dXs, nxs)) return FDTD_ERR_NOMEM;
+      args_gen++;
+    }
     ArgSlot &slot = argslots[cur][slot_mode];
     FusedDev f;
     {
@@ -550,7 +556,7 @@ struct fdtd_sim {
       f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
       f.prm = dPrm;
       if (pw) {
-        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1;
+        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1; f.xs = dXs;
         f.act_h = pw->active_h; f.act_e = pw->active_e;
         for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
       }
@@ -559,14 +565,37 @@ struct fdtd_sim {
     if (slot.gen != args_gen) {
       slot.host.assign(step_args_size(), 0);
       slot.nwg = make_step_args(f, tune, slot.host.data());
-      if (slot.nwg < 0) return fail(FDTD_ERR_INVALID, "grid not supported by the fused kernel");
-      if (!slot.dptr && dalloc((char **)&slot.dptr, slot.host.size())) return FDTD_ERR_NOMEM;
-      HIPCHK(hipMemcpyAsync(slot.dptr, slot.host.data(), slot.host.size(), hipMemcpyHostToDevice, stream));
+      if (slot.nwg < 0 && (tune.cpl || tune.tile_rows))
+        return fail(FDTD_ERR_INVALID, "launch tuning not supported for this grid");
+      if (slot.nwg >= 0) {
+        if (!slot.dptr && dalloc((char **)&slot.dptr, slot.host.size())) return FDTD_ERR_NOMEM;
+        HIPCHK(hipMemcpyAsync(slot.dptr, slot.host.data(), slot.host.size(), hipMemcpyHostToDevice, stream));
+      }
       slot.gen = args_gen;
     }
-    if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
-    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n));
-    if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
+    if (slot.nwg < 0) {
+      // a grid outside the fused step's layouts (rows over 1024 columns, P over 64, ...): the
+      // per-op kernels in the reference op order, in place on the current set
+      if (timed) {
+        for (int q = 0; q < 3; ++q) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
+      }
+      HIPCHK(launch_update_h_all(stream, g, fields(cur), coefs()));
+      if (pw) HIPCHK(launch_pw_h(stream, g, fields(cur), coefs(), *pw, n));
+      HIPCHK(launch_update_h_pml(stream, g, fields(cur), psis(cur), coefs()));
+      HIPCHK(launch_update_e_all(stream, g, fields(cur), coefs()));
+      if (pw) HIPCHK(launch_pw_e(stream, g, fields(cur), coefs(), *pw, n));
+      HIPCHK(launch_update_e_pml(stream, g, fields(cur), psis(cur), coefs()));
+      if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
+      return record_probes(cur);
+    }
+    // timed: events before the step, around the row kernel, after the step
+    hipEvent_t *e = timed ? &ev[*ev_slot] : nullptr;
+    if (timed) {
+      HIPCHK(hipEventRecord(e[0], stream));
+      *ev_slot += 4;
+    }
+    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr));
+    if (timed) HIPCHK(hipEventRecord(e[3], stream));
     cur = 1 - cur;
     return record_probes(cur);
   }
@@ -826,7 +855,7 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
     }
   }
   if (s->timing) {
-    while ((int)s->ev.size() < 2 * count) {
+    while ((int)s->ev.size() < 4 * count) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
       s->ev.push_back(e);
@@ -843,13 +872,16 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
   }
   if (s->timing && count > 0) {
     HIPCHK(hipStreamSynchronize(s->stream));
-    double tot = 0;
+    double tot = 0, rows = 0;
     for (int q = 0; q < count; ++q) {
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, s->ev[2 * q], s->ev[2 * q + 1]));
+      float ms = 0, rms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, s->ev[4 * q], s->ev[4 * q + 3]));
+      HIPCHK(hipEventElapsedTime(&rms, s->ev[4 * q + 1], s->ev[4 * q + 2]));
       tot += ms;
+      rows += rms;
     }
     s->kernel_ms = tot;
+    s->rows_ms = rows;
     s->kernel_launches = count;
   }
   return 0;
@@ -981,12 +1013,16 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
   return 0;
 }
 
-int fdtd_set_tuning(fdtd_sim *s, int32_t tile_rows, int32_t z_chunk) {
+int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane) {
   NEED(s);
-  if (tile_rows != 4 && tile_rows != 8 && tile_rows != 13) return fail(FDTD_ERR_INVALID, "tile_rows must be 4, 8 or 13");
+  if (rows_per_group != 0 && rows_per_group != 1 && rows_per_group != 2 && rows_per_group != 4)
+    return fail(FDTD_ERR_INVALID, "rows_per_group must be 0, 1, 2 or 4");
   if (z_chunk < 0) return fail(FDTD_ERR_INVALID, "z_chunk must be >= 0");
-  s->tune.tile_rows = tile_rows;
+  if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
+    return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2 or 4");
+  s->tune.tile_rows = rows_per_group;
   s->tune.z_chunk = z_chunk;
+  s->tune.cpl = cols_per_lane;
   s->args_gen++;
   return 0;
 }
@@ -1001,6 +1037,12 @@ int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches) {
   NEED(s);
   if (avg_ms) *avg_ms = s->kernel_launches ? s->kernel_ms / (double)s->kernel_launches : 0.0;
   if (launches) *launches = s->kernel_launches;
+  return 0;
+}
+
+int fdtd_row_kernel_time(fdtd_sim *s, double *avg_ms) {
+  NEED(s);
+  if (avg_ms) *avg_ms = s->kernel_launches ? s->rows_ms / (double)s->kernel_launches : 0.0;
   return 0;
 }
 

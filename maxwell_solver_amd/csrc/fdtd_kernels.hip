@@ -559,26 +559,23 @@ __global__ void k_probes(FieldSet F, const int *idx, int nAnt, const int *comps,
 // ------------------------------------------------------------------------------------------
 // fused time step: three launches per step, reading the cur state and writing the nxt state
 //
-//  A  k_slab_h   one thread per H cell of the CPML slabs S (the cells within P of a face): main
-//                update, TF/SF, CPML Y-X-Z with psi, in the reference order.
-//  B  k_interior the interior box C = [P, I-P) x [P, J-P) x [P, K-P), fused H and E.  Tiles of
-//                64 columns x TY rows x KC planes; waves 0..TY+1 own one row each (rows jb-1 and
-//                jb+TY are halo rows), wave TY+2 covers the halo columns ib-1 and ib+64.  Each
-//                thread stands for one cell (i, j) and marches k up the chunk: E(k) staged in LDS,
-//                barrier, H(k) at n+1/2 for rows jb-1..jb+TY-1 and columns ib-1..ib+63 into LDS
-//                (own cells also to HBM), barrier, E(k) at n+1 for own cells.  H of halo cells in C
-//                is recomputed from the old state; halo cells in S take the new H written by A.
-//                No CPML code in this kernel: its loop only streams fields, the material byte and
-//                the next plane's prefetch.
-//  C  k_slab_e   one thread per E cell of S: main update, TF/SF, CPML Y-Z-X with psi, reading the
-//                new H of A and B.
+//  A  k_slab_h  one thread per H cell of the y and z CPML slabs (rows j < P or j >= J-P of every
+//               plane, and the planes k < P or k >= K-P): main update, TF/SF, CPML Y-X-Z with psi,
+//               in the reference order.  Further workgroups of the same launch fill the x-face
+//               sheets of the TF/SF box for this time step.
+//  B  k_rows    every other row: j in [P, J-P) of the planes k in [P, K-P), all columns i (the
+//               x CPML slabs included).  One wave per aligned row segment of 64 CPL columns, CPL
+//               consecutive columns per lane, marching k up a chunk of planes; fused H and E.
+//  C  k_slab_e  one thread per E cell of the y and z slabs: main update, TF/SF, CPML Y-Z-X with
+//               psi, reading the new H of A and B.
 // All three apply per cell exactly the operations of the per-op kernels, in the same order.
 // ------------------------------------------------------------------------------------------
 struct StepArgs {
   int I, J, K, P;
   int i0, i1, j0, j1, k0, k1;
-  int nTx, nTy, KC, ntab;
-  int act_h, act_e, N1, pad;
+  int S, R, CPL, nTy;  // row segments per row, rows per workgroup, columns per lane, row tiles
+  int KC, ntab, act_h, act_e;
+  int N1, xnj, xnk, pad;
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -590,35 +587,26 @@ struct StepArgs {
   const float *prm;    // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
   const float *pwtab;  // E tables x(int,half) y(int,half) z(int,half), then the same for H
   const float *Eh, *Hh;  // 1D line histories [T][N1]
+  const float *xs;       // x-face sheets of the TF/SF box, 8 x [xnk][xnj]
   float Chh, Che, dx;
   float polE[3], polH[3];
 };
 
 // Pointers taken from the argument block are generic; casting them to the global address space
 // lets the compiler emit global_load/store (counted by vmcnt only) instead of flat accesses,
-// which also count against lgkmcnt and would drain the prefetched loads at every LDS wait.
+// which also count against lgkmcnt.
 template <typename T>
 __device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p) {
   return (__attribute__((address_space(1))) T *)p;
 }
 
-// Workgroup barrier ordering LDS only: global loads in flight (the next plane's prefetch) are not
-// waited for, unlike __syncthreads(), whose fence also covers global memory.
+// Workgroup barrier ordering LDS only: global loads in flight are not waited for, unlike
+// __syncthreads(), whose fence also covers global memory.
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-
-// psi groups of one set: x slabs [Eyx Ezx Hyx Hzx], y slabs [Exy Ezy Hxy Hzy], z slabs [Exz Eyz Hxz Hyz]
-struct PsiView {
-  __attribute__((address_space(1))) const float *c;
-  __attribute__((address_space(1))) float *n;
-  size_t sx, sy, sz;
-  __device__ __forceinline__ size_t off_x(int q) const { return (size_t)q * sx; }
-  __device__ __forceinline__ size_t off_y(int q) const { return 4 * sx + (size_t)q * sy; }
-  __device__ __forceinline__ size_t off_z(int q) const { return 4 * sx + 4 * sy + (size_t)q * sz; }
-};
 
 __device__ __forceinline__ float line_at_g(const float *rowp, float s) {
   const auto row = gptr(rowp);
@@ -644,30 +632,13 @@ __device__ __forceinline__ float tab_h(const StepArgs &a, const float *row, int 
   return a.polH[c] * line_at_g(row, (sx + sy) + sz);
 }
 
-struct F3 {
-  float x, y, z;
-};
-
-// TF/SF corrections of an H cell (faces x, y, z in order), generic form of src/gpu.h:152-573.
-// Out of line: only TF/SF surface cells take this path.
-__device__ __noinline__ F3 fused_pw_h(const StepArgs *__restrict__ pa, int n, int i, int j, int k, F3 hv) {
-  const StepArgs &a = *pa;
-  const float *row = a.Eh + (size_t)n * a.N1;
-  float hx = hv.x, hy = hv.y, hz = hv.z;
+// TF/SF corrections of an H cell on the y and z faces (in that order), generic form of
+// src/gpu.h:152-573.  The x-face corrections come first and are applied by the caller from the
+// x-face sheets.
+__device__ __forceinline__ void pw_h_yz(const StepArgs &a, const float *row, int i, int j, int k, float &hx, float &hy,
+                                        float &hz) {
   const int m = a.act_h;
   const float Che = a.Che;
-  if (i == a.i0 - 1 || i == a.i1) {
-    const bool lo = i == a.i0 - 1;
-    const int xe = lo ? a.i0 : a.i1;
-    if ((m & (1 << 1)) && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1 - 1) {
-      const float v = Che * tab_e(a, row, 2, xe, j, k);
-      hy = lo ? hy - v : hy + v;
-    }
-    if ((m & (1 << 2)) && j >= a.j0 && j <= a.j1 - 1 && k >= a.k0 && k <= a.k1) {
-      const float v = Che * tab_e(a, row, 1, xe, j, k);
-      hz = lo ? hz + v : hz - v;
-    }
-  }
   if (j == a.j0 - 1 || j == a.j1) {
     const bool lo = j == a.j0 - 1;
     const int ye = lo ? a.j0 : a.j1;
@@ -692,27 +663,11 @@ __device__ __noinline__ F3 fused_pw_h(const StepArgs *__restrict__ pa, int n, in
       hy = lo ? hy + v : hy - v;
     }
   }
-  return F3{hx, hy, hz};
 }
 
-__device__ __noinline__ F3 fused_pw_e(const StepArgs *__restrict__ pa, int n, F3 chv, int i, int j, int k, F3 ev) {
-  const StepArgs &a = *pa;
-  const float *row = a.Hh + (size_t)n * a.N1;
-  const float ch[3] = {chv.x, chv.y, chv.z};
-  float ex = ev.x, ey = ev.y, ez = ev.z;
+__device__ __forceinline__ void pw_e_yz(const StepArgs &a, const float *row, const float ch[3], int i, int j, int k,
+                                        float &ex, float &ey, float &ez) {
   const int m = a.act_e;
-  if (i == a.i0 || i == a.i1) {
-    const bool lo = i == a.i0;
-    const int xh = lo ? a.i0 - 1 : a.i1;
-    if ((m & (1 << 2)) && j >= a.j0 && j <= a.j1 && k >= a.k0 && k <= a.k1 - 1) {
-      const float v = ch[2] * tab_h(a, row, 1, xh, j, k);
-      ez = lo ? ez - v : ez + v;
-    }
-    if ((m & (1 << 1)) && j >= a.j0 && j <= a.j1 - 1 && k >= a.k0 && k <= a.k1) {
-      const float v = ch[1] * tab_h(a, row, 2, xh, j, k);
-      ey = lo ? ey + v : ey - v;
-    }
-  }
   if (j == a.j0 || j == a.j1) {
     const bool lo = j == a.j0;
     const int yh = lo ? a.j0 - 1 : a.j1;
@@ -737,183 +692,466 @@ __device__ __noinline__ F3 fused_pw_e(const StepArgs *__restrict__ pa, int n, F3
       ey = lo ? ey - v : ey + v;
     }
   }
-  return F3{ex, ey, ez};
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range
-// of tiles (neighbouring tiles of one z chunk share halo rows and columns in that XCD's L2).
+// of tiles (neighbouring row tiles of one z chunk share their halo rows in that XCD's L2).
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
   const int per = nb / 8, rem = nb % 8;
   const int x = b % 8, q = b / 8;
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
+// Byte-offset addressing of the row kernel: every access is a uniform base pointer (held in
+// scalar registers) plus the lane's byte offset, which is 32-bit when a field component spans
+// less than 4 GiB, so that each access needs one vector register for its address.
+template <bool WIDE>
+struct ByteOff {
+  using T = uint32_t;
+};
+template <>
+struct ByteOff<true> {
+  using T = uint64_t;
+};
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+#define AS1 __attribute__((address_space(1)))
 
-// Interior kernel: one wave per grid row segment, barrier-free.  Lane l of a wave stands for column
-// i = P - 1 + 62 tx + l of row j; lanes 1..62 own their cells, lane 0 supplies H at i-1 and lane 63
-// supplies E at i+1 through cross-lane shuffles.  Each lane marches k up the chunk and per plane
-// computes H at (i, j) and Hx, Hz at (i, j-1) (the row below belongs to another wave, so those two
-// components are recomputed from the old state), then E at (i, j).  H at (i, j, k-1) comes from
-// the previous plane in registers.  H of cells in the CPML slabs (outside C) is not recomputed: it
-// is read from the new state written by the slab-H kernel.
-constexpr int kOwnCols = 62;
-
-template <bool IDX>
-__global__ void __launch_bounds__(256) k_interior(const StepArgs *__restrict__ pa, int n) {
-  const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
-  const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
-  const int nTx = pa->nTx, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab, act_h = pa->act_h, act_e = pa->act_e;
-  const size_t IJK = pa->IJK, cs = pa->cs;
-  const auto Fc = gptr(pa->Fc);
-  const auto Fn = gptr(pa->Fn);
-  const auto mat = gptr(pa->mat);
-  const auto coef = gptr(pa->coef);
-  const float Chh = pa->Chh, Che = pa->Che;
-  __shared__ float sCoef[IDX ? 256 * 6 : 6];
-  if (IDX)
-    for (int q = threadIdx.x; q < ntab * 6; q += 256) sCoef[q] = coef[q];
-  __syncthreads();
-
-  const size_t IJ = (size_t)I * J;
-  const int cx1 = I - P, cy1 = J - P, cz1 = K - P;  // C = [P, cx1) x [P, cy1) x [P, cz1)
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int tx = L % nTx, t2 = L / nTx;
-  const int ty = t2 % nTy, tz = t2 / nTy;
-  const int i = P - 1 + kOwnCols * tx + lane;
-  const int j = P + 4 * ty + w;
-  const int kb = P + tz * KC, ke = min(kb + KC, cz1);
-  if (j >= cy1) return;  // whole wave: no barrier follows
-  const bool inI = i < I;
-  const bool iC = (i >= P) && (i < cx1);          // column inside C
-  const bool own = iC && lane >= 1 && lane <= kOwnCols;
-  const bool hlane = lane <= kOwnCols && inI;      // lanes whose H feeds an own E
-  const bool hC = hlane && iC;                     // H recomputed here
-  const bool jm_C = (j - 1) >= P;                  // row j-1 inside C
-
-  const auto Ex = Fc, Ey = Fc + cs, Ez = Fc + 2 * cs;
-  const auto Hx = Fc + 3 * cs, Hy = Fc + 4 * cs, Hz = Fc + 5 * cs;
-  const auto Nx = Fn + 3 * cs, Ny = Fn + 4 * cs, Nz = Fn + 5 * cs;
-  const size_t col = inI ? (size_t)i + (size_t)j * I : (size_t)j * I;
-
-  int k = kb > P ? kb - 1 : kb;
-  // E of row j at plane k, and Ey of row j-1 at plane k (carried from plane to plane)
-  float ex = 0, ey = 0, ez = 0, eym = 0;
-  float hpx = 0, hpy = 0;  // H(i, j, k-1) of own cells
-  {
-    const size_t id = col + (size_t)k * IJ;
-    if (inI) {
-      ex = Ex[id]; ey = Ey[id]; ez = Ez[id];
-      eym = Ey[id - I];
-    }
-    if (own && kb == P) {  // plane P-1 is a slab plane: its new H is already written
-      hpx = Nx[id - IJ];
-      hpy = Ny[id - IJ];
-    }
+template <typename O>
+__device__ __forceinline__ float ldo(const float *base, O off) {
+  return *(const AS1 float *)((const AS1 char *)base + off);
+}
+template <typename O>
+__device__ __forceinline__ void sto(float *base, O off, float v) {
+  *(AS1 float *)((AS1 char *)base + off) = v;
+}
+// CPL consecutive floats at a 4*CPL-byte aligned offset, one vector access
+template <int CPL, typename O>
+__device__ __forceinline__ void ldv(const float *base, O off, float (&v)[CPL]) {
+  const AS1 char *p = (const AS1 char *)base + off;
+  if constexpr (CPL == 4) {
+    const f4v t = *(const AS1 f4v *)p;
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  } else if constexpr (CPL == 2) {
+    const f2v t = *(const AS1 f2v *)p;
+    v[0] = t.x; v[1] = t.y;
+  } else {
+    v[0] = *(const AS1 float *)p;
   }
-  for (; k < ke; ++k) {
-    const size_t id = col + (size_t)k * IJ;
-    const bool own_plane = k >= kb;
-    // ---- loads of this plane (independent, issued together) ----
-    float nx = 0, ny = 0, nz = 0, nym = 0;       // E(k+1) row j, Ey(k+1) row j-1
-    float exm = 0, ezm = 0, exp_ = 0, ezp = 0;   // Ex, Ez of rows j-1 and j+1
-    float hx = 0, hy = 0, hz = 0, hxm = 0, hzm = 0;
-    int mt = 0;
-    if (inI) {
-      nx = Ex[id + IJ]; ny = Ey[id + IJ]; nz = Ez[id + IJ];
-      nym = Ey[id + IJ - I];
-      exm = Ex[id - I]; ezm = Ez[id - I];
-      exp_ = Ex[id + I]; ezp = Ez[id + I];
-    }
-    if (hC) { hx = Hx[id]; hy = Hy[id]; hz = Hz[id]; }
-    else if (hlane) { hx = Nx[id]; hy = Ny[id]; hz = Nz[id]; }
-    if (hC && jm_C) { hxm = Hx[id - I]; hzm = Hz[id - I]; }
-    else if (hlane) { hxm = Nx[id - I]; hzm = Nz[id - I]; }
-    if (IDX && own && own_plane) mt = mat[id];
-    // ---- x neighbours through the wave ----
-    const float ey_i1 = __shfl_down(ey, 1), ez_i1 = __shfl_down(ez, 1), eym_i1 = __shfl_down(eym, 1);
-    // ---- H(i, j, k) and Hx, Hz at (i, j-1, k): K1 GPU_updateH, src/gpu.h:592-601 ----
-    // (C cells satisfy its mask; 1/kappa is exactly 1 in C, so the x * 1.0f factors are omitted)
-    if (hC) {
-      hx = Chh * hx + Che * (-(ezp - ez) + (ny - ey));
-      hy = Chh * hy + Che * (-(nx - ex) + (ez_i1 - ez));
-      hz = Chh * hz + Che * (-(ey_i1 - ey) + (exp_ - ex));
-      if (act_h && i >= bi0 - 1 && i <= bi1 && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1 &&
-          (i == bi0 - 1 || i == bi1 || j == bj0 - 1 || j == bj1 || k == bk0 - 1 || k == bk1)) {
-        const F3 o = fused_pw_h(pa, n, i, j, k, F3{hx, hy, hz});
-        hx = o.x; hy = o.y; hz = o.z;
-      }
-      if (jm_C) {
-        const int jm = j - 1;
-        hxm = Chh * hxm + Che * (-(ez - ezm) + (nym - eym));
-        hzm = Chh * hzm + Che * (-(eym_i1 - eym) + (ex - exm));
-        if (act_h && i >= bi0 - 1 && i <= bi1 && jm >= bj0 - 1 && jm <= bj1 && k >= bk0 - 1 && k <= bk1 &&
-            (i == bi0 - 1 || i == bi1 || jm == bj0 - 1 || jm == bj1 || k == bk0 - 1 || k == bk1)) {
-          const F3 o = fused_pw_h(pa, n, i, jm, k, F3{hxm, 0.0f, hzm});
-          hxm = o.x; hzm = o.z;
-        }
-      }
-    }
-    const float hy_im = __shfl_up(hy, 1), hz_im = __shfl_up(hz, 1);
-    if (own && own_plane) {
-      Nx[id] = hx;
-      Ny[id] = hy;
-      Nz[id] = hz;
-      float ce[3], ch[3];
-      if (IDX) {
-        const float *row = sCoef + 6 * mt;
-        ce[0] = row[0]; ce[1] = row[1]; ce[2] = row[2];
-        ch[0] = row[3]; ch[1] = row[4]; ch[2] = row[5];  // C cells are never on the last x plane
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { ce[c] = coef[c * IJK + id]; ch[c] = coef[(3 + c) * IJK + id]; }
-      }
-      // K3 GPU_updateE, src/gpu.h:745-762 (C cells satisfy its masks; 1/kappa = 1 in C)
-      float ux = ce[0] * ex + ch[0] * ((hz - hzm) - (hy - hpy));
-      float uy = ce[1] * ey + ch[1] * ((hx - hpx) - (hz - hz_im));
-      float uz = ce[2] * ez + ch[2] * ((hy - hy_im) - (hx - hxm));
-      if (act_e && i >= bi0 && i <= bi1 && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1 &&
-          (i == bi0 || i == bi1 || j == bj0 || j == bj1 || k == bk0 || k == bk1)) {
-        const F3 o = fused_pw_e(pa, n, F3{ch[0], ch[1], ch[2]}, i, j, k, F3{ux, uy, uz});
-        ux = o.x; uy = o.y; uz = o.z;
-      }
-      Fn[id] = ux;
-      Fn[cs + id] = uy;
-      Fn[2 * cs + id] = uz;
-    }
-    hpx = hx;
-    hpy = hy;
-    ex = nx; ey = ny; ez = nz; eym = nym;
+}
+template <int CPL, typename O>
+__device__ __forceinline__ void stv(float *base, O off, const float (&v)[CPL]) {
+  AS1 char *p = (AS1 char *)base + off;
+  if constexpr (CPL == 4) {
+    *(AS1 f4v *)p = f4v{v[0], v[1], v[2], v[3]};
+  } else if constexpr (CPL == 2) {
+    *(AS1 f2v *)p = f2v{v[0], v[1]};
+  } else {
+    *(AS1 float *)p = v[0];
+  }
+}
+// CPL consecutive material bytes
+template <int CPL, typename O>
+__device__ __forceinline__ void ldm(const uint8_t *base, O off, int (&m)[CPL]) {
+  const AS1 uint8_t *p = (const AS1 uint8_t *)base + off;
+  if constexpr (CPL == 4) {
+    const uint32_t w = *(const AS1 uint32_t *)p;
+    m[0] = w & 255; m[1] = (w >> 8) & 255; m[2] = (w >> 16) & 255; m[3] = w >> 24;
+  } else if constexpr (CPL == 2) {
+    const uint32_t w = *(const AS1 uint16_t *)p;
+    m[0] = w & 255; m[1] = w >> 8;
+  } else {
+    m[0] = *p;
   }
 }
 
-// Decode thread t of the slab kernels into a cell of S (z slabs, then y slabs of the remaining
-// planes, then x slabs of the remaining rows).
-__device__ __forceinline__ bool slab_cell(long t, int I, int J, int K, int P, int &i, int &j, int &k) {
-  const long nz = 2L * P * I * J, ny = 2L * P * I * (K - 2 * P), nx = 2L * P * (J - 2 * P) * (K - 2 * P);
+// x-face sheets: the incident values the x faces of the TF/SF box need at time step n, computed
+// once per step by the slab-H launch.  Sheet q = 4 eh + 2 hi + u over (k - k0, j - j0), j fastest:
+//   eh 0 (H update, face cells i0 - 1 / i1, incident E at x = i0 / i1):
+//     u 0: E_z for Hy (bit 1, j in [j0, j1], k in [k0, k1 - 1]); u 1: E_y for Hz (bit 2, j in
+//     [j0, j1 - 1], k in [k0, k1])
+//   eh 1 (E update, face cells i0 / i1, incident H at x = i0 - 1 / i1):
+//     u 0: H_y for Ez (bit 2, ranges of u 0 above); u 1: H_z for Ey (bit 1, ranges of u 1)
+// Only entries inside their ranges are written (and read).
+__device__ __forceinline__ void tfsf_x_sheet(const GridDev &g, const PWDev &pw, const float *rowE, const float *rowH,
+                                             float *xs, int nj, int nk, long t) {
+  const long S = (long)nj * nk;
+  const int q = (int)(t / S), r = (int)(t % S);
+  const int j = g.j0 + r % nj, k = g.k0 + r / nj;
+  const int eh = q >> 2, hi = (q >> 1) & 1, u = q & 1;
+  const bool in = u == 0 ? (k <= g.k1 - 1) : (j <= g.j1 - 1);
+  if (!in) return;
+  float v;
+  if (eh == 0) {
+    if (!(pw.active_h & (u == 0 ? 2 : 4))) return;
+    v = inc_e(pw, rowE, u == 0 ? 2 : 1, hi ? g.i1 : g.i0, j, k);
+  } else {
+    if (!(pw.active_e & (u == 0 ? 4 : 2))) return;
+    v = inc_h(pw, rowH, u == 0 ? 1 : 2, hi ? g.i1 : g.i0 - 1, j, k);
+  }
+  xs[(size_t)q * S + r] = v;
+}
+
+// Row kernel (B).  Waves are S row segments x R rows per workgroup; lane l of segment g holds the
+// columns i = 64 CPL g + CPL l + c, c < CPL, of row j and marches k through its chunk of planes.
+// Per plane it computes H(i, j, k) at n+1/2 and Hx, Hz at (i, j-1, k) (the row below belongs to
+// another wave, so those two components are recomputed from the old state; on the first row the
+// row below is a y-slab row, whose new H the slab-H kernel has written), then E(i, j, k) at n+1.
+// H at (i, j, k-1) comes from the previous plane in registers; a chunk that does not start on
+// plane P first recomputes H on the plane below it.  Column neighbours come from the lane's own
+// registers, from the next / previous lane through cross-lane shuffles and, across segments,
+// through LDS with one barrier per plane.  The lane's byte offset o addresses (column, j-1, k);
+// rows j and j+1 and plane k+1 are reached through uniform base offsets.  1/kappa_y and 1/kappa_z
+// are exactly 1 on these rows and planes and their factors are omitted; 1/kappa_x is applied in
+// the x-slab columns (it is 1, an exact no-op, elsewhere).
+template <bool IDX, bool WIDE, int CPL>
+__global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
+  using O = typename ByteOff<WIDE>::T;
+  constexpr int W = 64 * CPL;
+  const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
+  const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
+  const int S = pa->S, R = pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
+  const int act_h = pa->act_h, act_e = pa->act_e;
+  const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J;
+  const float Chh = pa->Chh, Che = pa->Che, dx = pa->dx;
+  const int P2 = 2 * P;
+
+  __shared__ float sCoef[IDX ? 256 * 6 : 6];
+  __shared__ float sSlab[6][128];   // per x-slab index s: bE cE bH cH, 1/kappa_x of E and of H
+  __shared__ float sX[2][4][5];     // per plane parity and wave: E (lane 0) ey ez eym, H (lane 63) hy hz
+  {
+    const auto prm = gptr(pa->prm);
+    if (IDX) {
+      const auto coef = gptr(pa->coef);
+      for (int q = threadIdx.x; q < ntab * 6; q += blockDim.x) sCoef[q] = coef[q];
+    }
+    for (int q = threadIdx.x; q < P2; q += blockDim.x) {
+      const int i = q < P ? q : I - P2 + q;
+      sSlab[0][q] = prm[q];
+      sSlab[1][q] = prm[P2 + q];
+      sSlab[2][q] = prm[2 * P2 + q];
+      sSlab[3][q] = prm[3 * P2 + q];
+      sSlab[4][q] = prm[4 * P2 + i];
+      sSlab[5][q] = prm[4 * P2 + I + J + K + i];
+    }
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int seg = wv % S, r = wv / S;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int ty = L % nTy, tz = L / nTy;
+  const int j = P + R * ty + r;
+  const int kb = P + tz * KC, ke = min(kb + KC, K - P);
+  const bool row_ok = j < J - P;
+  if (S == 1 && !row_ok) return;  // whole wave, and no barrier follows when S == 1
+  const int ib = seg * W + CPL * lane;  // first column of the lane
+  const bool on = row_ok && ib < I;     // I is a multiple of CPL: all CPL columns are in the grid
+  const bool jm_C = j - 1 >= P;         // row j-1 is recomputed (else: a y-slab row, read new)
+  const float *rowE = act_h ? pa->Eh + (size_t)n * pa->N1 : nullptr;
+  const float *rowH = act_e ? pa->Hh + (size_t)n * pa->N1 : nullptr;
+
+  // x-slab index of each column (-1 outside the x slabs)
+  int sl[CPL];
+  bool slab_lane = false;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int i = ib + c;
+    sl[c] = !on ? -1 : (i < P ? i : (i >= I - P ? i - (I - P2) : -1));
+    slab_lane |= sl[c] >= 0;
+  }
+
+  // x-face TF/SF columns of the lane (the box is at least 8 columns wide: at most one each)
+  int xh_c = -1, xe_c = -1;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int i = ib + c;
+    if (on && act_h && (i == bi0 - 1 || i == bi1)) xh_c = c;
+    if (on && act_e && (i == bi0 || i == bi1)) xe_c = c;
+  }
+  const bool xh_lo = xh_c >= 0 && ib + xh_c == bi0 - 1, xe_lo = xe_c >= 0 && ib + xe_c == bi0;
+  const int xnj = pa->xnj;
+  const size_t xS = (size_t)xnj * pa->xnk;
+  const float *xs = pa->xs;
+  const uint32_t xh_sel = xh_lo ? 0u : (uint32_t)(2 * xS), xe_sel = xe_lo ? (uint32_t)(4 * xS) : (uint32_t)(6 * xS);
+  const bool xj0 = j >= bj0 && j <= bj1, xj1 = j >= bj0 && j <= bj1 - 1;
+  const bool xjm1 = j - 1 >= bj0 && j - 1 <= bj1 - 1;
+
+  // uniform bases: row j-1 at +0, row j at +I, row j+1 at +2I, plane k+1 at +IJ
+  const float *Fc = pa->Fc;
+  float *Fn = pa->Fn;
+  const float *Ex = Fc, *Ey = Fc + cs, *Ez = Fc + 2 * cs;
+  const float *Hx = Fc + 3 * cs, *Hy = Fc + 4 * cs, *Hz = Fc + 5 * cs;
+  float *NEx = Fn, *NEy = Fn + cs, *NEz = Fn + 2 * cs;
+  float *NHx = Fn + 3 * cs, *NHy = Fn + 4 * cs, *NHz = Fn + 5 * cs;
+  const uint8_t *mat = pa->mat;
+  const float *coef = pa->coef;
+  // x-slab psi [Eyx Ezx Hyx Hzx], element s + 2P j + 2P J k
+  const size_t sx = pa->sx;
+  const float *qEy = pa->Pc, *qEz = pa->Pc + sx, *qHy = pa->Pc + 2 * sx, *qHz = pa->Pc + 3 * sx;
+  float *wEy = pa->Pn, *wEz = pa->Pn + sx, *wHy = pa->Pn + 2 * sx, *wHz = pa->Pn + 3 * sx;
+
+  int k = kb > P ? kb - 1 : kb;
+  const size_t col = (on ? (size_t)ib : 0) + (size_t)(j - 1) * I;
+  O o = (O)((col + (size_t)k * IJ) * 4);
+  const O dk = (O)(IJ * 4);
+  uint32_t so[CPL];  // psi byte offsets of the columns within a (row, plane)
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) so[c] = (uint32_t)((sl[c] < 0 ? 0 : sl[c]) * 4);
+
+  // E of row j at plane k, and Ey of row j-1 at plane k (carried from plane to plane)
+  float ex[CPL], ey[CPL], ez[CPL], eym[CPL];
+  float hpx[CPL], hpy[CPL];  // H(i, j, k-1)
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) ex[c] = ey[c] = ez[c] = eym[c] = hpx[c] = hpy[c] = 0.0f;
+  if (on) {
+    ldv(Ex + I, o, ex); ldv(Ey + I, o, ey); ldv(Ez + I, o, ez);
+    ldv(Ey, o, eym);
+    if (kb == P) {  // plane P-1 is a z-slab plane: its new H is already written
+      ldv(NHx + I, o - dk, hpx);
+      ldv(NHy + I, o - dk, hpy);
+    }
+  }
+  // E at the first column of the next segment (plane k), for the lane's last column
+  float bey = 0, bez = 0, beym = 0;
+  int par = 0;
+  if (S > 1) {
+    if (lane == 0) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
+    lds_barrier();
+    if (lane == 63 && seg + 1 < S) { bey = sX[0][wv + 1][0]; bez = sX[0][wv + 1][1]; beym = sX[0][wv + 1][2]; }
+    par = 1;
+  }
+
+  for (; k < ke; ++k, o += dk) {
+    const bool own = k >= kb;  // planes of the chunk (the plane below only feeds H(k-1))
+    const size_t pq = (size_t)P2 * ((size_t)j + (size_t)J * k);
+    // ---- loads of this plane (independent, issued together) ----
+    float nx[CPL], ny[CPL], nz[CPL], nym[CPL];        // E(k+1) row j, Ey(k+1) row j-1
+    float exm[CPL], ezm[CPL], exq[CPL], ezq[CPL];     // Ex, Ez of rows j-1 and j+1
+    float hx[CPL], hy[CPL], hz[CPL], hxm[CPL], hzm[CPL];
+    float pHy[CPL], pHz[CPL], pHzm[CPL], pEy[CPL], pEz[CPL];
+    int mt[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      nx[c] = ny[c] = nz[c] = nym[c] = exm[c] = ezm[c] = exq[c] = ezq[c] = 0.0f;
+      hx[c] = hy[c] = hz[c] = hxm[c] = hzm[c] = 0.0f;
+      pHy[c] = pHz[c] = pHzm[c] = pEy[c] = pEz[c] = 0.0f;
+      mt[c] = 0;
+    }
+    if (on) {
+      ldv(Ex + I + IJ, o, nx); ldv(Ey + I + IJ, o, ny); ldv(Ez + I + IJ, o, nz);
+      ldv(Ey + IJ, o, nym);
+      ldv(Ex, o, exm); ldv(Ez, o, ezm);
+      ldv(Ex + 2 * I, o, exq); ldv(Ez + 2 * I, o, ezq);
+      ldv(Hx + I, o, hx); ldv(Hy + I, o, hy); ldv(Hz + I, o, hz);
+      if (jm_C) { ldv(Hx, o, hxm); ldv(Hz, o, hzm); }
+      else { ldv(NHx, o, hxm); ldv(NHz, o, hzm); }
+      if (IDX && own) ldm(mat + I, o >> 2, mt);
+      if (slab_lane) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          if (sl[c] >= 0) {
+            pHy[c] = ldo(qHy + pq, so[c]);
+            pHz[c] = ldo(qHz + pq, so[c]);
+            if (jm_C) pHzm[c] = ldo(qHz + pq - P2, so[c]);
+            if (own) { pEy[c] = ldo(qEy + pq, so[c]); pEz[c] = ldo(qEz + pq, so[c]); }
+          }
+      }
+    }
+    // x-face incident values of this plane
+    float vhy = 0, vhz = 0, vhzm = 0, vez = 0, vey = 0;
+    const uint32_t xo = (uint32_t)((k - bk0) * xnj + (j - bj0));
+    const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
+    const bool dhy = xh_c >= 0 && (act_h & 2) && xj0 && xk0;
+    const bool dhz = xh_c >= 0 && (act_h & 4) && xj1 && xk1;
+    const bool dhzm = xh_c >= 0 && jm_C && (act_h & 4) && xjm1 && xk1;
+    const bool dez = xe_c >= 0 && own && (act_e & 4) && xj0 && xk0;
+    const bool dey = xe_c >= 0 && own && (act_e & 2) && xj1 && xk1;
+    if (dhy) vhy = ldo(xs, (xh_sel + xo) * 4u);
+    if (dhz) vhz = ldo(xs + xS, (xh_sel + xo) * 4u);
+    if (dhzm) vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
+    if (dez) vez = ldo(xs, (xe_sel + xo) * 4u);
+    if (dey) vey = ldo(xs + xS, (xe_sel + xo) * 4u);
+
+    // ---- E at i+1 of the lane's last column ----
+    float ey_n = __shfl_down(ey[0], 1), ez_n = __shfl_down(ez[0], 1), eym_n = __shfl_down(eym[0], 1);
+    if (S > 1 && lane == 63) { ey_n = bey; ez_n = bez; eym_n = beym; }
+
+    // ---- H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML X (src/gpu.h:604-725) ----
+    const bool yz_h = act_h && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1 &&
+                      (j == bj0 - 1 || j == bj1 || k == bk0 - 1 || k == bk1);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const int i = ib + c;
+      const float ey1 = c + 1 < CPL ? ey[c + 1 < CPL ? c + 1 : 0] : ey_n;
+      const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
+      const int s = sl[c];
+      const float kx = s >= 0 ? sSlab[5][s] : 1.0f;
+      if (on && i < I - 1) {
+        hx[c] = Chh * hx[c] + Che * (-(ezq[c] - ez[c]) + (ny[c] - ey[c]));
+        hy[c] = Chh * hy[c] + Che * (-(nx[c] - ex[c]) + (ez1 - ez[c]) * kx);
+        hz[c] = Chh * hz[c] + Che * (-(ey1 - ey[c]) * kx + (exq[c] - ex[c]));
+      }
+      if (c == xh_c) {
+        if (dhy) { const float v = Che * vhy; hy[c] = xh_lo ? hy[c] - v : hy[c] + v; }
+        if (dhz) { const float v = Che * vhz; hz[c] = xh_lo ? hz[c] + v : hz[c] - v; }
+      }
+      if (yz_h && on && i >= bi0 - 1 && i <= bi1) pw_h_yz(*pa, rowE, i, j, k, hx[c], hy[c], hz[c]);
+      if (s >= 0 && s < P2 - 1) {
+        const float bH = sSlab[2][s], cH = sSlab[3][s];
+        const float pyx = bH * pHy[c] + cH * (ez1 - ez[c]) / dx;
+        hy[c] = hy[c] + Che * pyx * dx;
+        const float pzx = bH * pHz[c] + cH * (ey1 - ey[c]) / dx;
+        hz[c] = hz[c] - Che * pzx * dx;
+        pHy[c] = pyx;
+        pHz[c] = pzx;
+      }
+    }
+    // ---- Hx, Hz at (i, j-1, k), recomputed the same way ----
+    if (jm_C) {
+      const int jm = j - 1;
+      const bool yz_hm = act_h && jm >= bj0 - 1 && jm <= bj1 && k >= bk0 - 1 && k <= bk1 &&
+                         (jm == bj0 - 1 || jm == bj1 || k == bk0 - 1 || k == bk1);
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        const int i = ib + c;
+        const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
+        const int s = sl[c];
+        const float kx = s >= 0 ? sSlab[5][s] : 1.0f;
+        if (on && i < I - 1) {
+          hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - ezm[c]) + (nym[c] - eym[c]));
+          hzm[c] = Chh * hzm[c] + Che * (-(eym1 - eym[c]) * kx + (ex[c] - exm[c]));
+        }
+        if (c == xh_c && dhzm) { const float v = Che * vhzm; hzm[c] = xh_lo ? hzm[c] + v : hzm[c] - v; }
+        if (yz_hm && on && i >= bi0 - 1 && i <= bi1) {
+          float dummy = 0.0f;
+          pw_h_yz(*pa, rowE, i, jm, k, hxm[c], dummy, hzm[c]);
+        }
+        if (s >= 0 && s < P2 - 1) {
+          const float pzx = sSlab[2][s] * pHzm[c] + sSlab[3][s] * (eym1 - eym[c]) / dx;
+          hzm[c] = hzm[c] - Che * pzx * dx;
+        }
+      }
+    }
+
+    // ---- H at i-1 of the lane's first column; segment exchange ----
+    float hy_p = __shfl_up(hy[CPL - 1], 1), hz_p = __shfl_up(hz[CPL - 1], 1);
+    if (S > 1) {
+      if (lane == 63) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
+      if (lane == 0) { sX[par][wv][0] = ny[0]; sX[par][wv][1] = nz[0]; sX[par][wv][2] = nym[0]; }
+      lds_barrier();
+      if (lane == 0 && seg > 0) { hy_p = sX[par][wv - 1][3]; hz_p = sX[par][wv - 1][4]; }
+      if (lane == 63 && seg + 1 < S) { bey = sX[par][wv + 1][0]; bez = sX[par][wv + 1][1]; beym = sX[par][wv + 1][2]; }
+      par ^= 1;
+    }
+
+    if (on && own) {
+      stv(NHx + I, o, hx);
+      stv(NHy + I, o, hy);
+      stv(NHz + I, o, hz);
+      if (slab_lane) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          if (sl[c] >= 0) {
+            sto(wHy + pq, so[c], pHy[c]);
+            sto(wHz + pq, so[c], pHz[c]);
+          }
+      }
+      // ---- E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML X (src/gpu.h:765-868) ----
+      float ce[3][CPL], ch[3][CPL];
+      if (IDX) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const float *rw = sCoef + 6 * mt[c];
+          const bool last = ib + c == I - 1;  // den_ex = 0 on the last x plane
+          ce[0][c] = rw[0]; ce[1][c] = rw[1]; ce[2][c] = rw[2];
+          ch[0][c] = last ? 0.0f : rw[3];
+          ch[1][c] = last ? 0.0f : rw[4];
+          ch[2][c] = last ? 0.0f : rw[5];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          ldv(coef + q * IJK + I, o, ce[q]);
+          ldv(coef + (3 + q) * IJK + I, o, ch[q]);
+        }
+      }
+      const bool yz_e = act_e && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1 &&
+                        (j == bj0 || j == bj1 || k == bk0 || k == bk1);
+      float ux[CPL], uy[CPL], uz[CPL];
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) {
+        const int i = ib + c;
+        const float hym1 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_p;
+        const float hzm1 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_p;
+        const int s = sl[c];
+        const float kx = s >= 0 ? sSlab[4][s] : 1.0f;
+        ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) - (hy[c] - hpy[c]));
+        uy[c] = ey[c];
+        uz[c] = ez[c];
+        if (i != 0) {
+          uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) - (hz[c] - hzm1) * kx);
+          uz[c] = ce[2][c] * ez[c] + ch[2][c] * ((hy[c] - hym1) * kx - (hx[c] - hxm[c]));
+        }
+        if (c == xe_c) {
+          if (dez) { const float v = ch[2][c] * vez; uz[c] = xe_lo ? uz[c] - v : uz[c] + v; }
+          if (dey) { const float v = ch[1][c] * vey; uy[c] = xe_lo ? uy[c] + v : uy[c] - v; }
+        }
+        if (yz_e && i >= bi0 && i <= bi1) {
+          const float chc[3] = {ch[0][c], ch[1][c], ch[2][c]};
+          pw_e_yz(*pa, rowH, chc, i, j, k, ux[c], uy[c], uz[c]);
+        }
+        if (s >= 1) {
+          const float bE = sSlab[0][s], cE = sSlab[1][s];
+          const float pyx = bE * pEy[c] + cE * (hz[c] - hzm1) / dx;
+          uy[c] = uy[c] - ch[1][c] * pyx * dx;
+          const float pzx = bE * pEz[c] + cE * (hy[c] - hym1) / dx;
+          uz[c] = uz[c] + ch[2][c] * pzx * dx;
+          pEy[c] = pyx;
+          pEz[c] = pzx;
+        }
+      }
+      stv(NEx + I, o, ux);
+      stv(NEy + I, o, uy);
+      stv(NEz + I, o, uz);
+      if (slab_lane) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          if (sl[c] >= 0) {
+            sto(wEy + pq, so[c], pEy[c]);
+            sto(wEz + pq, so[c], pEz[c]);
+          }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      hpx[c] = hx[c];
+      hpy[c] = hy[c];
+      ex[c] = nx[c]; ey[c] = ny[c]; ez[c] = nz[c]; eym[c] = nym[c];
+    }
+  }
+}
+
+// Decode thread t of the slab kernels into a cell of the y and z slabs (z slabs, then the y slabs
+// of the remaining planes).  32-bit arithmetic when the slab cell count allows it.
+template <typename IT>
+__device__ __forceinline__ bool slab_cell(IT t, int I, int J, int K, int P, int &i, int &j, int &k) {
+  const IT IJ = (IT)I * J;
+  const IT nz = (IT)2 * P * IJ, ny = (IT)2 * P * I * (K - 2 * P);
   if (t < nz) {
-    const long pl = t / ((long)I * J), rem = t % ((long)I * J);
-    k = pl < P ? (int)pl : K - 2 * P + (int)pl;
+    const IT pl = t / IJ, rem = t - pl * IJ;
+    k = pl < (IT)P ? (int)pl : K - 2 * P + (int)pl;
     j = (int)(rem / I);
-    i = (int)(rem % I);
+    i = (int)(rem - (IT)j * I);
     return true;
   }
   t -= nz;
   if (t < ny) {
-    const long q = t / I;
-    i = (int)(t % I);
+    const IT q = t / I;
+    i = (int)(t - q * I);
     const int js = (int)(q % (2 * P));
     k = P + (int)(q / (2 * P));
     j = js < P ? js : J - 2 * P + js;
-    return true;
-  }
-  t -= ny;
-  if (t < nx) {
-    const long q = t / (2 * P);
-    const int is = (int)(t % (2 * P));
-    i = is < P ? is : I - 2 * P + is;
-    j = P + (int)(q % (J - 2 * P));
-    k = P + (int)(q / (J - 2 * P));
     return true;
   }
   return false;
@@ -926,12 +1164,25 @@ struct SlabArgs {
   CoefDev co;
   PWDev pw;
   int n;
+  unsigned nbs;  // workgroups of the slab cells; k_slab_h's further workgroups fill the x-face sheets
+  float *xs;
+  int xnj, xnk;
 };
 
-// A: H cells of the CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
+// A: H cells of the y and z CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
+template <typename IT>
 __global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
+  if (blockIdx.x >= a.nbs) {
+    const long t = (long)(blockIdx.x - a.nbs) * 256 + threadIdx.x;
+    if (t < 8L * a.xnj * a.xnk) {
+      const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
+      const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
+      tfsf_x_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xnj, a.xnk, t);
+    }
+    return;
+  }
   int i, j, k;
-  if (!slab_cell((long)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  if (!slab_cell<IT>((IT)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
   const int I = a.g.I, J = a.g.J, K = a.g.K;
   const size_t IJ = (size_t)I * J;
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
@@ -953,12 +1204,12 @@ __global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
   a.nxt.f[HZ][id] = hz;
 }
 
-// C: E cells of the CPML slabs (src/gpu.h:727-763 + plane wave + src/gpu.h:765-868), reading the
-// new H of kernels A and B.
-template <bool IDX>
+// C: E cells of the y and z CPML slabs (src/gpu.h:727-763 + plane wave + src/gpu.h:765-868),
+// reading the new H of kernels A and B.
+template <bool IDX, typename IT>
 __global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
   int i, j, k;
-  if (!slab_cell((long)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  if (!slab_cell<IT>((IT)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
   const int I = a.g.I, J = a.g.J;
   const size_t IJ = (size_t)I * J;
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
@@ -1076,50 +1327,101 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
   return hipGetLastError();
 }
 
-bool fused_supported(const GridDev &g) { return g.I >= 2 * g.P + 2 && g.J >= 2 * g.P + 2 && g.K >= 2 * g.P + 2; }
+bool fused_supported(const GridDev &g) {
+  return g.P >= 1 && g.P <= 64 && g.I >= 2 * g.P + 2 && g.J >= 2 * g.P + 2 && g.K >= 2 * g.P + 2;
+}
 
-template <bool IDX>
-static hipError_t launch_interior_t(hipStream_t st, const StepArgs *dargs, int nwg, int n) {
-  hipLaunchKernelGGL((k_interior<IDX>), dim3(nwg), dim3(256), 0, st, dargs, n);
+// Row layout of the row kernel: CPL columns per lane (I a multiple of CPL, for aligned vector
+// accesses) and S segments of 64 CPL columns per row, with the fewest idle lanes, then the fewest
+// segments; at most 4 segments (rows up to 1024 columns).  force_cpl != 0 restricts CPL.
+static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
+  int best_waste = -1;
+  for (int c : {4, 2, 1}) {
+    if (I % c || (force_cpl && c != force_cpl)) continue;
+    const int s = (I + 64 * c - 1) / (64 * c);
+    if (s > 4) continue;
+    const int waste = s * 64 * c - I;
+    if (best_waste < 0 || waste < best_waste || (waste == best_waste && s < S)) {
+      best_waste = waste;
+      cpl = c;
+      S = s;
+    }
+  }
+  return best_waste >= 0;
+}
+
+template <bool IDX, bool WIDE>
+static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const StepArgs *d, int n) {
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, WIDE, 4>), grid, blk, 0, st, d, n);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, WIDE, 2>), grid, blk, 0, st, d, n);
+  else hipLaunchKernelGGL((k_rows<IDX, WIDE, 1>), grid, blk, 0, st, d, n);
+}
+
+static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int nwg, int n) {
+  // 32-bit byte offsets reach every access when a component plus one plane and two rows stays
+  // below 4 GiB
+  const bool wide = (a.IJK + (size_t)a.I * a.J + 2 * (size_t)a.I) * 4 >= ((size_t)1 << 32);
+  const dim3 grid(nwg), blk(64 * a.S * a.R);
+  if (a.mat) {
+    if (wide) launch_rows_w<true, true>(st, a.CPL, grid, blk, dargs, n);
+    else launch_rows_w<true, false>(st, a.CPL, grid, blk, dargs, n);
+  } else {
+    if (wide) launch_rows_w<false, true>(st, a.CPL, grid, blk, dargs, n);
+    else launch_rows_w<false, false>(st, a.CPL, grid, blk, dargs, n);
+  }
   return hipGetLastError();
 }
 
 size_t step_args_size() { return sizeof(StepArgs); }
 
-// Argument block of the interior kernel (host copy, uploaded once per state parity / plane-wave
-// mode); returns the number of interior workgroups (0 if the interior is empty), -1 if unsupported.
+// Argument block of the row kernel (host copy, uploaded once per state parity / plane-wave mode);
+// returns the number of row-kernel workgroups (0 if there are no such rows), -1 if the grid is
+// not supported by the fused step.
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const GridDev &g = f.g;
   if (!fused_supported(g)) return -1;
   if (f.mat && f.ntab > 256) return -1;
-  const int ty = 4;  // rows (waves) per workgroup
-  const int cI = g.I - 2 * g.P, cJ = g.J - 2 * g.P, cK = g.K - 2 * g.P;
-  const int nTx = (cI + kOwnCols - 1) / kOwnCols, nTy = (cJ + ty - 1) / ty;
+  int cpl = 1, S = 1;
+  if (!row_layout(g.I, tune.cpl, cpl, S)) return -1;
+  // the two x faces of the TF/SF box must not share a lane (at most one face column per lane)
+  if ((f.act_h | f.act_e) && g.i1 - g.i0 < cpl) {
+    if (tune.cpl || !row_layout(g.I, 1, cpl, S)) return -1;
+  }
+  int R = 4 / S;
+  if (tune.tile_rows > 0 && tune.tile_rows < R) R = tune.tile_rows;
+  const int cJ = g.J - 2 * g.P, cK = g.K - 2 * g.P;
+  const int nTy = (cJ + R - 1) / R;
   int kc = tune.z_chunk;
   if (kc <= 0) {
-    const int want = 4096;  // workgroups per launch (several per CU)
-    int nkc = (want + nTx * nTy - 1) / (nTx * nTy);
+    // about two waves per wave slot of the chip (256 CUs x 4 SIMDs x 4 waves x 2)
+    const long want = 8192, layer = (long)nTy * R * S;
+    long nkc = (want + layer - 1) / layer;
     if (nkc < 1) nkc = 1;
-    kc = (cK + nkc - 1) / nkc;
-    if (kc < 16) kc = 16;
+    if (nkc > cK) nkc = cK;
+    kc = (int)((cK + nkc - 1) / nkc);
+    if (kc < 4) kc = cK < 4 ? cK : 4;
   }
   const int nKc = (cK + kc - 1) / kc;
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
   a.i0 = g.i0; a.i1 = g.i1; a.j0 = g.j0; a.j1 = g.j1; a.k0 = g.k0; a.k1 = g.k1;
-  a.nTx = nTx; a.nTy = nTy; a.KC = kc; a.ntab = f.ntab;
-  a.act_h = f.act_h; a.act_e = f.act_e; a.N1 = f.N1; a.pad = ty;
+  a.S = S; a.R = R; a.CPL = cpl; a.nTy = nTy;
+  a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
+  a.N1 = f.N1;
+  a.xnj = g.j1 - g.j0 + 1;
+  a.xnk = g.k1 - g.k0 + 1;
   a.IJK = (size_t)g.I * g.J * g.K;
   a.cs = f.cs;
   a.sx = (size_t)2 * g.P * g.J * g.K; a.sy = (size_t)2 * g.I * g.P * g.K; a.sz = (size_t)2 * g.I * g.J * g.P;
   a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
   a.mat = f.mat; a.coef = f.coef; a.prm = f.prm; a.pwtab = f.pwtab;
   a.Eh = f.Eh; a.Hh = f.Hh;
+  a.xs = f.xs;
   a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
   for (int c = 0; c < 3; ++c) { a.polE[c] = f.polE[c]; a.polH[c] = f.polH[c]; }
   std::memcpy(out, &a, sizeof(a));
-  return (cI > 0 && cJ > 0 && cK > 0) ? nTx * nTy * nKc : 0;
+  return (cJ > 0 && cK > 0) ? nTy * nKc : 0;
 }
 
 static SlabArgs slab_args(const FusedDev &f, int n) {
@@ -1167,24 +1469,46 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
   }
   for (int c = 0; c < 3; ++c) { pw.polE[c] = f.polE[c]; pw.polH[c] = f.polH[c]; }
   s.n = n;
+  s.nbs = 0;
+  s.xs = f.xs;
+  s.xnj = g.j1 - g.j0 + 1;
+  s.xnk = g.k1 - g.k0 + 1;
   return s;
 }
 
-// One time step: A (slab H), B (interior H and E), C (slab E).
-hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n) {
+// One time step: A (slab H + x-face sheets), B (rows), C (slab E).  ev_rows (optional): two
+// events recorded around B.
+hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
+                             hipEvent_t *ev_rows) {
   const GridDev &g = f.g;
-  const SlabArgs sa = slab_args(f, n);
-  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P) + 2L * g.P * (g.J - 2 * g.P) * (g.K - 2 * g.P);
+  SlabArgs sa = slab_args(f, n);
+  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P);
   const unsigned nb = cdiv(nslab, 256);
-  hipLaunchKernelGGL(k_slab_h, dim3(nb), dim3(256), 0, st, sa);
-  if (nwg > 0) {
-    const StepArgs &a = *(const StepArgs *)hargs;
-    const StepArgs *d = (const StepArgs *)dargs;
-    const hipError_t e = a.mat ? launch_interior_t<true>(st, d, nwg, n) : launch_interior_t<false>(st, d, nwg, n);
+  sa.nbs = nb;
+  const bool sheets = f.xs && ((f.act_h | f.act_e) & 6) && sa.xnj > 0 && sa.xnk > 0;
+  const unsigned nbx = sheets ? cdiv(8L * sa.xnj * sa.xnk, 256) : 0;
+  const bool narrow = (long)nb * 256 < (1L << 31);
+  if (narrow) hipLaunchKernelGGL(k_slab_h<uint32_t>, dim3(nb + nbx), dim3(256), 0, st, sa);
+  else hipLaunchKernelGGL(k_slab_h<uint64_t>, dim3(nb + nbx), dim3(256), 0, st, sa);
+  if (ev_rows) {
+    const hipError_t e = hipEventRecord(ev_rows[0], st);
     if (e != hipSuccess) return e;
   }
-  if (f.mat) hipLaunchKernelGGL(k_slab_e<true>, dim3(nb), dim3(256), 0, st, sa);
-  else hipLaunchKernelGGL(k_slab_e<false>, dim3(nb), dim3(256), 0, st, sa);
+  if (nwg > 0) {
+    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, nwg, n);
+    if (e != hipSuccess) return e;
+  }
+  if (ev_rows) {
+    const hipError_t e = hipEventRecord(ev_rows[1], st);
+    if (e != hipSuccess) return e;
+  }
+  if (f.mat) {
+    if (narrow) hipLaunchKernelGGL((k_slab_e<true, uint32_t>), dim3(nb), dim3(256), 0, st, sa);
+    else hipLaunchKernelGGL((k_slab_e<true, uint64_t>), dim3(nb), dim3(256), 0, st, sa);
+  } else {
+    if (narrow) hipLaunchKernelGGL((k_slab_e<false, uint32_t>), dim3(nb), dim3(256), 0, st, sa);
+    else hipLaunchKernelGGL((k_slab_e<false, uint64_t>), dim3(nb), dim3(256), 0, st, sa);
+  }
   return hipGetLastError();
 }
 

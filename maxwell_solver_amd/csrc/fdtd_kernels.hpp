@@ -54,8 +54,9 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
 
 // Fused step: reads the cur state, writes the nxt state (packed buffers).
 struct StepTuning {
-  int tile_rows = 8;  // 4, 8 or 13 rows per tile
-  int z_chunk = 0;    // 0: automatic
+  int tile_rows = 0;  // rows per workgroup of the row kernel (1, 2, 4; 0: automatic)
+  int z_chunk = 0;    // planes per chunk (0: automatic)
+  int cpl = 0;        // columns per lane (1, 2, 4; 0: automatic)
 };
 struct FusedDev {
   GridDev g;
@@ -70,6 +71,7 @@ struct FusedDev {
   const float *prm = nullptr;   // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
   const float *pwtab = nullptr; // incident-field projection tables (E then H; x,y,z; integer, half)
   const float *Eh = nullptr, *Hh = nullptr;
+  float *xs = nullptr;           // x-face sheets of the TF/SF box: 8 x (k1-k0+1) x (j1-j0+1) floats
   int N1 = 0, act_h = 0, act_e = 0;
   float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
   float Chh = 1, Che = 0, dx = 0;
@@ -78,7 +80,9 @@ struct FusedDev {
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.
 size_t step_args_size();
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out);
-hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n);
+// ev_rows (optional): two events recorded on st around the row kernel
+hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
+                             hipEvent_t *ev_rows = nullptr);
 bool fused_supported(const GridDev &g);
 
 }  // namespace fdtd

@@ -26,7 +26,7 @@ EXPORTS = [
     "fdtd_set_cpml", "fdtd_set_tfsf", "fdtd_set_plane_wave", "fdtd_set_material_index",
     "fdtd_set_probe_components", "fdtd_step", "fdtd_run", "fdtd_snapshot_count",
     "fdtd_output_snapshots", "fdtd_synchronize", "fdtd_read_array", "fdtd_prepare",
-    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_set_timing", "fdtd_kernel_time",
+    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
 ]
 
 
@@ -62,8 +62,9 @@ def lib():
             "fdtd_run": [vp, i32, i32, i32, i32, i32], "fdtd_snapshot_count": [vp, ip],
             "fdtd_output_snapshots": [vp, fp, u64], "fdtd_synchronize": [vp],
             "fdtd_read_array": [vp, i32, fp, u64p], "fdtd_prepare": [vp],
-            "fdtd_read_setup": [vp, i32, fp, u64p], "fdtd_set_tuning": [vp, i32, i32],
+            "fdtd_read_setup": [vp, i32, fp, u64p], "fdtd_set_tuning": [vp, i32, i32, i32],
             "fdtd_set_timing": [vp, i32], "fdtd_kernel_time": [vp, dp, C.POINTER(C.c_int64)],
+            "fdtd_row_kernel_time": [vp, dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -215,8 +216,8 @@ class Simulation:
     def synchronize(self):
         self._chk(lib().fdtd_synchronize(self._h))
 
-    def set_tuning(self, tile_rows=8, z_chunk=0):
-        self._chk(lib().fdtd_set_tuning(self._h, int(tile_rows), int(z_chunk)))
+    def set_tuning(self, rows_per_group=0, z_chunk=0, cols_per_lane=0):
+        self._chk(lib().fdtd_set_tuning(self._h, int(rows_per_group), int(z_chunk), int(cols_per_lane)))
 
     def set_timing(self, on=True):
         self._chk(lib().fdtd_set_timing(self._h, 1 if on else 0))
@@ -225,6 +226,12 @@ class Simulation:
         ms, n = C.c_double(), C.c_int64()
         self._chk(lib().fdtd_kernel_time(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def row_kernel_time(self):
+        """Average device time (ms) of the row kernel over the last timed run."""
+        ms = C.c_double()
+        self._chk(lib().fdtd_row_kernel_time(self._h, C.byref(ms)))
+        return ms.value
 
     # ---- readback ----
     def output_vis(self, flag):

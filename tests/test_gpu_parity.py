@@ -99,17 +99,36 @@ def test_fused_step_axis_modes(mode, cpml):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
-@pytest.mark.parametrize("tile_rows,z_chunk", [(4, 0), (8, 0), (8, 5), (4, 13)])
-def test_fused_tilings_oblique(tile_rows, z_chunk):
-    """Oblique plane wave through every tiling: halo rows/planes recomputed from old data."""
+# (I, rows per group, z chunk, columns per lane): every row layout of the row kernel -- 1, 2 and 4
+# columns per lane, one or several row segments (I = 52 with 1 column per lane and 200 with 2
+# take 1 and 2 segments, 264 with 4 takes 2, 136 with 1 takes 3), several rows per workgroup.
+@pytest.mark.parametrize("I,rows,z_chunk,cpl", [
+    (52, 0, 0, 0), (52, 1, 5, 1), (52, 2, 13, 4), (52, 4, 3, 2),
+    (136, 0, 0, 1), (200, 0, 7, 2), (264, 0, 0, 4), (264, 1, 4, 2)])
+def test_fused_tilings_oblique(I, rows, z_chunk, cpl):
+    """Oblique plane wave through every row layout: halo rows/planes recomputed from old data."""
     from maxwell_solver_amd import scenarios as S
-    sc = _scenario(I=52, J=37, K=45, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7))
+    sc = _scenario(I=I, J=37, K=45, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7))
     g, o = _backends()
     S.setup(g, sc)
     S.setup(o, sc)
-    g.set_tuning(tile_rows, z_chunk)
+    g.set_tuning(rows, z_chunk, cpl)
     g.run(0, sc.T, 5)
     o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def test_step_outside_fused_layouts():
+    """Rows longer than 1024 columns are outside the row kernel's layouts: Step and Run take the
+    per-op kernels in the reference op order instead, with the same result."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=1032, J=34, K=36, T=40, mode=3, cpml=CFG_CPML, inset=12)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, 3)
+    o.run(0, sc.T, 3)
     assert_close(compare_state(g, o), exact=True)
     np.testing.assert_array_equal(g.probes(), o.probes())
 
