@@ -52,7 +52,7 @@ def copy_calibration(nbytes=1 << 30, reps=10):
     print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": reps}), flush=True)
 
 
-def cpu_baseline(sc, budget_s=15.0, max_steps=200):
+def cpu_baseline(sc, budget_s=15.0, max_steps=1000):
     """The CPU oracle (float32, OpenMP over k planes) on the same workload, bounded to ~budget_s."""
     import numpy as np
 
@@ -132,7 +132,7 @@ def main():
     achieved = rows_bytes / (rows_ms * 1e-3) / 1e9 if rows_ms > 0 else 0.0
     step_achieved = bpc * sc.cells / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_k_rows_{sc.I}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_rows_{sc.I}.json")  # rocprofv3 FETCH/WRITE, copy-calibrated
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")
@@ -179,7 +179,7 @@ def main():
     }
     g.close()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(S.config2(T=400) if args.size == 256 else sc, budget_s=args.cpu_budget)
+        rec["cpu_baseline"] = cpu_baseline(S.config2(T=1200) if args.size == 256 else sc, budget_s=args.cpu_budget)
     print(json.dumps(rec), flush=True)
 
 

@@ -131,6 +131,10 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
  * grid's I must be a multiple of it and a row at most 256 of it).  An unsupported choice makes
  * the next step fail with FDTD_ERR_INVALID. */
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
+/* Time the fused step at time index n, plane-wave mode `mode`, for each available columns-per-lane
+ * and z-chunk choice (reps steps each) and keep the fastest (*best_ms: its average step time).
+ * The steps are run dry: the fields, psi, probe record and time step are unchanged. */
+int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms);
 /* Average device time of the fused step kernel over the last fdtd_run with timing on
  * (HIP events around each launch on the simulation's stream). */
 int fdtd_set_timing(fdtd_sim *s, int32_t on);

@@ -107,7 +107,7 @@ struct fdtd_sim {
   float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
   float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
   float *dPwTab[6] = {};      // per plane-wave mode: E tables x,y,z (int, half), then H tables
-  float *dXs = nullptr;       // x-face sheets of the TF/SF box (fused step, refilled every step)
+  float *dXs = nullptr;       // face sheets of the TF/SF box (fused step, refilled every step)
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
@@ -534,14 +534,17 @@ struct fdtd_sim {
     return 0;
   }
 
-  int step(int n, int mode, bool timed, int *ev_slot) {
+  // One fused time step.  dry: launch the kernels (they read the current set and write the next)
+  // but keep the current set and the probe recorder: the state is unchanged (used for timing).
+  int step(int n, int mode, bool timed, int *ev_slot, bool dry = false) {
     if (!init_ready) return fail(FDTD_ERR_STATE, "Step before ReadySimulation");
     const PWDev *pw = nullptr;
     if (int rc = check_pw(n, mode, &pw)) return rc;
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
-    if (pw && !dXs && g.j1 >= g.j0 && g.k1 >= g.k0) {
-      const size_t nxs = (size_t)8 * (g.j1 - g.j0 + 1) * (g.k1 - g.k0 + 1);
+    if (pw && !dXs && g.i1 >= g.i0 && g.j1 >= g.j0 && g.k1 >= g.k0) {
+      const size_t ni = g.i1 - g.i0 + 1, nj = g.j1 - g.j0 + 1, nk = g.k1 - g.k0 + 1;
+      const size_t nxs = 8 * (nj * nk + ni * nk + ni * nj);
       if (dalloc(&dXs, nxs)) return FDTD_ERR_NOMEM;
       args_gen++;
     }
@@ -574,6 +577,7 @@ struct fdtd_sim {
       slot.gen = args_gen;
     }
     if (slot.nwg < 0) {
+      if (dry) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
       // a grid outside the fused step's layouts (rows over 1024 columns, P over 64, ...): the
       // per-op kernels in the reference op order, in place on the current set
       if (timed) {
@@ -596,6 +600,7 @@ struct fdtd_sim {
     }
     HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr));
     if (timed) HIPCHK(hipEventRecord(e[3], stream));
+    if (dry) return 0;
     cur = 1 - cur;
     return record_probes(cur);
   }
@@ -836,6 +841,51 @@ int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode) {
   NEED(s);
   int slot = 0;
   return s->step(n, mode, false, &slot);
+}
+
+int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms) {
+  NEED_READY(s);
+  if (reps < 1) return fail(FDTD_ERR_INVALID, "reps must be >= 1");
+  const StepTuning saved = s->tune;
+  StepTuning best = saved;
+  double best_t = -1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  for (int cpl : {1, 2, 4}) {
+    for (int kc : {4, 8, 16}) {
+      s->tune = StepTuning();
+      s->tune.cpl = cpl;
+      s->tune.z_chunk = kc;
+      s->args_gen++;
+      int slot = 0;
+      if (s->step(n, mode, false, &slot, true)) continue;  // layout not available for this grid
+      if ((rc = hipEventRecord(e0, s->stream) == hipSuccess ? 0 : FDTD_ERR_DEVICE)) break;
+      for (int q = 0; q < reps && !rc; ++q) rc = s->step(n, mode, false, &slot, true);
+      if (rc || hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = rc ? rc : FDTD_ERR_DEVICE;
+        break;
+      }
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double t = ms / reps;
+      if (best_t < 0 || t < best_t) {
+        best_t = t;
+        best = s->tune;
+      }
+    }
+    if (rc) break;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  s->tune = best_t < 0 ? saved : best;
+  s->args_gen++;
+  if (rc) return rc;
+  if (best_t < 0) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
+  g_err.clear();
+  if (best_ms) *best_ms = best_t;
+  return 0;
 }
 
 int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_every, int32_t snap_comp) {

@@ -575,7 +575,7 @@ struct StepArgs {
   int i0, i1, j0, j1, k0, k1;
   int S, R, CPL, nTy;  // row segments per row, rows per workgroup, columns per lane, row tiles
   int KC, ntab, act_h, act_e;
-  int N1, xnj, xnk, pad;
+  int xni, xnj, xnk, pad;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -585,11 +585,8 @@ struct StepArgs {
   const uint8_t *mat;
   const float *coef;   // index mode: [ntab][6]; full mode: [6][IJK]
   const float *prm;    // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
-  const float *pwtab;  // E tables x(int,half) y(int,half) z(int,half), then the same for H
-  const float *Eh, *Hh;  // 1D line histories [T][N1]
-  const float *xs;       // x-face sheets of the TF/SF box, 8 x [xnk][xnj]
+  const float *xs;      // TF/SF face sheets (tfsf_sheet)
   float Chh, Che, dx;
-  float polE[3], polH[3];
 };
 
 // Pointers taken from the argument block are generic; casting them to the global address space
@@ -608,92 +605,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__device__ __forceinline__ float line_at_g(const float *rowp, float s) {
-  const auto row = gptr(rowp);
-  const float fl = floorf(s);
-  const int m = (int)fl;
-  const float w = s - fl;
-  if (w == 0.0f) return row[m];
-  return row[m] + w * (row[m + 1] - row[m]);
-}
-
-__device__ __forceinline__ float tab_e(const StepArgs &a, const float *row, int c, int x, int y, int z) {
-  const auto t = gptr(a.pwtab);
-  const float sx = t[(c == 0 ? a.I : 0) + x];
-  const float sy = t[2 * a.I + (c == 1 ? a.J : 0) + y];
-  const float sz = t[2 * a.I + 2 * a.J + (c == 2 ? a.K : 0) + z];
-  return a.polE[c] * line_at_g(row, (sx + sy) + sz);
-}
-__device__ __forceinline__ float tab_h(const StepArgs &a, const float *row, int c, int x, int y, int z) {
-  const auto t = gptr(a.pwtab + 2 * (a.I + a.J + a.K));
-  const float sx = t[(c != 0 ? a.I : 0) + x];
-  const float sy = t[2 * a.I + (c != 1 ? a.J : 0) + y];
-  const float sz = t[2 * a.I + 2 * a.J + (c != 2 ? a.K : 0) + z];
-  return a.polH[c] * line_at_g(row, (sx + sy) + sz);
-}
-
-// TF/SF corrections of an H cell on the y and z faces (in that order), generic form of
-// src/gpu.h:152-573.  The x-face corrections come first and are applied by the caller from the
-// x-face sheets.
-__device__ __forceinline__ void pw_h_yz(const StepArgs &a, const float *row, int i, int j, int k, float &hx, float &hy,
-                                        float &hz) {
-  const int m = a.act_h;
-  const float Che = a.Che;
-  if (j == a.j0 - 1 || j == a.j1) {
-    const bool lo = j == a.j0 - 1;
-    const int ye = lo ? a.j0 : a.j1;
-    if ((m & (1 << 3)) && i >= a.i0 && i <= a.i1 && k >= a.k0 && k <= a.k1 - 1) {
-      const float v = Che * tab_e(a, row, 2, i, ye, k);
-      hx = lo ? hx + v : hx - v;
-    }
-    if ((m & (1 << 5)) && i >= a.i0 && i <= a.i1 - 1 && k >= a.k0 && k <= a.k1) {
-      const float v = Che * tab_e(a, row, 0, i, ye, k);
-      hz = lo ? hz - v : hz + v;
-    }
-  }
-  if (k == a.k0 - 1 || k == a.k1) {
-    const bool lo = k == a.k0 - 1;
-    const int ze = lo ? a.k0 : a.k1;
-    if ((m & (1 << 6)) && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 - 1) {
-      const float v = Che * tab_e(a, row, 1, i, j, ze);
-      hx = lo ? hx - v : hx + v;
-    }
-    if ((m & (1 << 7)) && i >= a.i0 && i <= a.i1 - 1 && j >= a.j0 && j <= a.j1) {
-      const float v = Che * tab_e(a, row, 0, i, j, ze);
-      hy = lo ? hy + v : hy - v;
-    }
-  }
-}
-
-__device__ __forceinline__ void pw_e_yz(const StepArgs &a, const float *row, const float ch[3], int i, int j, int k,
-                                        float &ex, float &ey, float &ez) {
-  const int m = a.act_e;
-  if (j == a.j0 || j == a.j1) {
-    const bool lo = j == a.j0;
-    const int yh = lo ? a.j0 - 1 : a.j1;
-    if ((m & (1 << 3)) && i >= a.i0 && i <= a.i1 - 1 && k >= a.k0 && k <= a.k1) {
-      const float v = ch[0] * tab_h(a, row, 2, i, yh, k);
-      ex = lo ? ex - v : ex + v;
-    }
-    if ((m & (1 << 5)) && i >= a.i0 && i <= a.i1 && k >= a.k0 && k <= a.k1 - 1) {
-      const float v = ch[2] * tab_h(a, row, 0, i, yh, k);
-      ez = lo ? ez + v : ez - v;
-    }
-  }
-  if (k == a.k0 || k == a.k1) {
-    const bool lo = k == a.k0;
-    const int zh = lo ? a.k0 - 1 : a.k1;
-    if ((m & (1 << 6)) && i >= a.i0 && i <= a.i1 - 1 && j >= a.j0 && j <= a.j1) {
-      const float v = ch[0] * tab_h(a, row, 1, i, j, zh);
-      ex = lo ? ex + v : ex - v;
-    }
-    if ((m & (1 << 7)) && i >= a.i0 && i <= a.i1 && j >= a.j0 && j <= a.j1 - 1) {
-      const float v = ch[1] * tab_h(a, row, 0, i, j, zh);
-      ey = lo ? ey - v : ey + v;
-    }
-  }
-}
-
 // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD walks a contiguous range
 // of tiles (neighbouring row tiles of one z chunk share their halo rows in that XCD's L2).
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -702,94 +613,189 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (per + 1) + q : rem * (per + 1) + (x - rem) * per + q;
 }
 
-// Byte-offset addressing of the row kernel: every access is a uniform base pointer (held in
-// scalar registers) plus the lane's byte offset, which is 32-bit when a field component spans
-// less than 4 GiB, so that each access needs one vector register for its address.
-template <bool WIDE>
-struct ByteOff {
-  using T = uint32_t;
-};
-template <>
-struct ByteOff<true> {
-  using T = uint64_t;
-};
+// Addressing of the row kernel.  Every access is (array, uniform offset u, lane offset v), both in
+// bytes.  Buffer mode (each array under 4 GiB): a buffer resource per array and 32-bit offsets,
+// u in a scalar register (soffset) and v in one vector register (voffset), so that the many
+// row / plane / component neighbours of a cell cost no 64-bit address registers; out-of-range
+// buffer accesses read 0 and drop stores.  Global mode: 64-bit pointer arithmetic.
+#define AS1 __attribute__((address_space(1)))
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
-#define AS1 __attribute__((address_space(1)))
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+// third dword of a gfx9-family buffer resource: 32-bit data format, raw (untyped) access
+constexpr int kBufWord3 = 0x00020000;
+
+template <bool BUF>
+struct Acc;
+
+template <>
+struct Acc<true> {
+  using O = uint32_t;
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Acc(const void *base, size_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)(uint32_t)bytes, kBufWord3)) {}
+  template <int CPL>
+  __device__ __forceinline__ void ld(O v, O u, float (&x)[CPL]) const {
+    if constexpr (CPL == 4) {
+      const f4v t = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, (int)v, (int)u, 0));
+      x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
+    } else if constexpr (CPL == 2) {
+      const f2v t = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, (int)v, (int)u, 0));
+      x[0] = t.x; x[1] = t.y;
+    } else {
+      x[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)u, 0));
+    }
+  }
+  __device__ __forceinline__ float ld1(O v, O u) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)u, 0));
+  }
+  template <int CPL>
+  __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
+    if constexpr (CPL == 4) {
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, f4v{x[0], x[1], x[2], x[3]}), r, (int)v, (int)u, 0);
+    } else if constexpr (CPL == 2) {
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, f2v{x[0], x[1]}), r, (int)v, (int)u, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), r, (int)v, (int)u, 0);
+    }
+  }
+  __device__ __forceinline__ void st1(O v, O u, float x) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)u, 0);
+  }
+  // CPL consecutive bytes at byte offset v + u
+  template <int CPL>
+  __device__ __forceinline__ void ldm(O v, O u, int (&m)[CPL]) const {
+    if constexpr (CPL == 4) {
+      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)u, 0);
+      m[0] = w & 255; m[1] = (w >> 8) & 255; m[2] = (w >> 16) & 255; m[3] = w >> 24;
+    } else if constexpr (CPL == 2) {
+      const uint32_t w = __builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)u, 0);
+      m[0] = w & 255; m[1] = w >> 8;
+    } else {
+      m[0] = __builtin_amdgcn_raw_buffer_load_b8(r, (int)v, (int)u, 0);
+    }
+  }
+};
+
+template <>
+struct Acc<false> {
+  using O = uint64_t;
+  AS1 char *p;
+  __device__ __forceinline__ Acc(const void *base, size_t) : p((AS1 char *)const_cast<void *>(base)) {}
+  template <int CPL>
+  __device__ __forceinline__ void ld(O v, O u, float (&x)[CPL]) const {
+    const AS1 char *q = p + u + v;
+    if constexpr (CPL == 4) {
+      const f4v t = *(const AS1 f4v *)q;
+      x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
+    } else if constexpr (CPL == 2) {
+      const f2v t = *(const AS1 f2v *)q;
+      x[0] = t.x; x[1] = t.y;
+    } else {
+      x[0] = *(const AS1 float *)q;
+    }
+  }
+  __device__ __forceinline__ float ld1(O v, O u) const { return *(const AS1 float *)(p + u + v); }
+  template <int CPL>
+  __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
+    AS1 char *q = p + u + v;
+    if constexpr (CPL == 4) {
+      *(AS1 f4v *)q = f4v{x[0], x[1], x[2], x[3]};
+    } else if constexpr (CPL == 2) {
+      *(AS1 f2v *)q = f2v{x[0], x[1]};
+    } else {
+      *(AS1 float *)q = x[0];
+    }
+  }
+  __device__ __forceinline__ void st1(O v, O u, float x) const { *(AS1 float *)(p + u + v) = x; }
+  template <int CPL>
+  __device__ __forceinline__ void ldm(O v, O u, int (&m)[CPL]) const {
+    const AS1 uint8_t *q = (const AS1 uint8_t *)(p + u + v);
+    if constexpr (CPL == 4) {
+      const uint32_t w = *(const AS1 uint32_t *)q;
+      m[0] = w & 255; m[1] = (w >> 8) & 255; m[2] = (w >> 16) & 255; m[3] = w >> 24;
+    } else if constexpr (CPL == 2) {
+      const uint32_t w = *(const AS1 uint16_t *)q;
+      m[0] = w & 255; m[1] = w >> 8;
+    } else {
+      m[0] = *q;
+    }
+  }
+};
 
 template <typename O>
 __device__ __forceinline__ float ldo(const float *base, O off) {
   return *(const AS1 float *)((const AS1 char *)base + off);
 }
-template <typename O>
-__device__ __forceinline__ void sto(float *base, O off, float v) {
-  *(AS1 float *)((AS1 char *)base + off) = v;
+
+// Values the whole wave shares, moved to scalar registers.  The compiler cannot always prove a
+// value wave-uniform (anything derived from threadIdx.x >> 6, or selected under per-lane control
+// flow); a buffer access with such an operand would otherwise be wrapped in a loop over its
+// distinct values.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uni(uint64_t x) {
+  return (uint64_t)uni((uint32_t)x) | ((uint64_t)uni((uint32_t)(x >> 32)) << 32);
 }
-// CPL consecutive floats at a 4*CPL-byte aligned offset, one vector access
-template <int CPL, typename O>
-__device__ __forceinline__ void ldv(const float *base, O off, float (&v)[CPL]) {
-  const AS1 char *p = (const AS1 char *)base + off;
-  if constexpr (CPL == 4) {
-    const f4v t = *(const AS1 f4v *)p;
-    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-  } else if constexpr (CPL == 2) {
-    const f2v t = *(const AS1 f2v *)p;
-    v[0] = t.x; v[1] = t.y;
-  } else {
-    v[0] = *(const AS1 float *)p;
-  }
+template <typename T>
+__device__ __forceinline__ T *uni(T *p) {
+  return (T *)uni((uint64_t)(uintptr_t)p);
 }
-template <int CPL, typename O>
-__device__ __forceinline__ void stv(float *base, O off, const float (&v)[CPL]) {
-  AS1 char *p = (AS1 char *)base + off;
-  if constexpr (CPL == 4) {
-    *(AS1 f4v *)p = f4v{v[0], v[1], v[2], v[3]};
-  } else if constexpr (CPL == 2) {
-    *(AS1 f2v *)p = f2v{v[0], v[1]};
-  } else {
-    *(AS1 float *)p = v[0];
-  }
-}
-// CPL consecutive material bytes
-template <int CPL, typename O>
-__device__ __forceinline__ void ldm(const uint8_t *base, O off, int (&m)[CPL]) {
-  const AS1 uint8_t *p = (const AS1 uint8_t *)base + off;
-  if constexpr (CPL == 4) {
-    const uint32_t w = *(const AS1 uint32_t *)p;
-    m[0] = w & 255; m[1] = (w >> 8) & 255; m[2] = (w >> 16) & 255; m[3] = w >> 24;
-  } else if constexpr (CPL == 2) {
-    const uint32_t w = *(const AS1 uint16_t *)p;
-    m[0] = w & 255; m[1] = w >> 8;
-  } else {
-    m[0] = *p;
-  }
+// The same pointer, opaque to the optimiser: loads through it are not hoisted out of the branch
+// that makes them, so rarely used arguments do not stay live in scalar registers across a loop.
+template <typename T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
 }
 
-// x-face sheets: the incident values the x faces of the TF/SF box need at time step n, computed
-// once per step by the slab-H launch.  Sheet q = 4 eh + 2 hi + u over (k - k0, j - j0), j fastest:
-//   eh 0 (H update, face cells i0 - 1 / i1, incident E at x = i0 / i1):
-//     u 0: E_z for Hy (bit 1, j in [j0, j1], k in [k0, k1 - 1]); u 1: E_y for Hz (bit 2, j in
-//     [j0, j1 - 1], k in [k0, k1])
-//   eh 1 (E update, face cells i0 / i1, incident H at x = i0 - 1 / i1):
-//     u 0: H_y for Ez (bit 2, ranges of u 0 above); u 1: H_z for Ey (bit 1, ranges of u 1)
-// Only entries inside their ranges are written (and read).
-__device__ __forceinline__ void tfsf_x_sheet(const GridDev &g, const PWDev &pw, const float *rowE, const float *rowH,
-                                             float *xs, int nj, int nk, long t) {
-  const long S = (long)nj * nk;
-  const int q = (int)(t / S), r = (int)(t % S);
-  const int j = g.j0 + r % nj, k = g.k0 + r / nj;
+// TF/SF face sheets: the incident values the six faces of the TF/SF box need at time step n,
+// computed once per step by the slab-H launch, so that the row kernel applies every TF/SF
+// correction with one load.  Sheet q = 4 eh + 2 hi + u of face axis a (x, y, z): update eh (0: H,
+// from the incident E; 1: E, from the incident H), face hi (0: low, 1: high) and the face's two
+// tangential components u, over the face's other two axes within the box (x faces (j, k), y faces
+// (i, k), z faces (i, j), first index fastest).  The x sheets come first, then y, then z.
+// Component, range and face position per (a, eh, u), those of the TF/SF corrections of
+// src/gpu.h:152-573 (range: the box, one short along the axis named):
+//   a  H u=0           H u=1           E u=0            E u=1
+//   x  E_z  k<k1 ->Hy  E_y  j<j1 ->Hz  H_y  k<k1 ->Ez   H_z  j<j1 ->Ey
+//   y  E_z  k<k1 ->Hx  E_x  i<i1 ->Hz  H_z  i<i1 ->Ex   H_x  k<k1 ->Ez
+//   z  E_y  j<j1 ->Hx  E_x  i<i1 ->Hy  H_y  i<i1 ->Ex   H_x  j<j1 ->Ey
+// H sheets sit on the face planes i0|i1 (j0|j1, k0|k1), E sheets on i0-1|i1 (j0-1|j1, k0-1|k1).
+// Entries outside their range are not written (and not read).
+__device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, const float *rowE, const float *rowH,
+                                           float *xs, int ni, int nj, int nk, long t) {
+  const long Sx = (long)nj * nk, Sy = (long)ni * nk, Sz = (long)ni * nj;
+  int a;
+  long r;
+  if (t < 8 * Sx) { a = 0; r = t; }
+  else if (t < 8 * (Sx + Sy)) { a = 1; r = t - 8 * Sx; }
+  else { a = 2; r = t - 8 * (Sx + Sy); }
+  const long Sa = a == 0 ? Sx : (a == 1 ? Sy : Sz);
+  const int q = (int)(r / Sa);
+  const long e = r - q * Sa;
   const int eh = q >> 2, hi = (q >> 1) & 1, u = q & 1;
-  const bool in = u == 0 ? (k <= g.k1 - 1) : (j <= g.j1 - 1);
-  if (!in) return;
-  float v;
-  if (eh == 0) {
-    if (!(pw.active_h & (u == 0 ? 2 : 4))) return;
-    v = inc_e(pw, rowE, u == 0 ? 2 : 1, hi ? g.i1 : g.i0, j, k);
-  } else {
-    if (!(pw.active_e & (u == 0 ? 4 : 2))) return;
-    v = inc_h(pw, rowH, u == 0 ? 1 : 2, hi ? g.i1 : g.i0 - 1, j, k);
-  }
-  xs[(size_t)q * S + r] = v;
+  const int nf = a == 0 ? nj : ni;
+  const int fa = (int)(e % nf), sa = (int)(e / nf);
+  // bit, incident component and the axis whose range stops one short, per [eh][a][u]
+  constexpr int kBit[2][3][2] = {{{1, 2}, {3, 5}, {6, 7}}, {{2, 1}, {3, 5}, {6, 7}}};
+  constexpr int kComp[2][3][2] = {{{2, 1}, {2, 0}, {1, 0}}, {{1, 2}, {2, 0}, {1, 0}}};
+  constexpr int kShort[2][3][2] = {{{2, 1}, {2, 0}, {1, 0}}, {{2, 1}, {0, 2}, {0, 1}}};
+  const int bit = kBit[eh][a][u], comp = kComp[eh][a][u], sh = kShort[eh][a][u];
+  if (!((eh == 0 ? pw.active_h : pw.active_e) & (1 << bit))) return;
+  int i, j, k;
+  const int lo0[3] = {g.i0, g.j0, g.k0}, hi0[3] = {g.i1, g.j1, g.k1};
+  const int pos = eh == 0 ? (hi ? hi0[a] : lo0[a]) : (hi ? hi0[a] : lo0[a] - 1);
+  if (a == 0) { i = pos; j = g.j0 + fa; k = g.k0 + sa; }
+  else if (a == 1) { i = g.i0 + fa; j = pos; k = g.k0 + sa; }
+  else { i = g.i0 + fa; j = g.j0 + sa; k = pos; }
+  const int c3[3] = {i, j, k};
+  if (c3[sh] > hi0[sh] - 1) return;
+  const float v = eh == 0 ? inc_e(pw, rowE, comp, i, j, k) : inc_h(pw, rowH, comp, i, j, k);
+  const long off = a == 0 ? 0 : (a == 1 ? 8 * Sx : 8 * (Sx + Sy));
+  xs[off + r] = v;
 }
 
 // Row kernel (B).  Waves are S row segments x R rows per workgroup; lane l of segment g holds the
@@ -800,13 +806,13 @@ __device__ __forceinline__ void tfsf_x_sheet(const GridDev &g, const PWDev &pw, 
 // H at (i, j, k-1) comes from the previous plane in registers; a chunk that does not start on
 // plane P first recomputes H on the plane below it.  Column neighbours come from the lane's own
 // registers, from the next / previous lane through cross-lane shuffles and, across segments,
-// through LDS with one barrier per plane.  The lane's byte offset o addresses (column, j-1, k);
-// rows j and j+1 and plane k+1 are reached through uniform base offsets.  1/kappa_y and 1/kappa_z
+// through LDS with one barrier per plane.  The lane offset o addresses (first column, j-1, k);
+// rows j and j+1, plane k+1 and the field component are uniform offsets.  1/kappa_y and 1/kappa_z
 // are exactly 1 on these rows and planes and their factors are omitted; 1/kappa_x is applied in
 // the x-slab columns (it is 1, an exact no-op, elsewhere).
-template <bool IDX, bool WIDE, int CPL>
+template <bool IDX, bool BUF, int CPL>
 __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
-  using O = typename ByteOff<WIDE>::T;
+  using O = typename Acc<BUF>::O;
   constexpr int W = 64 * CPL;
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
@@ -837,19 +843,19 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
   }
   __syncthreads();
 
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // the wave index is uniform across the wave; readfirstlane lets the compiler keep the row, its
+  // offsets and flags in scalar registers (and branch on them) instead of per-lane values
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = wv % S, r = wv / S;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int ty = L % nTy, tz = L / nTy;
-  const int j = P + R * ty + r;
+  const int j = uni(P + R * ty + r);
   const int kb = P + tz * KC, ke = min(kb + KC, K - P);
   const bool row_ok = j < J - P;
   if (S == 1 && !row_ok) return;  // whole wave, and no barrier follows when S == 1
   const int ib = seg * W + CPL * lane;  // first column of the lane
   const bool on = row_ok && ib < I;     // I is a multiple of CPL: all CPL columns are in the grid
   const bool jm_C = j - 1 >= P;         // row j-1 is recomputed (else: a y-slab row, read new)
-  const float *rowE = act_h ? pa->Eh + (size_t)n * pa->N1 : nullptr;
-  const float *rowH = act_e ? pa->Hh + (size_t)n * pa->N1 : nullptr;
 
   // x-slab index of each column (-1 outside the x slabs)
   int sl[CPL];
@@ -861,7 +867,7 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     slab_lane |= sl[c] >= 0;
   }
 
-  // x-face TF/SF columns of the lane (the box is at least 8 columns wide: at most one each)
+  // x-face TF/SF columns of the lane (the faces are at least CPL columns apart: at most one each)
   int xh_c = -1, xe_c = -1;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
@@ -870,34 +876,35 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     if (on && act_e && (i == bi0 || i == bi1)) xe_c = c;
   }
   const bool xh_lo = xh_c >= 0 && ib + xh_c == bi0 - 1, xe_lo = xe_c >= 0 && ib + xe_c == bi0;
-  const int xnj = pa->xnj;
-  const size_t xS = (size_t)xnj * pa->xnk;
+  // face sheets: x sheets (xnj x xnk each) first, then y (xni x xnk), then z (xni x xnj)
+  const int xni = pa->xni, xnj = pa->xnj;
+  const size_t xS = (size_t)xnj * pa->xnk, yS = (size_t)xni * pa->xnk, zS = (size_t)xni * xnj;
   const float *xs = pa->xs;
+  const float *ys = xs + 8 * xS, *zs = ys + 8 * yS;
   const uint32_t xh_sel = xh_lo ? 0u : (uint32_t)(2 * xS), xe_sel = xe_lo ? (uint32_t)(4 * xS) : (uint32_t)(6 * xS);
   const bool xj0 = j >= bj0 && j <= bj1, xj1 = j >= bj0 && j <= bj1 - 1;
   const bool xjm1 = j - 1 >= bj0 && j - 1 <= bj1 - 1;
 
-  // uniform bases: row j-1 at +0, row j at +I, row j+1 at +2I, plane k+1 at +IJ
-  const float *Fc = pa->Fc;
-  float *Fn = pa->Fn;
-  const float *Ex = Fc, *Ey = Fc + cs, *Ez = Fc + 2 * cs;
-  const float *Hx = Fc + 3 * cs, *Hy = Fc + 4 * cs, *Hz = Fc + 5 * cs;
-  float *NEx = Fn, *NEy = Fn + cs, *NEz = Fn + 2 * cs;
-  float *NHx = Fn + 3 * cs, *NHy = Fn + 4 * cs, *NHz = Fn + 5 * cs;
-  const uint8_t *mat = pa->mat;
-  const float *coef = pa->coef;
-  // x-slab psi [Eyx Ezx Hyx Hzx], element s + 2P j + 2P J k
+  // arrays and uniform offsets (bytes): component c at c*cs, row j-1 / j / j+1 at 0 / I / 2I,
+  // plane k+1 at IJ
   const size_t sx = pa->sx;
-  const float *qEy = pa->Pc, *qEz = pa->Pc + sx, *qHy = pa->Pc + 2 * sx, *qHz = pa->Pc + 3 * sx;
-  float *wEy = pa->Pn, *wEz = pa->Pn + sx, *wHy = pa->Pn + 2 * sx, *wHz = pa->Pn + 3 * sx;
+  const Acc<BUF> Fc(pa->Fc, 6 * cs * 4), Fn(pa->Fn, 6 * cs * 4);
+  // H of row j-1: the old state where it is recomputed, else the new state
+  const Acc<BUF> Fm(uni(jm_C ? pa->Fc : (const float *)pa->Fn), 6 * cs * 4);
+  const Acc<BUF> Pc(pa->Pc, 16 * (sx + pa->sy + pa->sz)), Pn(pa->Pn, 16 * (sx + pa->sy + pa->sz));
+  const Acc<BUF> Mt(pa->mat, IDX ? IJK : 0), Cf(pa->coef, IDX ? 0 : 24 * IJK);
+  const O c1 = (O)(cs * 4), c2 = 2 * c1, c3 = 3 * c1, c4 = 4 * c1, c5 = 5 * c1;
+  const O r1 = (O)(I * 4), r2 = 2 * r1, pz = (O)(IJ * 4);
+  // x-slab psi groups [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k
+  const O gEz = (O)(sx * 4), gHy = 2 * gEz, gHz = 3 * gEz;
 
-  int k = kb > P ? kb - 1 : kb;
+  const int k_first = uni(kb > P ? kb - 1 : kb), n_planes = uni(ke - k_first);
+  const int k = k_first;
   const size_t col = (on ? (size_t)ib : 0) + (size_t)(j - 1) * I;
   O o = (O)((col + (size_t)k * IJ) * 4);
-  const O dk = (O)(IJ * 4);
-  uint32_t so[CPL];  // psi byte offsets of the columns within a (row, plane)
+  O so[CPL];  // psi byte offsets of the columns within a (row, plane)
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) so[c] = (uint32_t)((sl[c] < 0 ? 0 : sl[c]) * 4);
+  for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 4);
 
   // E of row j at plane k, and Ey of row j-1 at plane k (carried from plane to plane)
   float ex[CPL], ey[CPL], ez[CPL], eym[CPL];
@@ -905,11 +912,11 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
 #pragma unroll
   for (int c = 0; c < CPL; ++c) ex[c] = ey[c] = ez[c] = eym[c] = hpx[c] = hpy[c] = 0.0f;
   if (on) {
-    ldv(Ex + I, o, ex); ldv(Ey + I, o, ey); ldv(Ez + I, o, ez);
-    ldv(Ey, o, eym);
+    Fc.ld(o + r1, 0, ex); Fc.ld(o + r1, c1, ey); Fc.ld(o + r1, c2, ez);
+    Fc.ld(o, c1, eym);
     if (kb == P) {  // plane P-1 is a z-slab plane: its new H is already written
-      ldv(NHx + I, o - dk, hpx);
-      ldv(NHy + I, o - dk, hpy);
+      Fn.ld(o + r1 - pz, c3, hpx);
+      Fn.ld(o + r1 - pz, c4, hpy);
     }
   }
   // E at the first column of the next segment (plane k), for the lane's last column
@@ -922,9 +929,10 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     par = 1;
   }
 
-  for (; k < ke; ++k, o += dk) {
+  for (int it = 0; it < n_planes; ++it, o += pz) {
+    const int k = uni(k_first + it);
     const bool own = k >= kb;  // planes of the chunk (the plane below only feeds H(k-1))
-    const size_t pq = (size_t)P2 * ((size_t)j + (size_t)J * k);
+    const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
     // ---- loads of this plane (independent, issued together) ----
     float nx[CPL], ny[CPL], nz[CPL], nym[CPL];        // E(k+1) row j, Ey(k+1) row j-1
     float exm[CPL], ezm[CPL], exq[CPL], ezq[CPL];     // Ex, Ez of rows j-1 and j+1
@@ -938,23 +946,25 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
       pHy[c] = pHz[c] = pHzm[c] = pEy[c] = pEz[c] = 0.0f;
       mt[c] = 0;
     }
+    // lane offsets of rows j-1 (o), j (o1) and j+1 (o2); uniform offsets: component, plane k+1
+    const O o1 = o + r1, o2 = o + r2;
     if (on) {
-      ldv(Ex + I + IJ, o, nx); ldv(Ey + I + IJ, o, ny); ldv(Ez + I + IJ, o, nz);
-      ldv(Ey + IJ, o, nym);
-      ldv(Ex, o, exm); ldv(Ez, o, ezm);
-      ldv(Ex + 2 * I, o, exq); ldv(Ez + 2 * I, o, ezq);
-      ldv(Hx + I, o, hx); ldv(Hy + I, o, hy); ldv(Hz + I, o, hz);
-      if (jm_C) { ldv(Hx, o, hxm); ldv(Hz, o, hzm); }
-      else { ldv(NHx, o, hxm); ldv(NHz, o, hzm); }
-      if (IDX && own) ldm(mat + I, o >> 2, mt);
+      Fc.ld(o1, pz, nx); Fc.ld(o1, c1 + pz, ny); Fc.ld(o1, c2 + pz, nz);
+      Fc.ld(o, c1 + pz, nym);
+      Fc.ld(o, 0, exm); Fc.ld(o, c2, ezm);
+      Fc.ld(o2, 0, exq); Fc.ld(o2, c2, ezq);
+      Fc.ld(o1, c3, hx); Fc.ld(o1, c4, hy); Fc.ld(o1, c5, hz);
+      Fm.ld(o, c3, hxm);
+      Fm.ld(o, c5, hzm);
+      if (IDX && own) Mt.ldm(o1 >> 2, 0, mt);
       if (slab_lane) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
           if (sl[c] >= 0) {
-            pHy[c] = ldo(qHy + pq, so[c]);
-            pHz[c] = ldo(qHz + pq, so[c]);
-            if (jm_C) pHzm[c] = ldo(qHz + pq - P2, so[c]);
-            if (own) { pEy[c] = ldo(qEy + pq, so[c]); pEz[c] = ldo(qEz + pq, so[c]); }
+            pHy[c] = Pc.ld1(so[c], gHy + pq);
+            pHz[c] = Pc.ld1(so[c], gHz + pq);
+            if (jm_C) pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
+            if (own) { pEy[c] = Pc.ld1(so[c], pq); pEz[c] = Pc.ld1(so[c], gEz + pq); }
           }
       }
     }
@@ -978,8 +988,13 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     if (S > 1 && lane == 63) { ey_n = bey; ez_n = bez; eym_n = beym; }
 
     // ---- H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML X (src/gpu.h:604-725) ----
-    const bool yz_h = act_h && j >= bj0 - 1 && j <= bj1 && k >= bk0 - 1 && k <= bk1 &&
-                      (j == bj0 - 1 || j == bj1 || k == bk0 - 1 || k == bk1);
+    // TF/SF y face (rows j0-1, j1) and z face (planes k0-1, k1) of H: the sheet row of this plane
+    // (y) or row (z), offset by the column's i - i0
+    const bool yf_h = act_h && (j == bj0 - 1 || j == bj1) && k >= bk0 && k <= bk1;
+    const bool zf_h = act_h && (k == bk0 - 1 || k == bk1) && j >= bj0 && j <= bj1;
+    const bool yf_lo = j == bj0 - 1, zf_lo = k == bk0 - 1;
+    const float *yrow = ys + (yf_lo ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * xni - bi0);
+    const float *zrow = zs + (zf_lo ? 0 : 2 * zS) + ((ptrdiff_t)(j - bj0) * xni - bi0);
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const int i = ib + c;
@@ -996,7 +1011,26 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
         if (dhy) { const float v = Che * vhy; hy[c] = xh_lo ? hy[c] - v : hy[c] + v; }
         if (dhz) { const float v = Che * vhz; hz[c] = xh_lo ? hz[c] + v : hz[c] - v; }
       }
-      if (yz_h && on && i >= bi0 - 1 && i <= bi1) pw_h_yz(*pa, rowE, i, j, k, hx[c], hy[c], hz[c]);
+      if (yf_h && on && i >= bi0 && i <= bi1) {
+        if ((act_h & (1 << 3)) && k <= bk1 - 1) {
+          const float v = Che * ldo(yrow, (uint32_t)i * 4u);
+          hx[c] = yf_lo ? hx[c] + v : hx[c] - v;
+        }
+        if ((act_h & (1 << 5)) && i <= bi1 - 1) {
+          const float v = Che * ldo(yrow + yS, (uint32_t)i * 4u);
+          hz[c] = yf_lo ? hz[c] - v : hz[c] + v;
+        }
+      }
+      if (zf_h && on && i >= bi0 && i <= bi1) {
+        if ((act_h & (1 << 6)) && j <= bj1 - 1) {
+          const float v = Che * ldo(zrow, (uint32_t)i * 4u);
+          hx[c] = zf_lo ? hx[c] - v : hx[c] + v;
+        }
+        if ((act_h & (1 << 7)) && i <= bi1 - 1) {
+          const float v = Che * ldo(zrow + zS, (uint32_t)i * 4u);
+          hy[c] = zf_lo ? hy[c] + v : hy[c] - v;
+        }
+      }
       if (s >= 0 && s < P2 - 1) {
         const float bH = sSlab[2][s], cH = sSlab[3][s];
         const float pyx = bH * pHy[c] + cH * (ez1 - ez[c]) / dx;
@@ -1010,8 +1044,11 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     // ---- Hx, Hz at (i, j-1, k), recomputed the same way ----
     if (jm_C) {
       const int jm = j - 1;
-      const bool yz_hm = act_h && jm >= bj0 - 1 && jm <= bj1 && k >= bk0 - 1 && k <= bk1 &&
-                         (jm == bj0 - 1 || jm == bj1 || k == bk0 - 1 || k == bk1);
+      const bool yf_hm = act_h && (jm == bj0 - 1 || jm == bj1) && k >= bk0 && k <= bk1;
+      const bool zf_hm = act_h && (act_h & (1 << 6)) && (k == bk0 - 1 || k == bk1) && jm >= bj0 && jm <= bj1 - 1;
+      const bool yf_lom = jm == bj0 - 1;
+      const float *yrowm = ys + (yf_lom ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * xni - bi0);
+      const float *zrowm = zs + (zf_lo ? 0 : 2 * zS) + ((ptrdiff_t)(jm - bj0) * xni - bi0);
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         const int i = ib + c;
@@ -1023,9 +1060,19 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
           hzm[c] = Chh * hzm[c] + Che * (-(eym1 - eym[c]) * kx + (ex[c] - exm[c]));
         }
         if (c == xh_c && dhzm) { const float v = Che * vhzm; hzm[c] = xh_lo ? hzm[c] + v : hzm[c] - v; }
-        if (yz_hm && on && i >= bi0 - 1 && i <= bi1) {
-          float dummy = 0.0f;
-          pw_h_yz(*pa, rowE, i, jm, k, hxm[c], dummy, hzm[c]);
+        if (yf_hm && on && i >= bi0 && i <= bi1) {
+          if ((act_h & (1 << 3)) && k <= bk1 - 1) {
+            const float v = Che * ldo(yrowm, (uint32_t)i * 4u);
+            hxm[c] = yf_lom ? hxm[c] + v : hxm[c] - v;
+          }
+          if ((act_h & (1 << 5)) && i <= bi1 - 1) {
+            const float v = Che * ldo(yrowm + yS, (uint32_t)i * 4u);
+            hzm[c] = yf_lom ? hzm[c] - v : hzm[c] + v;
+          }
+        }
+        if (zf_hm && on && i >= bi0 && i <= bi1) {
+          const float v = Che * ldo(zrowm, (uint32_t)i * 4u);
+          hxm[c] = zf_lo ? hxm[c] - v : hxm[c] + v;
         }
         if (s >= 0 && s < P2 - 1) {
           const float pzx = sSlab[2][s] * pHzm[c] + sSlab[3][s] * (eym1 - eym[c]) / dx;
@@ -1046,15 +1093,15 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
     }
 
     if (on && own) {
-      stv(NHx + I, o, hx);
-      stv(NHy + I, o, hy);
-      stv(NHz + I, o, hz);
+      Fn.st(o1, c3, hx);
+      Fn.st(o1, c4, hy);
+      Fn.st(o1, c5, hz);
       if (slab_lane) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
           if (sl[c] >= 0) {
-            sto(wHy + pq, so[c], pHy[c]);
-            sto(wHz + pq, so[c], pHz[c]);
+            Pn.st1(so[c], gHy + pq, pHy[c]);
+            Pn.st1(so[c], gHz + pq, pHz[c]);
           }
       }
       // ---- E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML X (src/gpu.h:765-868) ----
@@ -1070,14 +1117,19 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
           ch[2][c] = last ? 0.0f : rw[5];
         }
       } else {
+        const O cq = (O)(IJK * 4);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          ldv(coef + q * IJK + I, o, ce[q]);
-          ldv(coef + (3 + q) * IJK + I, o, ch[q]);
+          Cf.ld(o1, (O)q * cq, ce[q]);
+          Cf.ld(o1, (O)(3 + q) * cq, ch[q]);
         }
       }
-      const bool yz_e = act_e && j >= bj0 && j <= bj1 && k >= bk0 && k <= bk1 &&
-                        (j == bj0 || j == bj1 || k == bk0 || k == bk1);
+      // TF/SF y face (rows j0, j1) and z face (planes k0, k1) of E
+      const bool yf_e = act_e && (j == bj0 || j == bj1) && k >= bk0 && k <= bk1;
+      const bool zf_e = act_e && (k == bk0 || k == bk1) && j >= bj0 && j <= bj1;
+      const bool yf_loe = j == bj0, zf_loe = k == bk0;
+      const float *yrowe = ys + (yf_loe ? 4 : 6) * yS + ((ptrdiff_t)(k - bk0) * xni - bi0);
+      const float *zrowe = zs + (zf_loe ? 4 : 6) * zS + ((ptrdiff_t)(j - bj0) * xni - bi0);
       float ux[CPL], uy[CPL], uz[CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
@@ -1097,9 +1149,25 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
           if (dez) { const float v = ch[2][c] * vez; uz[c] = xe_lo ? uz[c] - v : uz[c] + v; }
           if (dey) { const float v = ch[1][c] * vey; uy[c] = xe_lo ? uy[c] + v : uy[c] - v; }
         }
-        if (yz_e && i >= bi0 && i <= bi1) {
-          const float chc[3] = {ch[0][c], ch[1][c], ch[2][c]};
-          pw_e_yz(*pa, rowH, chc, i, j, k, ux[c], uy[c], uz[c]);
+        if (yf_e && i >= bi0 && i <= bi1) {
+          if ((act_e & (1 << 3)) && i <= bi1 - 1) {
+            const float v = ch[0][c] * ldo(yrowe, (uint32_t)i * 4u);
+            ux[c] = yf_loe ? ux[c] - v : ux[c] + v;
+          }
+          if ((act_e & (1 << 5)) && k <= bk1 - 1) {
+            const float v = ch[2][c] * ldo(yrowe + yS, (uint32_t)i * 4u);
+            uz[c] = yf_loe ? uz[c] + v : uz[c] - v;
+          }
+        }
+        if (zf_e && i >= bi0 && i <= bi1) {
+          if ((act_e & (1 << 6)) && i <= bi1 - 1) {
+            const float v = ch[0][c] * ldo(zrowe, (uint32_t)i * 4u);
+            ux[c] = zf_loe ? ux[c] + v : ux[c] - v;
+          }
+          if ((act_e & (1 << 7)) && j <= bj1 - 1) {
+            const float v = ch[1][c] * ldo(zrowe + zS, (uint32_t)i * 4u);
+            uy[c] = zf_loe ? uy[c] - v : uy[c] + v;
+          }
         }
         if (s >= 1) {
           const float bE = sSlab[0][s], cE = sSlab[1][s];
@@ -1111,15 +1179,15 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
           pEz[c] = pzx;
         }
       }
-      stv(NEx + I, o, ux);
-      stv(NEy + I, o, uy);
-      stv(NEz + I, o, uz);
+      Fn.st(o1, 0, ux);
+      Fn.st(o1, c1, uy);
+      Fn.st(o1, c2, uz);
       if (slab_lane) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
           if (sl[c] >= 0) {
-            sto(wEy + pq, so[c], pEy[c]);
-            sto(wEz + pq, so[c], pEz[c]);
+            Pn.st1(so[c], pq, pEy[c]);
+            Pn.st1(so[c], gEz + pq, pEz[c]);
           }
       }
     }
@@ -1164,9 +1232,9 @@ struct SlabArgs {
   CoefDev co;
   PWDev pw;
   int n;
-  unsigned nbs;  // workgroups of the slab cells; k_slab_h's further workgroups fill the x-face sheets
+  unsigned nbs;  // workgroups of the slab cells; k_slab_h's further workgroups fill the face sheets
   float *xs;
-  int xnj, xnk;
+  int xni, xnj, xnk;
 };
 
 // A: H cells of the y and z CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
@@ -1174,10 +1242,10 @@ template <typename IT>
 __global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
   if (blockIdx.x >= a.nbs) {
     const long t = (long)(blockIdx.x - a.nbs) * 256 + threadIdx.x;
-    if (t < 8L * a.xnj * a.xnk) {
+    if (t < 8L * ((long)a.xnj * a.xnk + (long)a.xni * a.xnk + (long)a.xni * a.xnj)) {
       const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
       const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
-      tfsf_x_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xnj, a.xnk, t);
+      tfsf_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xni, a.xnj, a.xnk, t);
     }
     return;
   }
@@ -1333,15 +1401,16 @@ bool fused_supported(const GridDev &g) {
 
 // Row layout of the row kernel: CPL columns per lane (I a multiple of CPL, for aligned vector
 // accesses) and S segments of 64 CPL columns per row, with the fewest idle lanes, then the fewest
-// segments; at most 4 segments (rows up to 1024 columns).  force_cpl != 0 restricts CPL.
+// columns per lane (more waves in flight: measured faster at 256^3); at most 4 segments (rows up
+// to 1024 columns).  force_cpl != 0 restricts CPL.
 static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
   int best_waste = -1;
-  for (int c : {4, 2, 1}) {
+  for (int c : {1, 2, 4}) {
     if (I % c || (force_cpl && c != force_cpl)) continue;
     const int s = (I + 64 * c - 1) / (64 * c);
     if (s > 4) continue;
     const int waste = s * 64 * c - I;
-    if (best_waste < 0 || waste < best_waste || (waste == best_waste && s < S)) {
+    if (best_waste < 0 || waste < best_waste) {
       best_waste = waste;
       cpl = c;
       S = s;
@@ -1350,23 +1419,24 @@ static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
   return best_waste >= 0;
 }
 
-template <bool IDX, bool WIDE>
+template <bool IDX, bool BUF>
 static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const StepArgs *d, int n) {
-  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, WIDE, 4>), grid, blk, 0, st, d, n);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, WIDE, 2>), grid, blk, 0, st, d, n);
-  else hipLaunchKernelGGL((k_rows<IDX, WIDE, 1>), grid, blk, 0, st, d, n);
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4>), grid, blk, 0, st, d, n);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2>), grid, blk, 0, st, d, n);
+  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1>), grid, blk, 0, st, d, n);
 }
 
 static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int nwg, int n) {
-  // 32-bit byte offsets reach every access when a component plus one plane and two rows stays
-  // below 4 GiB
-  const bool wide = (a.IJK + (size_t)a.I * a.J + 2 * (size_t)a.I) * 4 >= ((size_t)1 << 32);
+  // buffer addressing when every array the kernel touches (a state set, a psi set, the material
+  // bytes or the per-cell coefficients) is below 4 GiB
+  const size_t lim = ((size_t)1 << 32) - 1;
+  const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
   const dim3 grid(nwg), blk(64 * a.S * a.R);
   if (a.mat) {
-    if (wide) launch_rows_w<true, true>(st, a.CPL, grid, blk, dargs, n);
+    if (buf) launch_rows_w<true, true>(st, a.CPL, grid, blk, dargs, n);
     else launch_rows_w<true, false>(st, a.CPL, grid, blk, dargs, n);
   } else {
-    if (wide) launch_rows_w<false, true>(st, a.CPL, grid, blk, dargs, n);
+    if (buf) launch_rows_w<false, true>(st, a.CPL, grid, blk, dargs, n);
     else launch_rows_w<false, false>(st, a.CPL, grid, blk, dargs, n);
   }
   return hipGetLastError();
@@ -1393,13 +1463,12 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const int nTy = (cJ + R - 1) / R;
   int kc = tune.z_chunk;
   if (kc <= 0) {
-    // about two waves per wave slot of the chip (256 CUs x 4 SIMDs x 4 waves x 2)
-    const long want = 8192, layer = (long)nTy * R * S;
-    long nkc = (want + layer - 1) / layer;
-    if (nkc < 1) nkc = 1;
-    if (nkc > cK) nkc = cK;
-    kc = (int)((cK + nkc - 1) / nkc);
-    if (kc < 4) kc = cK < 4 ? cK : 4;
+    // 8 planes per chunk (one recomputed plane below each chunk: 1/8 extra reads), fewer when
+    // the grid would not fill the chip with about two waves per wave slot (256 CUs x 32)
+    const long layer = (long)nTy * R * S;
+    kc = 8;
+    while (kc > 2 && layer * ((cK + kc - 1) / kc) < 16384) kc /= 2;
+    if (kc > cK) kc = cK;
   }
   const int nKc = (cK + kc - 1) / kc;
   StepArgs a;
@@ -1408,18 +1477,16 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.i0 = g.i0; a.i1 = g.i1; a.j0 = g.j0; a.j1 = g.j1; a.k0 = g.k0; a.k1 = g.k1;
   a.S = S; a.R = R; a.CPL = cpl; a.nTy = nTy;
   a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
-  a.N1 = f.N1;
+  a.xni = g.i1 - g.i0 + 1;
   a.xnj = g.j1 - g.j0 + 1;
   a.xnk = g.k1 - g.k0 + 1;
   a.IJK = (size_t)g.I * g.J * g.K;
   a.cs = f.cs;
   a.sx = (size_t)2 * g.P * g.J * g.K; a.sy = (size_t)2 * g.I * g.P * g.K; a.sz = (size_t)2 * g.I * g.J * g.P;
   a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
-  a.mat = f.mat; a.coef = f.coef; a.prm = f.prm; a.pwtab = f.pwtab;
-  a.Eh = f.Eh; a.Hh = f.Hh;
+  a.mat = f.mat; a.coef = f.coef; a.prm = f.prm;
   a.xs = f.xs;
   a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
-  for (int c = 0; c < 3; ++c) { a.polE[c] = f.polE[c]; a.polH[c] = f.polH[c]; }
   std::memcpy(out, &a, sizeof(a));
   return (cJ > 0 && cK > 0) ? nTy * nKc : 0;
 }
@@ -1471,6 +1538,7 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
   s.n = n;
   s.nbs = 0;
   s.xs = f.xs;
+  s.xni = g.i1 - g.i0 + 1;
   s.xnj = g.j1 - g.j0 + 1;
   s.xnk = g.k1 - g.k0 + 1;
   return s;
@@ -1485,8 +1553,9 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
   const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P);
   const unsigned nb = cdiv(nslab, 256);
   sa.nbs = nb;
-  const bool sheets = f.xs && ((f.act_h | f.act_e) & 6) && sa.xnj > 0 && sa.xnk > 0;
-  const unsigned nbx = sheets ? cdiv(8L * sa.xnj * sa.xnk, 256) : 0;
+  const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
+  const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
+  const unsigned nbx = sheets ? cdiv(nsheet, 256) : 0;
   const bool narrow = (long)nb * 256 < (1L << 31);
   if (narrow) hipLaunchKernelGGL(k_slab_h<uint32_t>, dim3(nb + nbx), dim3(256), 0, st, sa);
   else hipLaunchKernelGGL(k_slab_h<uint64_t>, dim3(nb + nbx), dim3(256), 0, st, sa);

@@ -26,7 +26,7 @@ EXPORTS = [
     "fdtd_set_cpml", "fdtd_set_tfsf", "fdtd_set_plane_wave", "fdtd_set_material_index",
     "fdtd_set_probe_components", "fdtd_step", "fdtd_run", "fdtd_snapshot_count",
     "fdtd_output_snapshots", "fdtd_synchronize", "fdtd_read_array", "fdtd_prepare",
-    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
+    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_autotune", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
 ]
 
 
@@ -64,7 +64,7 @@ def lib():
             "fdtd_read_array": [vp, i32, fp, u64p], "fdtd_prepare": [vp],
             "fdtd_read_setup": [vp, i32, fp, u64p], "fdtd_set_tuning": [vp, i32, i32, i32],
             "fdtd_set_timing": [vp, i32], "fdtd_kernel_time": [vp, dp, C.POINTER(C.c_int64)],
-            "fdtd_row_kernel_time": [vp, dp],
+            "fdtd_row_kernel_time": [vp, dp], "fdtd_autotune": [vp, i32, i32, i32, dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -218,6 +218,13 @@ class Simulation:
 
     def set_tuning(self, rows_per_group=0, z_chunk=0, cols_per_lane=0):
         self._chk(lib().fdtd_set_tuning(self._h, int(rows_per_group), int(z_chunk), int(cols_per_lane)))
+
+    def autotune(self, n=0, mode=0, reps=10):
+        """Pick the fastest row layout and z-chunk for this grid by timing dry steps (state
+        unchanged); returns the best average step time in ms."""
+        ms = C.c_double()
+        self._chk(lib().fdtd_autotune(self._h, int(n), int(mode), int(reps), C.byref(ms)))
+        return ms.value
 
     def set_timing(self, on=True):
         self._chk(lib().fdtd_set_timing(self._h, 1 if on else 0))

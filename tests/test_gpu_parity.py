@@ -133,6 +133,27 @@ def test_step_outside_fused_layouts():
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+def test_autotune_leaves_state_and_keeps_parity():
+    """The autotuner runs dry steps over every row layout and z-chunk: fields, psi and probes are
+    unchanged, and the chosen layout then steps bit-exactly with the oracle."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.7, 1.9, 0.4))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, 10, 5)
+    o.run(0, 10, 5)
+    before = compare_state(g, o)
+    ms = g.autotune(10, 5, reps=3)
+    assert ms > 0
+    assert_close(before, exact=True)
+    assert_close(compare_state(g, o), exact=True)
+    g.run(10, sc.T - 10, 5)
+    o.run(10, sc.T - 10, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def test_oblique_per_op_and_lines():
     from maxwell_solver_amd import scenarios as S
     sc = _scenario(mode=5, cpml=CFG_CPML, oblique=(2.2, 4.0, 1.1))
