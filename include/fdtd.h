@@ -101,8 +101,9 @@ int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);
 int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *table, int32_t n);
 /* Probe components (1..6), recorded after every step; default {3} (Ez). */
 int fdtd_set_probe_components(fdtd_sim *s, const int32_t *comps, int32_t n);
-/* One full time step n (H update, plane wave, CPML, E update, plane wave, CPML, probes) as one
- * fused kernel; mode 0 = no plane wave. */
+/* One full time step n (H update, plane wave, CPML, E update, plane wave, CPML, probes) as the
+ * fused step (three kernel launches over a ping-ponged state: y/z-slab H, every other row, y/z-slab
+ * E), or the per-op kernels for grids outside its layouts; mode 0 = no plane wave. */
 int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode);
 /* Steps n0 .. n0+count-1 (fused).  If snap_every > 0, component snap_comp (1..6) is copied to a
  * device snapshot buffer after every snap_every-th step (counted since ResetFieldAll). */

@@ -3,13 +3,11 @@
 // Two families:
 //  * per-op kernels, one per op of the reference MEX gateway, updating the state in place with
 //    the reference's masks and operation order (src/gpu.h);
-//  * k_step_fused: one launch per time step.  Workgroups own (row tile, z chunk) blocks, one
-//    wave per grid row (x fastest, 64 lanes x NC chunks, coalesced), and march up the z chunk.
-//    For each plane k the workgroup computes H(k) at n+1/2 (main update, TF/SF correction, CPML)
-//    from E(k), E(k+1) and H(k), stages it in LDS, then E(k) at n+1 from H(k), H(k-1) (kept in
-//    registers) and E(k).  The state is ping-ponged (read cur, write nxt), so a workgroup can
-//    recompute the H values of its halo row / halo plane from old data without racing its
-//    neighbours.  Each field is read once and written once per step.
+//  * the fused time step: three launches per step over a ping-ponged state (read the current
+//    set, write the next): k_slab_h (H of the y/z CPML slabs, plus the TF/SF face sheets of the
+//    step), k_rows (H and E of every other row: one wave per aligned row segment marching up a
+//    chunk of planes, the x CPML slabs and the TF/SF corrections inline), k_slab_e (E of the y/z
+//    slabs).  Each field is read once and written once per step outside the recomputed halo.
 //
 // All arithmetic is plain IEEE float with no contraction (-ffp-contract=off); expressions keep the
 // reference's association order so results equal the CPU oracle bit for bit.

@@ -18,7 +18,7 @@
  * (P=10, m=3, kappa_max=1, alpha_max=0.08, alpha_m=1, inset 15, vacuum 1D line).
  *
  * Parity pins (checked in tests/test_oracle_*.py): closed-form CPML profiles and coefficients,
- * brute-force numpy restatement on tiny non-cubic grids, discrete divergence of H, Yee energy in a
+ * a vectorised numpy statement of whole steps on a tiny non-cubic grid, discrete divergence of H, Yee energy in a
  * lossless closed cavity, SF-region leakage of the axis-aligned TF/SF (exact up to rounding), 1D
  * line vs the textbook recurrence, CPML reflection, float vs double convergence, Fresnel
  * reflection of a lossy half-space.

@@ -317,3 +317,165 @@ def test_oblique_leakage_small_and_axis_case_exact():
         peak = max(np.abs(F[n]).max() for n in F)
         leak = max(np.abs(F[n][sf]).max() for n in F)
         assert peak > 0.01 and leak <= bound * peak, (ang, leak / peak)
+
+
+# ---- an independent vectorised statement of one step ----------
+# Written from the specification with whole-array slices instead of the oracle's per-cell loops:
+# a wrong psi index, slab mirror, mask or PML order in the oracle makes the two disagree.
+PSI_NAMES = ["Exy", "Exz", "Eyx", "Eyz", "Ezx", "Ezy", "Hxy", "Hxz", "Hyx", "Hyz", "Hzx", "Hzy"]
+
+
+def _slab(n, P):
+    """Cell index and slab index s of the two CPML slabs along an axis of n cells."""
+    cells = np.r_[0:P, n - P:n]
+    return cells, np.arange(2 * P)
+
+
+def _np_h_step(F, psi, c, P):
+    Ex, Ey, Ez, Hx, Hy, Hz = (F[n] for n in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+    K, J, I = Ex.shape
+    kx, ky, kz = c["ikH"]
+    Chh, Che, dx, bH, cH = c["Chh"], c["Che"], c["dx"], c["bH"], c["cH"]
+    sl = (slice(0, K - 1), slice(0, J - 1), slice(0, I - 1))
+    dEz_y = Ez[:-1, 1:, :-1] - Ez[:-1, :-1, :-1]
+    dEy_z = Ey[1:, :-1, :-1] - Ey[:-1, :-1, :-1]
+    dEx_z = Ex[1:, :-1, :-1] - Ex[:-1, :-1, :-1]
+    dEz_x = Ez[:-1, :-1, 1:] - Ez[:-1, :-1, :-1]
+    dEy_x = Ey[:-1, :-1, 1:] - Ey[:-1, :-1, :-1]
+    dEx_y = Ex[:-1, 1:, :-1] - Ex[:-1, :-1, :-1]
+    KX, KY, KZ = kx[None, None, :-1], ky[None, :-1, None], kz[:-1, None, None]
+    Hx[sl] = Chh * Hx[sl] + Che * (-dEz_y * KY + dEy_z * KZ)
+    Hy[sl] = Chh * Hy[sl] + Che * (-dEx_z * KZ + dEz_x * KX)
+    Hz[sl] = Chh * Hz[sl] + Che * (-dEy_x * KX + dEx_y * KY)
+    # Y slabs (K5): psi[k][s][i], s < 2P-1, i < I-1, k < K-1
+    jc, s = _slab(J, P)
+    for jj, ss in zip(jc, s):
+        if ss == 2 * P - 1:
+            continue
+        a = Ez[:-1, jj + 1, :-1] - Ez[:-1, jj, :-1]
+        b = Ex[:-1, jj + 1, :-1] - Ex[:-1, jj, :-1]
+        pxy = bH[ss] * psi["Hxy"][:-1, ss, :-1] + cH[ss] * a / dx
+        pzy = bH[ss] * psi["Hzy"][:-1, ss, :-1] + cH[ss] * b / dx
+        Hx[:-1, jj, :-1] = Hx[:-1, jj, :-1] - Che * pxy * dx
+        Hz[:-1, jj, :-1] = Hz[:-1, jj, :-1] + Che * pzy * dx
+        psi["Hxy"][:-1, ss, :-1] = pxy
+        psi["Hzy"][:-1, ss, :-1] = pzy
+    # X slabs (K4): psi[k][j][s], s < 2P-1, j < J-1, k < K-1
+    ic, s = _slab(I, P)
+    for ii, ss in zip(ic, s):
+        if ss == 2 * P - 1:
+            continue
+        a = Ez[:-1, :-1, ii + 1] - Ez[:-1, :-1, ii]
+        b = Ey[:-1, :-1, ii + 1] - Ey[:-1, :-1, ii]
+        pyx = bH[ss] * psi["Hyx"][:-1, :-1, ss] + cH[ss] * a / dx
+        pzx = bH[ss] * psi["Hzx"][:-1, :-1, ss] + cH[ss] * b / dx
+        Hy[:-1, :-1, ii] = Hy[:-1, :-1, ii] + Che * pyx * dx
+        Hz[:-1, :-1, ii] = Hz[:-1, :-1, ii] - Che * pzx * dx
+        psi["Hyx"][:-1, :-1, ss] = pyx
+        psi["Hzx"][:-1, :-1, ss] = pzx
+    # Z slabs (K6): psi[s][j][i], s < 2P-1, i < I-1, j < J-1
+    kc, s = _slab(K, P)
+    for kk, ss in zip(kc, s):
+        if ss == 2 * P - 1:
+            continue
+        a = Ey[kk + 1, :-1, :-1] - Ey[kk, :-1, :-1]
+        b = Ex[kk + 1, :-1, :-1] - Ex[kk, :-1, :-1]
+        pxz = bH[ss] * psi["Hxz"][ss, :-1, :-1] + cH[ss] * a / dx
+        pyz = bH[ss] * psi["Hyz"][ss, :-1, :-1] + cH[ss] * b / dx
+        Hx[kk, :-1, :-1] = Hx[kk, :-1, :-1] + Che * pxz * dx
+        Hy[kk, :-1, :-1] = Hy[kk, :-1, :-1] - Che * pyz * dx
+        psi["Hxz"][ss, :-1, :-1] = pxz
+        psi["Hyz"][ss, :-1, :-1] = pyz
+
+
+def _np_e_step(F, psi, c, P):
+    Ex, Ey, Ez, Hx, Hy, Hz = (F[n] for n in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"))
+    K, J, I = Ex.shape
+    kx, ky, kz = c["ikE"]
+    (Cex, Cey, Cez), (Chx, Chy, Chz), dx, bE, cE = c["Ce"], c["Ch"], c["dx"], c["bE"], c["cE"]
+    KX, KY, KZ = kx[None, None, :], ky[None, :, None], kz[:, None, None]
+    # masks j != 0 and k != 0 etc. (A.2): update the sub-blocks
+    s = (slice(1, K), slice(1, J), slice(None))
+    Ex[s] = Cex[s] * Ex[s] + Chx[s] * ((Hz[s] - Hz[1:, :-1, :]) * KY[:, 1:, :] - (Hy[s] - Hy[:-1, 1:, :]) * KZ[1:])
+    s = (slice(1, K), slice(None), slice(1, I))
+    Ey[s] = Cey[s] * Ey[s] + Chy[s] * ((Hx[s] - Hx[:-1, :, 1:]) * KZ[1:] - (Hz[s] - Hz[1:, :, :-1]) * KX[:, :, 1:])
+    s = (slice(None), slice(1, J), slice(1, I))
+    Ez[s] = Cez[s] * Ez[s] + Chz[s] * ((Hy[s] - Hy[:, 1:, :-1]) * KX[:, :, 1:] - (Hx[s] - Hx[:, :-1, 1:]) * KY[:, 1:, :])
+    # Y slabs (K8): s >= 1, 0 < i < I-1, 1 <= k < K-1
+    jc, ss_ = _slab(J, P)
+    for jj, ss in zip(jc, ss_):
+        if ss == 0:
+            continue
+        r = (slice(1, K - 1), jj, slice(1, I - 1))
+        a = Hz[r] - Hz[1:K - 1, jj - 1, 1:I - 1]
+        b = Hx[r] - Hx[1:K - 1, jj - 1, 1:I - 1]
+        pxy = bE[ss] * psi["Exy"][1:K - 1, ss, 1:I - 1] + cE[ss] * a / dx
+        pzy = bE[ss] * psi["Ezy"][1:K - 1, ss, 1:I - 1] + cE[ss] * b / dx
+        Ex[r] = Ex[r] + Chx[r] * pxy * dx
+        Ez[r] = Ez[r] - Chz[r] * pzy * dx
+        psi["Exy"][1:K - 1, ss, 1:I - 1] = pxy
+        psi["Ezy"][1:K - 1, ss, 1:I - 1] = pzy
+    # Z slabs (K9): s >= 1, 0 < i < I-1, 0 < j < J-1
+    kc, ss_ = _slab(K, P)
+    for kk, ss in zip(kc, ss_):
+        if ss == 0:
+            continue
+        r = (kk, slice(1, J - 1), slice(1, I - 1))
+        a = Hy[r] - Hy[kk - 1, 1:J - 1, 1:I - 1]
+        b = Hx[r] - Hx[kk - 1, 1:J - 1, 1:I - 1]
+        pxz = bE[ss] * psi["Exz"][ss, 1:J - 1, 1:I - 1] + cE[ss] * a / dx
+        pyz = bE[ss] * psi["Eyz"][ss, 1:J - 1, 1:I - 1] + cE[ss] * b / dx
+        Ex[r] = Ex[r] - Chx[r] * pxz * dx
+        Ey[r] = Ey[r] + Chy[r] * pyz * dx
+        psi["Exz"][ss, 1:J - 1, 1:I - 1] = pxz
+        psi["Eyz"][ss, 1:J - 1, 1:I - 1] = pyz
+    # X slabs (K7): s >= 1, 0 < j < J-1, 1 <= k < K-1
+    ic, ss_ = _slab(I, P)
+    for ii, ss in zip(ic, ss_):
+        if ss == 0:
+            continue
+        r = (slice(1, K - 1), slice(1, J - 1), ii)
+        a = Hz[r] - Hz[1:K - 1, 1:J - 1, ii - 1]
+        b = Hy[r] - Hy[1:K - 1, 1:J - 1, ii - 1]
+        pyx = bE[ss] * psi["Eyx"][1:K - 1, 1:J - 1, ss] + cE[ss] * a / dx
+        pzx = bE[ss] * psi["Ezx"][1:K - 1, 1:J - 1, ss] + cE[ss] * b / dx
+        Ey[r] = Ey[r] - Chy[r] * pyx * dx
+        Ez[r] = Ez[r] + Chz[r] * pzx * dx
+        psi["Eyx"][1:K - 1, 1:J - 1, ss] = pyx
+        psi["Ezx"][1:K - 1, 1:J - 1, ss] = pzx
+
+
+def test_oracle_step_matches_vectorised_statement():
+    """Three full steps (H, CPML H, E, CPML E) of the float64 oracle on a tiny non-cubic grid with
+    random fields, random materials and kappa_max = 5 equal the vectorised statement exactly."""
+    I, J, K, P = 17, 13, 11, 3
+    s = _sim(I, J, K, cpml=[P, 3, 5, 0.05, 1, 0.8], inset=4)
+    rng = np.random.default_rng(11)
+    for flag in range(1, 7):
+        v = (rng.uniform(0, 0.05, I * J * K) if flag <= 3 else rng.uniform(1, 9, I * J * K)).astype(np.float32)
+        s.update_material(flag, v)
+    s.ready()
+    F = {n: rng.standard_normal((K, J, I)) for n in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+    for n, v in F.items():
+        s.set_field(n, v)
+    sc = s.scalars()
+    Ce, Ch = s.coeffs()
+    c = {"Chh": sc["Chh"], "Che": sc["Che"], "dx": DXF,
+         "bE": s.setup(0), "cE": s.setup(1), "bH": s.setup(2), "cH": s.setup(3),
+         "ikE": (s.setup(6), s.setup(7), s.setup(8)), "ikH": (s.setup(9), s.setup(10), s.setup(11)),
+         "Ce": [x.reshape(K, J, I).astype(np.float64) for x in Ce],
+         "Ch": [x.reshape(K, J, I).astype(np.float64) for x in Ch]}
+    assert (c["ikH"][0] != 1).any() and (c["ikE"][2] != 1).any()  # kappa_max = 5 exercises 1/kappa
+    shapes = {"x": (K, J, 2 * P), "y": (K, 2 * P, I), "z": (2 * P, J, I)}
+    psi = {n: np.zeros(shapes[n[2]]) for n in PSI_NAMES}
+    for _ in range(3):
+        s.update_h_all()
+        s.update_h_pml()
+        s.update_e_all()
+        s.update_e_pml()
+        _np_h_step(F, psi, c, P)
+        _np_e_step(F, psi, c, P)
+    for n in F:
+        np.testing.assert_array_equal(s.field(n), F[n], err_msg=n)
+    for q, n in enumerate(PSI_NAMES):
+        np.testing.assert_array_equal(s.psi(q).reshape(shapes[n[2]]), psi[n], err_msg=n)
