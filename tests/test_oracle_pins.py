@@ -479,3 +479,27 @@ def test_oracle_step_matches_vectorised_statement():
         np.testing.assert_array_equal(s.field(n), F[n], err_msg=n)
     for q, n in enumerate(PSI_NAMES):
         np.testing.assert_array_equal(s.psi(q).reshape(shapes[n[2]]), psi[n], err_msg=n)
+
+
+def test_cpml_reference_preset_golden():
+    """The oracle's E and H grading (low slab and its mirror) at the reference preset equals the
+    committed closed-form fixture tests/golden/cpml_reference_preset.json to float32 rounding."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cpml_reference_preset.json")))
+    p = g["params"]
+    P = p["P"]
+    s = _sim(40, 40, 40, cpml=[P, p["m"], p["kappa_max"], p["alpha_max"], p["alpha_m"], p["sig_factor"]])
+    s.ready()
+    bE, cE, bH, cH = (s.setup(w) for w in range(4))
+    for row in g["E"]:
+        q = row["q"]
+        for arr, key in ((bE, "b"), (cE, "c")):
+            assert arr[q] == pytest.approx(row[key], rel=3e-6, abs=1e-12), (key, q)
+            assert arr[2 * P - 1 - q] == arr[q]
+    for row in g["H"]:
+        q = row["q"]
+        for arr, key in ((bH, "b"), (cH, "c")):
+            assert arr[q] == pytest.approx(row[key], rel=3e-6, abs=1e-12), (key, q)
+            assert arr[2 * P - 2 - q] == arr[q]
+    assert bH[P - 1] == 0 or bH[2 * P - 1] == 0
