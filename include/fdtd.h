@@ -22,6 +22,7 @@ extern "C" {
 #endif
 
 typedef struct fdtd_sim fdtd_sim; /* one simulation; reference: class simulation, src/simulation.h:24 */
+typedef struct fdtd_batch fdtd_batch; /* a set of simulations advanced together (no reference counterpart) */
 
 enum {
   FDTD_OK = 0,
@@ -132,6 +133,24 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
  * grid's I must be a multiple of it and a row at most 256 of it).  An unsupported choice makes
  * the next step fail with FDTD_ERR_INVALID. */
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
+/* Batched engine (no counterpart in the reference; for many independent grids, each much smaller
+ * than the GPU).  fdtd_batch_add queues `count` steps of a ready simulation from time index n0
+ * with plane-wave mode `mode`; fdtd_batch_run advances every queued simulation to the end of its
+ * steps.  Each batch step launches every kernel of the fused step once for all grids still
+ * running, so the grids share the GPU and the launch count does not grow with their number.
+ * Results, psi, probe records and time steps are those of fdtd_run on each simulation alone
+ * (snapshots are not taken in a batch).  A simulation is queued at most once per batch; grids the
+ * fused step does not cover run alone after the batch. */
+int fdtd_batch_create(fdtd_batch **out);
+int fdtd_batch_destroy(fdtd_batch *b);
+int fdtd_batch_add(fdtd_batch *b, fdtd_sim *s, int32_t n0, int32_t count, int32_t mode);
+int fdtd_batch_clear(fdtd_batch *b);
+int fdtd_batch_run(fdtd_batch *b);
+/* How fdtd_batch_run advances the grids: 1 = merged launches (each kernel of the step launched once
+ * for all grids, in up to four interleaved sub-batches); 2 = one stream per grid (each grid's
+ * steps queued on its own stream); 0 (default) = merged launches for the grids under 2^18 cells
+ * and one stream each for the larger ones, both at once.  Results are the same either way. */
+int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy);
 /* Time the fused step at time index n, plane-wave mode `mode`, for each available columns-per-lane
  * and z-chunk choice (reps steps each) and keep the fastest (*best_ms: its average step time).
  * The steps are run dry: the fields, psi, probe record and time step are unchanged. */

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include <map>
 #include <vector>
 
 #include "../../include/fdtd.h"
@@ -534,12 +535,9 @@ struct fdtd_sim {
     return 0;
   }
 
-  // One fused time step.  dry: launch the kernels (they read the current set and write the next)
-  // but keep the current set and the probe recorder: the state is unchanged (used for timing).
-  int step(int n, int mode, bool timed, int *ev_slot, bool dry = false) {
-    if (!init_ready) return fail(FDTD_ERR_STATE, "Step before ReadySimulation");
-    const PWDev *pw = nullptr;
-    if (int rc = check_pw(n, mode, &pw)) return rc;
+  // The fused step's description and row-kernel argument block for current set `which` and
+  // plane wave pw (null: none), built and uploaded when anything it depends on changed.
+  int fused_slot(int which, int mode, const PWDev *pw, FusedDev &f, ArgSlot **out) {
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
     if (pw && !dXs && g.i1 >= g.i0 && g.j1 >= g.j0 && g.k1 >= g.k0) {
@@ -548,11 +546,11 @@ struct fdtd_sim {
       if (dalloc(&dXs, nxs)) return FDTD_ERR_NOMEM;
       args_gen++;
     }
-    ArgSlot &slot = argslots[cur][slot_mode];
-    FusedDev f;
+    ArgSlot &slot = argslots[which][slot_mode];
     {
+      f = FusedDev();
       f.g = g;
-      f.Fc = dFb[cur]; f.Fn = dFb[1 - cur]; f.Pc = dPsiB[cur]; f.Pn = dPsiB[1 - cur];
+      f.Fc = dFb[which]; f.Fn = dFb[1 - which]; f.Pc = dPsiB[which]; f.Pn = dPsiB[1 - which];
       f.cs = cs;
       f.mat = index_mode ? dMat : nullptr;
       f.coef = index_mode ? dTab : dCoefFull;
@@ -576,6 +574,20 @@ struct fdtd_sim {
       }
       slot.gen = args_gen;
     }
+    *out = &slot;
+    return 0;
+  }
+
+  // One fused time step.  dry: launch the kernels (they read the current set and write the next)
+  // but keep the current set and the probe recorder: the state is unchanged (used for timing).
+  int step(int n, int mode, bool timed, int *ev_slot, bool dry = false) {
+    if (!init_ready) return fail(FDTD_ERR_STATE, "Step before ReadySimulation");
+    const PWDev *pw = nullptr;
+    if (int rc = check_pw(n, mode, &pw)) return rc;
+    FusedDev f;
+    ArgSlot *slotp = nullptr;
+    if (int rc = fused_slot(cur, mode, pw, f, &slotp)) return rc;
+    ArgSlot &slot = *slotp;
     if (slot.nwg < 0) {
       if (dry) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
       // a grid outside the fused step's layouts (rows over 1024 columns, P over 64, ...): the
@@ -605,6 +617,298 @@ struct fdtd_sim {
     return record_probes(cur);
   }
 };
+
+
+// ------------------------------------------------------------------------------------------
+// Batched engine: many independent grids advanced together.  Every batch step launches each
+// kernel of the fused step once for all the grids still running (slab H + face sheets, the row
+// kernel per row-kernel variant, slab E per material mode, probes), so small grids fill the
+// chip together and the launch count per step does not grow with the number of grids.  Grids
+// whose layout the fused step does not cover run on their own afterwards.
+// ------------------------------------------------------------------------------------------
+// Grid size (cells) below which the automatic batch strategy merges a grid's launches with the
+// other small grids'; larger grids fill the GPU well enough on their own stream.
+constexpr size_t kBatchMergeCells = (size_t)1 << 18;
+
+struct fdtd_batch {
+  struct Item {
+    fdtd_sim *s;
+    int n0, count, mode;
+  };
+  std::vector<Item> items;
+  int strategy = 0;  // 0 automatic, 1 batched launches, 2 one stream per grid
+  // sub-batches run on their own streams (the GPU has four hardware queues per process), so the
+  // kernels of one sub-batch overlap those of another
+  static constexpr int kStreams = 4;
+  hipStream_t streams[kStreams] = {};
+  void *dtab = nullptr;
+  size_t dtab_bytes = 0;
+  std::vector<void *> dslab;  // per queued grid: slab argument blocks of sets 0 and 1
+  ~fdtd_batch() {
+    for (hipStream_t st : streams)
+      if (st) {
+        hipStreamSynchronize(st);
+        hipStreamDestroy(st);
+      }
+    if (dtab) hipFree(dtab);
+    for (void *p : dslab) hipFree(p);
+  }
+};
+
+namespace {
+
+struct BatchPlan {
+  // one epoch: the batch steps [t0, t1) with a fixed set of running grids
+  struct Launch {
+    int kind;  // 0 slab H, 1 rows, 2 slab E
+    RowsKey key;
+    bool idx = false, narrow = true;
+    size_t tab = 0;  // element offset of the table in the device buffer
+    int ng = 0, nwg = 0;
+  };
+  struct Epoch {
+    int t0 = 0, t1 = 0;
+    std::vector<Launch> launches;
+    size_t probes = 0;
+    int nprobe = 0, max_ant = 0;
+  };
+  std::vector<Epoch> epochs;
+};
+
+}  // namespace
+
+// One stream per grid: every grid's steps are queued at once on its own stream and the GPU
+// overlaps the grids as its hardware queues allow (the caller waits for the streams).
+static int batch_queue_streams(const std::vector<fdtd_batch::Item> &items) {
+  for (const auto &it : items) {
+    if (!it.s->init_ready) return fail(FDTD_ERR_STATE, "batch: ReadySimulation has not been called");
+    int slot = 0;
+    for (int q = 0; q < it.count; ++q)
+      if (int rc = it.s->step(it.n0 + q, it.mode, false, &slot)) return rc;
+  }
+  return 0;
+}
+
+static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &items);
+
+static int batch_run(fdtd_batch *b) {
+  // grids large enough to fill the GPU on their own run on their own streams; the others go
+  // through merged launches; both at once
+  std::vector<fdtd_batch::Item> big, small;
+  for (const auto &it : b->items) {
+    const bool own = b->strategy == 2 || (b->strategy == 0 && it.s->IJK >= kBatchMergeCells);
+    (own ? big : small).push_back(it);
+  }
+  if (int rc = batch_queue_streams(big)) return rc;
+  const int rc = small.empty() ? 0 : batch_run_merged(b, small);
+  for (const auto &it : big) HIPCHK(hipStreamSynchronize(it.s->stream));
+  return rc;
+}
+
+static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &items) {
+  std::vector<fdtd_batch::Item> fused, alone;
+  struct Prep {
+    fdtd_batch::Item it;
+    const void *rows_arg[2];
+    int rows_nwg;
+    RowsKey key;
+    const void *slab_arg[2];
+    unsigned nb_slab, nb_sheets;
+    bool narrow, idx;
+  };
+  std::vector<Prep> preps;
+  for (void *p : b->dslab) hipFree(p);
+  b->dslab.clear();
+  const size_t sab = slab_args_size();
+  for (const auto &it : items) {
+    fdtd_sim *s = it.s;
+    if (!s->init_ready) return fail(FDTD_ERR_STATE, "batch: ReadySimulation has not been called");
+    if (it.count <= 0) continue;
+    const PWDev *pw = nullptr;
+    if (int rc = s->check_pw(it.n0, it.mode, &pw)) return rc;
+    if (pw && it.n0 + it.count > s->T) return fail(FDTD_ERR_INVALID, "batch: steps beyond the source length T");
+    Prep pr;
+    pr.it = it;
+    bool ok = true;
+    std::vector<char> h0(sab), h1(sab);
+    for (int w = 0; w < 2 && ok; ++w) {
+      FusedDev f;
+      fdtd_sim::ArgSlot *slot = nullptr;
+      if (int rc = s->fused_slot(w, it.mode, pw, f, &slot)) return rc;
+      if (slot->nwg < 0) {
+        ok = false;
+        break;
+      }
+      pr.rows_arg[w] = slot->dptr;
+      pr.rows_nwg = slot->nwg;
+      pr.key = rows_key(slot->host.data());
+      make_slab_args(f, w ? h1.data() : h0.data(), &pr.nb_slab, &pr.nb_sheets, &pr.narrow);
+      pr.idx = f.mat != nullptr;
+    }
+    if (!ok) {
+      alone.push_back(it);
+      continue;
+    }
+    void *d = nullptr;
+    if (hipMalloc(&d, 2 * sab) != hipSuccess) return fail(FDTD_ERR_NOMEM, "batch: argument blocks");
+    b->dslab.push_back(d);
+    HIPCHK(hipMemcpyAsync(d, h0.data(), sab, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync((char *)d + sab, h1.data(), sab, hipMemcpyHostToDevice, s->stream));
+    pr.slab_arg[0] = d;
+    pr.slab_arg[1] = (char *)d + sab;
+    preps.push_back(pr);
+  }
+  // the batch streams run after everything already queued on the grids' own streams
+  for (int q = 0; q < fdtd_batch::kStreams; ++q)
+    if (!b->streams[q]) HIPCHK(hipStreamCreateWithFlags(&b->streams[q], hipStreamNonBlocking));
+  for (const auto &pr : preps) HIPCHK(hipStreamSynchronize(pr.it.s->stream));
+
+  // sub-batches balanced by cell updates (largest first onto the least loaded)
+  const int nsub = std::min<int>(fdtd_batch::kStreams, (int)preps.size());
+  std::vector<std::vector<const Prep *>> subs(nsub);
+  {
+    std::vector<const Prep *> order;
+    for (const auto &pr : preps) order.push_back(&pr);
+    std::sort(order.begin(), order.end(), [](const Prep *a, const Prep *c) {
+      const fdtd_sim *x = a->it.s, *y = c->it.s;
+      return (double)x->IJK * a->it.count > (double)y->IJK * c->it.count;
+    });
+    std::vector<double> load(nsub, 0.0);
+    for (const Prep *pr : order) {
+      const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      subs[q].push_back(pr);
+      load[q] += (double)pr->it.s->IJK * pr->it.count;
+    }
+  }
+  bool narrow_all = true;
+  for (const auto &pr : preps) narrow_all = narrow_all && pr.narrow;
+  std::vector<BatchGrid> tabs;
+  std::vector<ProbeJob> jobs;
+  std::vector<BatchPlan> plans(nsub);
+  for (int q = 0; q < nsub; ++q) {
+    // epochs: the running set changes only where a grid's step count ends
+    std::vector<int> ends;
+    for (const Prep *pr : subs[q]) ends.push_back(pr->it.count);
+    std::sort(ends.begin(), ends.end());
+    ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+    int t0 = 0;
+    for (int t1 : ends) {
+      BatchPlan::Epoch ep;
+      ep.t0 = t0;
+      ep.t1 = t1;
+      std::vector<const Prep *> run;
+      for (const Prep *pr : subs[q])
+        if (pr->it.count > t0) run.push_back(pr);
+      auto add_table = [&](int kind, const std::vector<const Prep *> &grids, RowsKey key, bool idx) {
+        BatchPlan::Launch L;
+        L.kind = kind;
+        L.key = key;
+        L.idx = idx;
+        L.narrow = narrow_all;
+        L.tab = tabs.size();
+        L.ng = (int)grids.size();
+        int off = 0;
+        for (const Prep *pr : grids) {
+          BatchGrid bg;
+          bg.off = off;
+          bg.cur0 = pr->it.s->cur ^ (t0 & 1);
+          bg.n0 = pr->it.n0 + t0;
+          for (int w = 0; w < 2; ++w) bg.arg[w] = kind == 1 ? pr->rows_arg[w] : pr->slab_arg[w];
+          tabs.push_back(bg);
+          off += kind == 0 ? (int)(pr->nb_slab + pr->nb_sheets) : kind == 1 ? pr->rows_nwg : (int)pr->nb_slab;
+        }
+        BatchGrid end;
+        end.off = off;
+        tabs.push_back(end);
+        L.nwg = off;
+        ep.launches.push_back(L);
+      };
+      add_table(0, run, RowsKey(), false);
+      std::map<RowsKey, std::vector<const Prep *>> by_key;
+      for (const Prep *pr : run) by_key[pr->key].push_back(pr);
+      for (auto &kv : by_key) add_table(1, kv.second, kv.first, false);
+      for (int idx = 0; idx < 2; ++idx) {
+        std::vector<const Prep *> grp;
+        for (const Prep *pr : run)
+          if (pr->idx == (idx == 1)) grp.push_back(pr);
+        if (!grp.empty()) add_table(2, grp, RowsKey(), idx == 1);
+      }
+      ep.probes = jobs.size();
+      for (const Prep *pr : run) {
+        fdtd_sim *s = pr->it.s;
+        if (!s->dPort) continue;
+        ProbeJob jb;
+        jb.F[0] = s->fields(0);
+        jb.F[1] = s->fields(1);
+        jb.idx = s->dProbeIdx;
+        jb.comps = s->dProbeComps;
+        jb.port = s->dPort;
+        jb.nAnt = s->nAnt;
+        jb.ncomp = (int)s->probe_comps.size();
+        jb.row0 = s->rec_step + t0;
+        jb.T = s->T;
+        jb.cur0 = s->cur ^ (t0 & 1);
+        jobs.push_back(jb);
+        ep.max_ant = std::max(ep.max_ant, s->nAnt);
+      }
+      ep.nprobe = (int)(jobs.size() - ep.probes);
+      plans[q].epochs.push_back(ep);
+      t0 = t1;
+    }
+  }
+  // one device buffer: all tables, then all probe jobs (uploaded on stream 0; the others wait)
+  const size_t tab_bytes = tabs.size() * sizeof(BatchGrid), job_bytes = jobs.size() * sizeof(ProbeJob);
+  if (tab_bytes + job_bytes > b->dtab_bytes) {
+    if (b->dtab) {
+      for (hipStream_t st : b->streams) HIPCHK(hipStreamSynchronize(st));
+      hipFree(b->dtab);
+    }
+    b->dtab = nullptr;
+    b->dtab_bytes = 0;
+    if (hipMalloc(&b->dtab, tab_bytes + job_bytes) != hipSuccess) return fail(FDTD_ERR_NOMEM, "batch: tables");
+    b->dtab_bytes = tab_bytes + job_bytes;
+  }
+  hipStream_t s0 = b->streams[0];
+  if (tab_bytes) HIPCHK(hipMemcpyAsync(b->dtab, tabs.data(), tab_bytes, hipMemcpyHostToDevice, s0));
+  if (job_bytes) HIPCHK(hipMemcpyAsync((char *)b->dtab + tab_bytes, jobs.data(), job_bytes, hipMemcpyHostToDevice, s0));
+  HIPCHK(hipStreamSynchronize(s0));
+  const BatchGrid *dt = (const BatchGrid *)b->dtab;
+  const ProbeJob *dj = (const ProbeJob *)((char *)b->dtab + tab_bytes);
+  // interleave the sub-batches step by step so every stream stays fed
+  std::vector<size_t> epi(nsub, 0);
+  int tmax = 0;
+  for (const auto &pr : preps) tmax = std::max(tmax, pr.it.count);
+  for (int t = 0; t < tmax; ++t) {
+    for (int q = 0; q < nsub; ++q) {
+      auto &eps = plans[q].epochs;
+      while (epi[q] < eps.size() && t >= eps[epi[q]].t1) ++epi[q];
+      if (epi[q] >= eps.size()) continue;
+      const BatchPlan::Epoch &ep = eps[epi[q]];
+      const int step = t - ep.t0;
+      hipStream_t st = b->streams[q];
+      for (const auto &L : ep.launches)
+        if (L.kind == 0) HIPCHK(launch_slab_h_batch(st, L.narrow, dt + L.tab, L.ng, L.nwg, step));
+      for (const auto &L : ep.launches)
+        if (L.kind == 1) HIPCHK(launch_rows_batch(st, L.key, dt + L.tab, L.ng, L.nwg, step));
+      for (const auto &L : ep.launches)
+        if (L.kind == 2) HIPCHK(launch_slab_e_batch(st, L.idx, L.narrow, dt + L.tab, L.ng, L.nwg, step));
+      HIPCHK(launch_probes_batch(st, dj + ep.probes, ep.nprobe, ep.max_ant, step));
+    }
+  }
+  for (int q = 0; q < nsub; ++q) HIPCHK(hipStreamSynchronize(b->streams[q]));
+  for (const auto &pr : preps) {
+    fdtd_sim *s = pr.it.s;
+    s->cur ^= pr.it.count & 1;
+    s->rec_step += pr.it.count;
+  }
+  // grids outside the fused layouts: their own steps
+  for (const auto &it : alone) {
+    int slot = 0;
+    for (int q = 0; q < it.count; ++q)
+      if (int rc = it.s->step(it.n0 + q, it.mode, false, &slot)) return rc;
+  }
+  return 0;
+}
 
 // ------------------------------------------------------------------------------------------
 // C ABI
@@ -841,6 +1145,45 @@ int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode) {
   NEED(s);
   int slot = 0;
   return s->step(n, mode, false, &slot);
+}
+
+int fdtd_batch_create(fdtd_batch **out) {
+  if (!out) return fail(FDTD_ERR_INVALID, "null output pointer");
+  *out = new fdtd_batch();
+  return 0;
+}
+
+int fdtd_batch_destroy(fdtd_batch *b) {
+  delete b;
+  return 0;
+}
+
+int fdtd_batch_add(fdtd_batch *b, fdtd_sim *s, int32_t n0, int32_t count, int32_t mode) {
+  if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
+  NEED_READY(s);
+  if (count < 0 || n0 < 0) return fail(FDTD_ERR_INVALID, "negative step index or count");
+  for (const auto &it : b->items)
+    if (it.s == s) return fail(FDTD_ERR_INVALID, "simulation already queued in this batch");
+  b->items.push_back({s, n0, count, mode});
+  return 0;
+}
+
+int fdtd_batch_clear(fdtd_batch *b) {
+  if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
+  b->items.clear();
+  return 0;
+}
+
+int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy) {
+  if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
+  if (strategy < 0 || strategy > 2) return fail(FDTD_ERR_INVALID, "strategy must be 0, 1 or 2");
+  b->strategy = strategy;
+  return 0;
+}
+
+int fdtd_batch_run(fdtd_batch *b) {
+  if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
+  return batch_run(b);
 }
 
 int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms) {

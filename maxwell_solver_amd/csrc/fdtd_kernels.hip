@@ -809,7 +809,7 @@ __device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, co
 // are exactly 1 on these rows and planes and their factors are omitted; 1/kappa_x is applied in
 // the x-slab columns (it is 1, an exact no-op, elsewhere).
 template <bool IDX, bool BUF, int CPL>
-__global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
+__device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa, int L) {
   using O = typename Acc<BUF>::O;
   constexpr int W = 64 * CPL;
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
@@ -845,7 +845,6 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
   // offsets and flags in scalar registers (and branch on them) instead of per-lane values
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = wv % S, r = wv / S;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int ty = L % nTy, tz = L / nTy;
   const int j = uni(P + R * ty + r);
   const int kb = P + tz * KC, ke = min(kb + KC, K - P);
@@ -1198,6 +1197,35 @@ __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, i
   }
 }
 
+template <bool IDX, bool BUF, int CPL>
+__global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
+  rows_body<IDX, BUF, CPL>(pa, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Batched launches cover the workgroups of many grids: tab[g].off is grid g's first workgroup
+// (tab[ng].off the total), arg[p] its argument block while its current state set is p.  At batch
+// step `step` grid g is at time index n0 + step with current set cur0 ^ (step & 1).
+__device__ __forceinline__ int find_grid(const BatchGrid *__restrict__ tab, int ng, int b) {
+  int lo = 0, hi = ng;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tab[mid].off <= b) lo = mid;
+    else hi = mid;
+  }
+  return uni(lo);
+}
+
+// Blocks of one grid whose local index agrees mod 8 run on the same XCD (blocks are dealt to the
+// XCDs round-robin), so the per-grid remap is xcd_remap of the local index.
+template <bool IDX, bool BUF, int CPL>
+__global__ void __launch_bounds__(256) k_rows_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
+  const int b = blockIdx.x, g = find_grid(tab, ng, b);
+  const int par = (tab[g].cur0 + step) & 1;
+  const int off = tab[g].off, nb = tab[g + 1].off - off;
+  rows_body<IDX, BUF, CPL>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
+}
+
+
 // Decode thread t of the slab kernels into a cell of the y and z slabs (z slabs, then the y slabs
 // of the remaining planes).  32-bit arithmetic when the slab cell count allows it.
 template <typename IT>
@@ -1235,20 +1263,27 @@ struct SlabArgs {
   int xni, xnj, xnk;
 };
 
+// The argument block of a batched slab workgroup, copied to LDS once (it is read field by field).
+__device__ __forceinline__ void load_slab_args(SlabArgs &dst, const void *src) {
+  const uint32_t *s = (const uint32_t *)src;
+  for (int q = threadIdx.x; q < (int)(sizeof(SlabArgs) / 4); q += blockDim.x) ((uint32_t *)&dst)[q] = s[q];
+  __syncthreads();
+}
+
 // A: H cells of the y and z CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
 template <typename IT>
-__global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
-  if (blockIdx.x >= a.nbs) {
-    const long t = (long)(blockIdx.x - a.nbs) * 256 + threadIdx.x;
+__device__ __forceinline__ void slab_h_body(const SlabArgs &a, unsigned blk, int n) {
+  if (blk >= a.nbs) {
+    const long t = (long)(blk - a.nbs) * 256 + threadIdx.x;
     if (t < 8L * ((long)a.xnj * a.xnk + (long)a.xni * a.xnk + (long)a.xni * a.xnj)) {
-      const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
-      const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
+      const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
+      const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
       tfsf_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xni, a.xnj, a.xnk, t);
     }
     return;
   }
   int i, j, k;
-  if (!slab_cell<IT>((IT)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  if (!slab_cell<IT>((IT)blk * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
   const int I = a.g.I, J = a.g.J, K = a.g.K;
   const size_t IJ = (size_t)I * J;
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
@@ -1263,19 +1298,34 @@ __global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
   e.ex_k1 = kn ? Ex[id + IJ] : 0.0f;
   e.ey_k1 = kn ? Ey[id + IJ] : 0.0f;
   float hx = a.cur.f[HX][id], hy = a.cur.f[HY][id], hz = a.cur.f[HZ][id];
-  const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)a.n * a.pw.N1 : nullptr;
+  const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
   h_cell(a.g, a.co, a.pw, rowE, a.pcur, a.pnxt, true, i, j, k, IJ, e, hx, hy, hz);
   a.nxt.f[HX][id] = hx;
   a.nxt.f[HY][id] = hy;
   a.nxt.f[HZ][id] = hz;
 }
 
+template <typename IT>
+__global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
+  slab_h_body<IT>(a, blockIdx.x, a.n);
+}
+
+template <typename IT>
+__global__ void __launch_bounds__(256) k_slab_h_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
+  __shared__ __align__(16) char raw[sizeof(SlabArgs)];
+  SlabArgs &sa = *reinterpret_cast<SlabArgs *>(raw);
+  const int b = blockIdx.x, g = find_grid(tab, ng, b);
+  const int par = (tab[g].cur0 + step) & 1;
+  load_slab_args(sa, tab[g].arg[par]);
+  slab_h_body<IT>(sa, b - tab[g].off, tab[g].n0 + step);
+}
+
 // C: E cells of the y and z CPML slabs (src/gpu.h:727-763 + plane wave + src/gpu.h:765-868),
 // reading the new H of kernels A and B.
 template <bool IDX, typename IT>
-__global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
+__device__ __forceinline__ void slab_e_body(const SlabArgs &a, unsigned blk, int n) {
   int i, j, k;
-  if (!slab_cell<IT>((IT)blockIdx.x * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
+  if (!slab_cell<IT>((IT)blk * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
   const int I = a.g.I, J = a.g.J;
   const size_t IJ = (size_t)I * J;
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
@@ -1291,11 +1341,37 @@ __global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
   float ce[3], ch[3];
   cell_coef<IDX>(a.co, a.co.tab, id, i, I, ce, ch);
   float ex = a.cur.f[EX][id], ey = a.cur.f[EY][id], ez = a.cur.f[EZ][id];
-  const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)a.n * a.pw.N1 : nullptr;
+  const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
   e_cell(a.g, a.co, a.pw, rowH, a.pcur, a.pnxt, i, j, k, IJ, ce, ch, h, ex, ey, ez);
   a.nxt.f[EX][id] = ex;
   a.nxt.f[EY][id] = ey;
   a.nxt.f[EZ][id] = ez;
+}
+
+template <bool IDX, typename IT>
+__global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
+  slab_e_body<IDX, IT>(a, blockIdx.x, a.n);
+}
+
+template <bool IDX, typename IT>
+__global__ void __launch_bounds__(256) k_slab_e_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
+  __shared__ __align__(16) char raw[sizeof(SlabArgs)];
+  SlabArgs &sa = *reinterpret_cast<SlabArgs *>(raw);
+  const int b = blockIdx.x, g = find_grid(tab, ng, b);
+  const int par = (tab[g].cur0 + step) & 1;
+  load_slab_args(sa, tab[g].arg[par]);
+  slab_e_body<IDX, IT>(sa, b - tab[g].off, tab[g].n0 + step);
+}
+
+// Probe gather of many grids after a batch step: job blockIdx.y, probes blockIdx.x * 64 + lane,
+// sampled from the set that is current after the step.
+__global__ void __launch_bounds__(64) k_probes_batch(const ProbeJob *__restrict__ jobs, int step) {
+  const ProbeJob &jb = jobs[blockIdx.y];
+  const int a = blockIdx.x * 64 + threadIdx.x, row = jb.row0 + step;
+  if (a >= jb.nAnt || row >= jb.T) return;
+  const int cur = (jb.cur0 + step + 1) & 1;
+  for (int q = 0; q < jb.ncomp; ++q)
+    jb.port[((size_t)q * jb.T + row) * jb.nAnt + a] = jb.F[cur].f[jb.comps[q]][jb.idx[a]];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1540,6 +1616,78 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
   s.xnj = g.j1 - g.j0 + 1;
   s.xnk = g.k1 - g.k0 + 1;
   return s;
+}
+
+RowsKey rows_key(const void *hargs) {
+  const StepArgs &a = *(const StepArgs *)hargs;
+  const size_t lim = ((size_t)1 << 32) - 1;
+  RowsKey k;
+  k.idx = a.mat != nullptr;
+  k.buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
+  k.cpl = a.CPL;
+  k.threads = 64 * a.S * a.R;
+  return k;
+}
+
+size_t slab_args_size() { return sizeof(SlabArgs); }
+
+void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow) {
+  const GridDev &g = f.g;
+  SlabArgs sa = slab_args(f, 0);
+  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P);
+  const unsigned nb = cdiv(nslab, 256);
+  sa.nbs = nb;
+  const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
+  const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
+  std::memcpy(out, &sa, sizeof(sa));
+  *nb_slab = nb;
+  *nb_sheets = sheets ? cdiv(nsheet, 256) : 0;
+  *narrow = (long)nb * 256 < (1L << 31);
+}
+
+template <bool IDX, bool BUF>
+static void launch_rows_batch_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const BatchGrid *tab, int ng, int step) {
+  if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4>), grid, blk, 0, st, tab, ng, step);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2>), grid, blk, 0, st, tab, ng, step);
+  else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1>), grid, blk, 0, st, tab, ng, step);
+}
+
+hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, const BatchGrid *tab, int ng, int nwg, int step) {
+  if (nwg <= 0) return hipSuccess;
+  const dim3 grid(nwg), blk(key.threads);
+  if (key.idx) {
+    if (key.buf) launch_rows_batch_w<true, true>(st, key.cpl, grid, blk, tab, ng, step);
+    else launch_rows_batch_w<true, false>(st, key.cpl, grid, blk, tab, ng, step);
+  } else {
+    if (key.buf) launch_rows_batch_w<false, true>(st, key.cpl, grid, blk, tab, ng, step);
+    else launch_rows_batch_w<false, false>(st, key.cpl, grid, blk, tab, ng, step);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step) {
+  if (nwg <= 0) return hipSuccess;
+  if (narrow) hipLaunchKernelGGL(k_slab_h_batch<uint32_t>, dim3(nwg), dim3(256), 0, st, tab, ng, step);
+  else hipLaunchKernelGGL(k_slab_h_batch<uint64_t>, dim3(nwg), dim3(256), 0, st, tab, ng, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_slab_e_batch(hipStream_t st, bool idx, bool narrow, const BatchGrid *tab, int ng, int nwg, int step) {
+  if (nwg <= 0) return hipSuccess;
+  if (idx) {
+    if (narrow) hipLaunchKernelGGL((k_slab_e_batch<true, uint32_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
+    else hipLaunchKernelGGL((k_slab_e_batch<true, uint64_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
+  } else {
+    if (narrow) hipLaunchKernelGGL((k_slab_e_batch<false, uint32_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
+    else hipLaunchKernelGGL((k_slab_e_batch<false, uint64_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, int max_ant, int step) {
+  if (njobs <= 0 || max_ant <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_probes_batch, dim3(cdiv(max_ant, 64), njobs), dim3(64), 0, st, jobs, step);
+  return hipGetLastError();
 }
 
 // One time step: A (slab H + x-face sheets), B (rows), C (slab E).  ev_rows (optional): two

@@ -85,4 +85,39 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
                              hipEvent_t *ev_rows = nullptr);
 bool fused_supported(const GridDev &g);
 
+// ---- batched fused step: one launch per kernel covers many grids ----
+// Per grid of a batched launch: first workgroup, current set and time index at batch step 0, and
+// the argument block (device copy) to use while the current set is 0 or 1.  A table has ng + 1
+// entries; the last holds the total workgroup count in `off`.
+struct BatchGrid {
+  int off = 0, cur0 = 0, n0 = 0, pad = 0;
+  const void *arg[2] = {nullptr, nullptr};
+};
+// Probe gather of one grid in a batch: the field sets 0 and 1, probe cells, components (0..5),
+// record [ncomp][T][nAnt] and the record row at batch step 0.
+struct ProbeJob {
+  FieldSet F[2];
+  const int *idx = nullptr;
+  const int *comps = nullptr;
+  float *port = nullptr;
+  int nAnt = 0, ncomp = 0, row0 = 0, T = 0, cur0 = 0, pad = 0;
+};
+// Row-kernel variant of a grid (grids of one batched row launch share it).
+struct RowsKey {
+  bool idx = false, buf = false;
+  int cpl = 1, threads = 256;
+  bool operator<(const RowsKey &o) const {
+    return idx != o.idx ? idx < o.idx : buf != o.buf ? buf < o.buf : cpl != o.cpl ? cpl < o.cpl : threads < o.threads;
+  }
+};
+RowsKey rows_key(const void *hargs);
+// Slab-kernel argument block of f (host copy; n is supplied per step) with its workgroup counts:
+// slab cells, and the face-sheet workgroups k_slab_h adds.
+size_t slab_args_size();
+void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow);
+hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, const BatchGrid *tab, int ng, int nwg, int step);
+hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
+hipError_t launch_slab_e_batch(hipStream_t st, bool idx, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
+hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, int max_ant, int step);
+
 }  // namespace fdtd

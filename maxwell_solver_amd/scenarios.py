@@ -3,7 +3,8 @@
 The reference fixes no seeds; these generators use SplitMix64 with seed 0x5EED0000 + workload
 number so that every run and every backend (HIP library, CPU oracle) sees the same bytes.
 Workloads: 1 = 64^3 vacuum, axis plane wave; 2 = 256^3 with 40 dielectric spheres and an oblique
-plane wave; 3 = 384x384x256 lossy layered ground under a vertical plane wave.
+plane wave; 3 = 384x384x256 lossy layered ground under a vertical plane wave; 4 = 96 independent
+grids of 32..384 cells per edge (the batched engine).
 """
 from __future__ import annotations
 
@@ -235,6 +236,42 @@ def config3(T: int = 4000, I: int = 384, J: int = 384, K: int = 256, lattice: in
     return Scenario("config3", I, J, K, T, dx, dt, fc, (P, 3.0, 5.0, 0.05, 1.0, 0.8), inset, 2, (1.0,),
                     gauss_pulse(T, dt, fc), mat_index=idx, mat_table=tab, probes=probes, probe_comps=(1, 5),
                     seed=seed)
+
+
+def config4(n_grids: int = 96, edge_lo: float = 32, edge_hi: float = 192) -> list:
+    """Many independent grids: base edge log-uniform in [32, 192], axes base * a_x, a_y, a_z with
+    a in U[1, 2] normalised to a unit product (volume base^3, max/min <= 2), edges rounded to
+    multiples of 4 (at least 32); 300..1500 steps; 0..20 spheres as in
+    workload 2; 8-cell CPML; each grid its own oblique plane wave (seeded angles) and
+    f0 in [1, 3] GHz; 4 probes recording Ez.  Returns (Scenario, steps) pairs."""
+    seed = 0x5EED0000 + 4
+    rng = SplitMix64(seed)
+    P = 8
+    cpml = (P, 3.0, 5.0, 0.05, 1.0, 0.8)
+    dx = 0.01
+    dt = _dt(dx)
+    out = []
+    for g in range(n_grids):
+        base = rng.log_uniform(edge_lo, edge_hi)
+        a = [rng.uniform(1.0, 2.0) for _ in range(3)]
+        gm = (a[0] * a[1] * a[2]) ** (1.0 / 3.0)
+        I, J, K = (max(32, 4 * int(round(base * x / gm / 4))) for x in a)
+        steps = rng.randint(300, 1500)
+        f0 = rng.uniform(1e9, 3e9)
+        inset = P + 5
+        box = (inset, I - inset - 1, inset, J - inset - 1, inset, K - inset - 1)
+        half = min(box[1] - box[0], box[3] - box[2], box[5] - box[4]) / 2
+        rmax = min(20.0, half - 3)
+        n_sph = rng.randint(0, 20) if rmax >= 2.0 else 0
+        idx = tab = None
+        if n_sph:
+            idx, tab = spheres(rng, I, J, K, box, n_sph, min(4.0, rmax), rmax, (2.0, 12.0), (0.0, 0.05))
+        theta, phi, psi = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
+        sc = Scenario(f"config4_{g}", I, J, K, steps, dx, dt, f0, cpml, inset, 5, (theta, phi, psi),
+                      gauss_sine(steps, dt, f0), mat_index=idx, mat_table=tab,
+                      probes=_probes(rng, 4, I, J, K, P), probe_comps=(3,), seed=seed + g)
+        out.append((sc, steps))
+    return out
 
 
 def small_variant(sc: Scenario, N: int, T: int) -> Scenario:

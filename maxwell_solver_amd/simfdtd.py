@@ -26,7 +26,8 @@ EXPORTS = [
     "fdtd_set_cpml", "fdtd_set_tfsf", "fdtd_set_plane_wave", "fdtd_set_material_index",
     "fdtd_set_probe_components", "fdtd_step", "fdtd_run", "fdtd_snapshot_count",
     "fdtd_output_snapshots", "fdtd_synchronize", "fdtd_read_array", "fdtd_prepare",
-    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_autotune", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
+    "fdtd_read_setup", "fdtd_set_tuning", "fdtd_autotune", "fdtd_batch_create", "fdtd_batch_destroy",
+    "fdtd_batch_add", "fdtd_batch_clear", "fdtd_batch_run", "fdtd_batch_set_strategy", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
 ]
 
 
@@ -65,6 +66,9 @@ def lib():
             "fdtd_read_setup": [vp, i32, fp, u64p], "fdtd_set_tuning": [vp, i32, i32, i32],
             "fdtd_set_timing": [vp, i32], "fdtd_kernel_time": [vp, dp, C.POINTER(C.c_int64)],
             "fdtd_row_kernel_time": [vp, dp], "fdtd_autotune": [vp, i32, i32, i32, dp],
+            "fdtd_batch_create": [C.POINTER(vp)], "fdtd_batch_destroy": [vp],
+            "fdtd_batch_add": [vp, vp, i32, i32, i32], "fdtd_batch_clear": [vp], "fdtd_batch_run": [vp],
+            "fdtd_batch_set_strategy": [vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -288,6 +292,43 @@ class Simulation:
         out = np.empty((n.value, I * J * K), dtype=np.float32)
         self._chk(lib().fdtd_output_snapshots(self._h, _p(out, C.c_float), out.size))
         return out
+
+
+class Batch:
+    """Many independent simulations advanced together (fdtd_batch_*): each batch step launches every
+    kernel of the fused step once for all grids still running."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        Simulation._chk(lib().fdtd_batch_create(C.byref(h)))
+        self._h = h
+        self._sims = []
+
+    def add(self, sim, n0, count, mode=0):
+        Simulation._chk(lib().fdtd_batch_add(self._h, sim._h, int(n0), int(count), int(mode)))
+        self._sims.append(sim)
+
+    def clear(self):
+        Simulation._chk(lib().fdtd_batch_clear(self._h))
+        self._sims = []
+
+    def run(self):
+        Simulation._chk(lib().fdtd_batch_run(self._h))
+
+    def set_strategy(self, strategy):
+        """0 automatic, 1 merged launches, 2 one stream per grid."""
+        Simulation._chk(lib().fdtd_batch_set_strategy(self._h, int(strategy)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------------------------
