@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <type_traits>
 
 #include "fdtd_kernels.hpp"
 
@@ -617,6 +618,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // row / plane / component neighbours of a cell cost no 64-bit address registers; out-of-range
 // buffer accesses read 0 and drop stores.  Global mode: 64-bit pointer arithmetic.
 #define AS1 __attribute__((address_space(1)))
+#define AS4 __attribute__((address_space(4)))
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
@@ -749,6 +751,12 @@ __device__ __forceinline__ const T *opaque(const T *p) {
   return p;
 }
 
+template <typename T>
+__device__ __forceinline__ const AS4 T *opaque_c(const AS4 T *p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // TF/SF face sheets: the incident values the six faces of the TF/SF box need at time step n,
 // computed once per step by the slab-H launch, so that the row kernel applies every TF/SF
 // correction with one load.  Sheet q = 4 eh + 2 hi + u of face axis a (x, y, z): update eh (0: H,
@@ -808,14 +816,39 @@ __device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, co
 // rows j and j+1, plane k+1 and the field component are uniform offsets.  1/kappa_y and 1/kappa_z
 // are exactly 1 on these rows and planes and their factors are omitted; 1/kappa_x is applied in
 // the x-slab columns (it is 1, an exact no-op, elsewhere).
+//
+// Two instantiations of the march share the kernel: plain waves (every lane in the grid, no x-slab
+// column, no column on an x face of the TF/SF box, a row no y face of the box touches) carry no
+// per-lane branches; edge waves hold their per-lane constants (x-slab profile values, 1/kappa_x,
+// predicate bits, face columns) in registers across the march.  (Issuing the next plane's loads
+// before computing the current one was measured slower at 256^3: the second plane buffer costs
+// occupancy, and the plain march already streams within 5% of a pure 7-in / 6-out copy of the
+// same arrays.)
+
+// Loads of one plane of the march (see rows_body).
+template <int CPL>
+struct RowPlane {
+  float nx[CPL], ny[CPL], nz[CPL], nym[CPL];     // E(k+1) of row j, Ey(k+1) of row j-1
+  float exm[CPL], ezm[CPL], exq[CPL], ezq[CPL];  // Ex, Ez of rows j-1 and j+1 at plane k
+  float hx[CPL], hy[CPL], hz[CPL];               // H of row j at plane k (old state)
+  float hxm[CPL], hzm[CPL];                      // Hx, Hz of row j-1 (old, or new if not recomputed)
+  int mt[CPL];                                   // material index (index mode)
+  float pHy[CPL], pHz[CPL], pHzm[CPL], pEy[CPL], pEz[CPL];  // x-slab psi (edge waves)
+  float vhy, vhz, vhzm, vez, vey;                // x-face incident values (edge waves)
+};
+
 template <bool IDX, bool BUF, int CPL>
-__device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa, int L) {
+__device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
+  // The argument block does not change during a launch: read through the constant address space
+  // it is loaded with scalar loads into scalar registers (also where the block's address was
+  // itself read from memory, as in the batched launch).
+  const AS4 StepArgs *pa = (const AS4 StepArgs *)pa_in;
   using O = typename Acc<BUF>::O;
   constexpr int W = 64 * CPL;
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
   const int S = pa->S, R = pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
-  const int act_h = pa->act_h, act_e = pa->act_e;
+  const int act_h = uni(pa->act_h), act_e = uni(pa->act_e);
   const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J;
   const float Chh = pa->Chh, Che = pa->Che, dx = pa->dx;
   const int P2 = 2 * P;
@@ -854,347 +887,483 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa, int L
   const bool on = row_ok && ib < I;     // I is a multiple of CPL: all CPL columns are in the grid
   const bool jm_C = j - 1 >= P;         // row j-1 is recomputed (else: a y-slab row, read new)
 
-  // x-slab index of each column (-1 outside the x slabs)
-  int sl[CPL];
-  bool slab_lane = false;
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int i = ib + c;
-    sl[c] = !on ? -1 : (i < P ? i : (i >= I - P ? i - (I - P2) : -1));
-    slab_lane |= sl[c] >= 0;
-  }
-
-  // x-face TF/SF columns of the lane (the faces are at least CPL columns apart: at most one each)
-  int xh_c = -1, xe_c = -1;
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int i = ib + c;
-    if (on && act_h && (i == bi0 - 1 || i == bi1)) xh_c = c;
-    if (on && act_e && (i == bi0 || i == bi1)) xe_c = c;
-  }
-  const bool xh_lo = xh_c >= 0 && ib + xh_c == bi0 - 1, xe_lo = xe_c >= 0 && ib + xe_c == bi0;
-  // face sheets: x sheets (xnj x xnk each) first, then y (xni x xnk), then z (xni x xnj)
-  const int xni = pa->xni, xnj = pa->xnj;
-  const size_t xS = (size_t)xnj * pa->xnk, yS = (size_t)xni * pa->xnk, zS = (size_t)xni * xnj;
-  const float *xs = pa->xs;
-  const float *ys = xs + 8 * xS, *zs = ys + 8 * yS;
-  const uint32_t xh_sel = xh_lo ? 0u : (uint32_t)(2 * xS), xe_sel = xe_lo ? (uint32_t)(4 * xS) : (uint32_t)(6 * xS);
-  const bool xj0 = j >= bj0 && j <= bj1, xj1 = j >= bj0 && j <= bj1 - 1;
-  const bool xjm1 = j - 1 >= bj0 && j - 1 <= bj1 - 1;
-
   // arrays and uniform offsets (bytes): component c at c*cs, row j-1 / j / j+1 at 0 / I / 2I,
   // plane k+1 at IJ
-  const size_t sx = pa->sx;
   const Acc<BUF> Fc(pa->Fc, 6 * cs * 4), Fn(pa->Fn, 6 * cs * 4);
   // H of row j-1: the old state where it is recomputed, else the new state
   const Acc<BUF> Fm(uni(jm_C ? pa->Fc : (const float *)pa->Fn), 6 * cs * 4);
-  const Acc<BUF> Pc(pa->Pc, 16 * (sx + pa->sy + pa->sz)), Pn(pa->Pn, 16 * (sx + pa->sy + pa->sz));
   const Acc<BUF> Mt(pa->mat, IDX ? IJK : 0), Cf(pa->coef, IDX ? 0 : 24 * IJK);
   const O c1 = (O)(cs * 4), c2 = 2 * c1, c3 = 3 * c1, c4 = 4 * c1, c5 = 5 * c1;
   const O r1 = (O)(I * 4), r2 = 2 * r1, pz = (O)(IJ * 4);
-  // x-slab psi groups [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k
-  const O gEz = (O)(sx * 4), gHy = 2 * gEz, gHz = 3 * gEz;
 
   const int k_first = uni(kb > P ? kb - 1 : kb), n_planes = uni(ke - k_first);
-  const int k = k_first;
   const size_t col = (on ? (size_t)ib : 0) + (size_t)(j - 1) * I;
-  O o = (O)((col + (size_t)k * IJ) * 4);
-  O so[CPL];  // psi byte offsets of the columns within a (row, plane)
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 4);
+  const O o0 = (O)((col + (size_t)k_first * IJ) * 4);
 
-  // E of row j at plane k, and Ey of row j-1 at plane k (carried from plane to plane)
-  float ex[CPL], ey[CPL], ez[CPL], eym[CPL];
-  float hpx[CPL], hpy[CPL];  // H(i, j, k-1)
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) ex[c] = ey[c] = ez[c] = eym[c] = hpx[c] = hpy[c] = 0.0f;
-  if (on) {
-    Fc.ld(o + r1, 0, ex); Fc.ld(o + r1, c1, ey); Fc.ld(o + r1, c2, ez);
-    Fc.ld(o, c1, eym);
-    if (kb == P) {  // plane P-1 is a z-slab plane: its new H is already written
-      Fn.ld(o + r1 - pz, c3, hpx);
-      Fn.ld(o + r1 - pz, c4, hpy);
-    }
+  // plain wave (see above); the z faces of the box are applied on their planes under uniform
+  // branches in both marches
+  bool plain = row_ok && seg * W >= P && (seg + 1) * W <= I - P;
+  if (act_h | act_e) {
+    const int lo = seg * W, hi = lo + W;
+    plain = plain && !(bi0 - 1 >= lo && bi0 - 1 < hi) && !(bi0 >= lo && bi0 < hi) && !(bi1 >= lo && bi1 < hi);
+    plain = plain && j != bj0 - 1 && j != bj0 && j != bj1 && j != bj1 + 1;
   }
-  // E at the first column of the next segment (plane k), for the lane's last column
-  float bey = 0, bez = 0, beym = 0;
-  int par = 0;
-  if (S > 1) {
-    if (lane == 0) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
-    lds_barrier();
-    if (lane == 63 && seg + 1 < S) { bey = sX[0][wv + 1][0]; bez = sX[0][wv + 1][1]; beym = sX[0][wv + 1][2]; }
-    par = 1;
-  }
+  const bool zrow = j >= bj0 && j <= bj1, zrow_m = j - 1 >= bj0 && j - 1 <= bj1 - 1;
 
-  for (int it = 0; it < n_planes; ++it, o += pz) {
-    const int k = uni(k_first + it);
-    const bool own = k >= kb;  // planes of the chunk (the plane below only feeds H(k-1))
-    const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
-    // ---- loads of this plane (independent, issued together) ----
-    float nx[CPL], ny[CPL], nz[CPL], nym[CPL];        // E(k+1) row j, Ey(k+1) row j-1
-    float exm[CPL], ezm[CPL], exq[CPL], ezq[CPL];     // Ex, Ez of rows j-1 and j+1
-    float hx[CPL], hy[CPL], hz[CPL], hxm[CPL], hzm[CPL];
-    float pHy[CPL], pHz[CPL], pHzm[CPL], pEy[CPL], pEz[CPL];
-    int mt[CPL];
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      nx[c] = ny[c] = nz[c] = nym[c] = exm[c] = ezm[c] = exq[c] = ezq[c] = 0.0f;
-      hx[c] = hy[c] = hz[c] = hxm[c] = hzm[c] = 0.0f;
-      pHy[c] = pHz[c] = pHzm[c] = pEy[c] = pEz[c] = 0.0f;
-      mt[c] = 0;
-    }
-    // lane offsets of rows j-1 (o), j (o1) and j+1 (o2); uniform offsets: component, plane k+1
-    const O o1 = o + r1, o2 = o + r2;
-    if (on) {
-      Fc.ld(o1, pz, nx); Fc.ld(o1, c1 + pz, ny); Fc.ld(o1, c2 + pz, nz);
-      Fc.ld(o, c1 + pz, nym);
-      Fc.ld(o, 0, exm); Fc.ld(o, c2, ezm);
-      Fc.ld(o2, 0, exq); Fc.ld(o2, c2, ezq);
-      Fc.ld(o1, c3, hx); Fc.ld(o1, c4, hy); Fc.ld(o1, c5, hz);
-      Fm.ld(o, c3, hxm);
-      Fm.ld(o, c5, hzm);
-      if (IDX && own) Mt.ldm(o1 >> 2, 0, mt);
-      if (slab_lane) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c)
-          if (sl[c] >= 0) {
-            pHy[c] = Pc.ld1(so[c], gHy + pq);
-            pHz[c] = Pc.ld1(so[c], gHz + pq);
-            if (jm_C) pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
-            if (own) { pEy[c] = Pc.ld1(so[c], pq); pEz[c] = Pc.ld1(so[c], gEz + pq); }
-          }
-      }
-    }
-    // x-face incident values of this plane
-    float vhy = 0, vhz = 0, vhzm = 0, vez = 0, vey = 0;
-    const uint32_t xo = (uint32_t)((k - bk0) * xnj + (j - bj0));
-    const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
-    const bool dhy = xh_c >= 0 && (act_h & 2) && xj0 && xk0;
-    const bool dhz = xh_c >= 0 && (act_h & 4) && xj1 && xk1;
-    const bool dhzm = xh_c >= 0 && jm_C && (act_h & 4) && xjm1 && xk1;
-    const bool dez = xe_c >= 0 && own && (act_e & 4) && xj0 && xk0;
-    const bool dey = xe_c >= 0 && own && (act_e & 2) && xj1 && xk1;
-    if (dhy) vhy = ldo(xs, (xh_sel + xo) * 4u);
-    if (dhz) vhz = ldo(xs + xS, (xh_sel + xo) * 4u);
-    if (dhzm) vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
-    if (dez) vez = ldo(xs, (xe_sel + xo) * 4u);
-    if (dey) vey = ldo(xs + xS, (xe_sel + xo) * 4u);
+  auto march = [&](auto edge_tag) {
+    constexpr bool EDGE = decltype(edge_tag)::value;
+    const bool lon = !EDGE || on;
 
-    // ---- E at i+1 of the lane's last column ----
-    float ey_n = __shfl_down(ey[0], 1), ez_n = __shfl_down(ez[0], 1), eym_n = __shfl_down(eym[0], 1);
-    if (S > 1 && lane == 63) { ey_n = bey; ez_n = bez; eym_n = beym; }
-
-    // ---- H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML X (src/gpu.h:604-725) ----
-    // TF/SF y face (rows j0-1, j1) and z face (planes k0-1, k1) of H: the sheet row of this plane
-    // (y) or row (z), offset by the column's i - i0
-    const bool yf_h = act_h && (j == bj0 - 1 || j == bj1) && k >= bk0 && k <= bk1;
-    const bool zf_h = act_h && (k == bk0 - 1 || k == bk1) && j >= bj0 && j <= bj1;
-    const bool yf_lo = j == bj0 - 1, zf_lo = k == bk0 - 1;
-    const float *yrow = ys + (yf_lo ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * xni - bi0);
-    const float *zrow = zs + (zf_lo ? 0 : 2 * zS) + ((ptrdiff_t)(j - bj0) * xni - bi0);
+    // ---- per-lane constants of the march ----
+    // Predicates are bits of one register (FS per column c, then lane bits), re-read through an
+    // opaque copy in every plane: the compiler would otherwise hoist each predicate out of the
+    // march as a 64-bit lane mask in scalar registers and spill them.  A column is an x-slab
+    // column when it updates H psi (s < 2P-1) or E psi (s >= 1).
+    constexpr int FS = 7, F_H = 0, F_E = 1, F_LAST = 2, F_SLH = 3, F_SLE = 4, F_XH = 5, F_XE = 6;
+    constexpr int F_SLAB = 28, F_ON = 29, F_XHL = 30, F_XEL = 31;  // lane: x slab, in grid, x faces
+    static_assert(FS * CPL <= F_SLAB, "lane flags overlap the column flags");
+    int sl[CPL];          // x-slab index of each column (-1 outside the x slabs)
+    float bH[CPL], cH[CPL], bE[CPL], cE[CPL], kH[CPL], kE[CPL];
+    unsigned lf = on ? 1u << F_ON : 0u;
+    int xh_c = -1, xe_c = -1;  // x-face TF/SF columns (the faces are at least CPL columns apart)
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const int i = ib + c;
-      const float ey1 = c + 1 < CPL ? ey[c + 1 < CPL ? c + 1 : 0] : ey_n;
-      const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
-      const int s = sl[c];
-      const float kx = s >= 0 ? sSlab[5][s] : 1.0f;
-      if (on && i < I - 1) {
-        hx[c] = Chh * hx[c] + Che * (-(ezq[c] - ez[c]) + (ny[c] - ey[c]));
-        hy[c] = Chh * hy[c] + Che * (-(nx[c] - ex[c]) + (ez1 - ez[c]) * kx);
-        hz[c] = Chh * hz[c] + Che * (-(ey1 - ey[c]) * kx + (exq[c] - ex[c]));
-      }
-      if (c == xh_c) {
-        if (dhy) { const float v = Che * vhy; hy[c] = xh_lo ? hy[c] - v : hy[c] + v; }
-        if (dhz) { const float v = Che * vhz; hz[c] = xh_lo ? hz[c] + v : hz[c] - v; }
-      }
-      if (yf_h && on && i >= bi0 && i <= bi1) {
-        if ((act_h & (1 << 3)) && k <= bk1 - 1) {
-          const float v = Che * ldo(yrow, (uint32_t)i * 4u);
-          hx[c] = yf_lo ? hx[c] + v : hx[c] - v;
-        }
-        if ((act_h & (1 << 5)) && i <= bi1 - 1) {
-          const float v = Che * ldo(yrow + yS, (uint32_t)i * 4u);
-          hz[c] = yf_lo ? hz[c] - v : hz[c] + v;
-        }
-      }
-      if (zf_h && on && i >= bi0 && i <= bi1) {
-        if ((act_h & (1 << 6)) && j <= bj1 - 1) {
-          const float v = Che * ldo(zrow, (uint32_t)i * 4u);
-          hx[c] = zf_lo ? hx[c] - v : hx[c] + v;
-        }
-        if ((act_h & (1 << 7)) && i <= bi1 - 1) {
-          const float v = Che * ldo(zrow + zS, (uint32_t)i * 4u);
-          hy[c] = zf_lo ? hy[c] + v : hy[c] - v;
-        }
-      }
-      if (s >= 0 && s < P2 - 1) {
-        const float bH = sSlab[2][s], cH = sSlab[3][s];
-        const float pyx = bH * pHy[c] + cH * (ez1 - ez[c]) / dx;
-        hy[c] = hy[c] + Che * pyx * dx;
-        const float pzx = bH * pHz[c] + cH * (ey1 - ey[c]) / dx;
-        hz[c] = hz[c] - Che * pzx * dx;
-        pHy[c] = pyx;
-        pHz[c] = pzx;
+      sl[c] = EDGE && on ? (i < P ? i : (i >= I - P ? i - (I - P2) : -1)) : -1;
+      const int s = sl[c] < 0 ? 0 : sl[c];
+      bE[c] = sSlab[0][s]; cE[c] = sSlab[1][s]; bH[c] = sSlab[2][s]; cH[c] = sSlab[3][s];
+      kE[c] = sl[c] >= 0 ? sSlab[4][s] : 1.0f;
+      kH[c] = sl[c] >= 0 ? sSlab[5][s] : 1.0f;
+      const unsigned b = (unsigned)FS * c;
+      if (sl[c] >= 0) lf |= 1u << F_SLAB;
+      if (sl[c] >= 0 && sl[c] < P2 - 1) lf |= 1u << (b + F_SLH);
+      if (sl[c] >= 1) lf |= 1u << (b + F_SLE);
+      if (on && i < I - 1) lf |= 1u << (b + F_H);
+      if (on && i != 0) lf |= 1u << (b + F_E);
+      if (i == I - 1) lf |= 1u << (b + F_LAST);  // den_ex = 0 on the last x plane
+      if (EDGE && on && act_h && (i == bi0 - 1 || i == bi1)) { lf |= (1u << (b + F_XH)) | (1u << F_XHL); xh_c = c; }
+      if (EDGE && on && act_e && (i == bi0 || i == bi1)) { lf |= (1u << (b + F_XE)) | (1u << F_XEL); xe_c = c; }
+    }
+    const bool xh_lo = xh_c >= 0 && ib + xh_c == bi0 - 1, xe_lo = xe_c >= 0 && ib + xe_c == bi0;
+    // sign of the x-face corrections: a - v == a + (-v) exactly, so the face side is a factor
+    const float sxh = xh_lo ? -1.0f : 1.0f, sxe = xe_lo ? -1.0f : 1.0f;
+    auto bit = [](unsigned f, int b) { return ((f >> b) & 1u) != 0; };
+    auto slab_col = [](unsigned f, int c) { return ((f >> (FS * c + F_SLH)) & 3u) != 0; };
+    const int xnj = pa->xnj;
+    const size_t xS = (size_t)xnj * pa->xnk;
+    const uint32_t xh_sel = xh_lo ? 0u : (uint32_t)(2 * xS), xe_sel = xe_lo ? (uint32_t)(4 * xS) : (uint32_t)(6 * xS);
+    // row predicates of the march (bits): j in [j0, j1], j in [j0, j1-1], j-1 in [j0, j1-1],
+    // and the rows the z faces of the box cover (j, and j-1 for the recomputed Hx)
+    const int jfl = uni((j >= bj0 && j <= bj1 ? 1 : 0) | (j >= bj0 && j <= bj1 - 1 ? 2 : 0) |
+                        (j - 1 >= bj0 && j - 1 <= bj1 - 1 ? 4 : 0) | (zrow ? 8 : 0) | (zrow_m ? 16 : 0));
+    // x-slab psi groups [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k
+    const size_t sx = pa->sx;
+    const Acc<BUF> Pc(pa->Pc, EDGE ? 16 * (sx + pa->sy + pa->sz) : 0);
+    const Acc<BUF> Pn(pa->Pn, EDGE ? 16 * (sx + pa->sy + pa->sz) : 0);
+    const O gEz = (O)(sx * 4), gHy = 2 * gEz, gHz = 3 * gEz;
+    O so[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 4);
+
+    // ---- carried state: E of row j at plane k, Ey of row j-1 at plane k, H(i, j, k-1) ----
+    float ex[CPL], ey[CPL], ez[CPL], eym[CPL], hpx[CPL], hpy[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) ex[c] = ey[c] = ez[c] = eym[c] = hpx[c] = hpy[c] = 0.0f;
+    if (lon) {
+      Fc.ld(o0 + r1, 0, ex); Fc.ld(o0 + r1, c1, ey); Fc.ld(o0 + r1, c2, ez);
+      Fc.ld(o0, c1, eym);
+      if (kb == P) {  // plane P-1 is a z-slab plane: its new H is already written
+        Fn.ld(o0 + r1 - pz, c3, hpx);
+        Fn.ld(o0 + r1 - pz, c4, hpy);
       }
     }
-    // ---- Hx, Hz at (i, j-1, k), recomputed the same way ----
-    if (jm_C) {
-      const int jm = j - 1;
-      const bool yf_hm = act_h && (jm == bj0 - 1 || jm == bj1) && k >= bk0 && k <= bk1;
-      const bool zf_hm = act_h && (act_h & (1 << 6)) && (k == bk0 - 1 || k == bk1) && jm >= bj0 && jm <= bj1 - 1;
-      const bool yf_lom = jm == bj0 - 1;
-      const float *yrowm = ys + (yf_lom ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * xni - bi0);
-      const float *zrowm = zs + (zf_lo ? 0 : 2 * zS) + ((ptrdiff_t)(jm - bj0) * xni - bi0);
+    // E at the first column of the next segment (plane k), for the lane's last column
+    float bey = 0, bez = 0, beym = 0;
+    int par = 0;
+    if (S > 1) {
+      if (lane == 0) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
+      lds_barrier();
+      if (lane == 63 && seg + 1 < S) { bey = sX[0][wv + 1][0]; bez = sX[0][wv + 1][1]; beym = sX[0][wv + 1][2]; }
+      par = 1;
+    }
+
+    // ---- loads of plane index it ----
+    auto load = [&](RowPlane<CPL> &Lp, int it) {
+      const int itc = uni(min(it, n_planes - 1));
+      const int k = k_first + itc;
+      const O o = o0 + (O)itc * pz;
+      const O o1 = o + r1, o2 = o + r2;
+      unsigned f = lf;
+      int ah = act_h, ae = act_e, jf = jfl;
+      if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
+      const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4;
+      if (!EDGE || bit(f, F_ON)) {
+        Fc.ld(o1, pz, Lp.nx); Fc.ld(o1, c1 + pz, Lp.ny); Fc.ld(o1, c2 + pz, Lp.nz);
+        Fc.ld(o, c1 + pz, Lp.nym);
+        Fc.ld(o, 0, Lp.exm); Fc.ld(o, c2, Lp.ezm);
+        Fc.ld(o2, 0, Lp.exq); Fc.ld(o2, c2, Lp.ezq);
+        Fc.ld(o1, c3, Lp.hx); Fc.ld(o1, c4, Lp.hy); Fc.ld(o1, c5, Lp.hz);
+        Fm.ld(o, c3, Lp.hxm);
+        Fm.ld(o, c5, Lp.hzm);
+        if (IDX) Mt.ldm(o1 >> 2, 0, Lp.mt);
+        if (EDGE && bit(f, F_SLAB)) {
+          const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c)
+            if (slab_col(f, c)) {
+              Lp.pHy[c] = Pc.ld1(so[c], gHy + pq);
+              Lp.pHz[c] = Pc.ld1(so[c], gHz + pq);
+              if (jm_C) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
+              Lp.pEy[c] = Pc.ld1(so[c], pq);
+              Lp.pEz[c] = Pc.ld1(so[c], gEz + pq);
+            }
+        }
+      }
+      if (EDGE) {
+        Lp.vhy = Lp.vhz = Lp.vhzm = Lp.vez = Lp.vey = 0.0f;
+        const uint32_t xo = (uint32_t)((k - bk0) * xnj + (j - bj0));
+        const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
+        const float *xs = opaque_c(pa)->xs;
+        if (bit(f, F_XHL)) {
+          if ((ah & 2) && xj0 && xk0) Lp.vhy = ldo(xs, (xh_sel + xo) * 4u);
+          if ((ah & 4) && xj1 && xk1) Lp.vhz = ldo(xs + xS, (xh_sel + xo) * 4u);
+          if (jm_C && (ah & 4) && xjm1 && xk1) Lp.vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
+        }
+        if (bit(f, F_XEL)) {
+          if ((ae & 4) && xj0 && xk0) Lp.vez = ldo(xs, (xe_sel + xo) * 4u);
+          if ((ae & 2) && xj1 && xk1) Lp.vey = ldo(xs + xS, (xe_sel + xo) * 4u);
+        }
+      }
+    };
+
+    // ---- plane index it: H(k) of row j, Hx, Hz(k) of row j-1, then E(k) of row j ----
+    auto compute = [&](RowPlane<CPL> &Lp, int it) {
+      const int k = uni(k_first + it);
+      const bool own = k >= kb;  // planes of the chunk (the plane below only feeds H(k-1))
+      const O o1 = o0 + (O)it * pz + r1;
+      const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
+      unsigned f = lf;
+      int ah = act_h, ae = act_e, jf = jfl;
+      if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
+      const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4, zrow = jf & 8, zrow_m = jf & 16;
+      const bool lon = !EDGE || bit(f, F_ON);
+
+      // E at i+1 of the lane's last column
+      float ey_n = __shfl_down(ey[0], 1), ez_n = __shfl_down(ez[0], 1), eym_n = __shfl_down(eym[0], 1);
+      if (S > 1 && lane == 63) { ey_n = bey; ez_n = bez; eym_n = beym; }
+
+      // H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML X (src/gpu.h:604-725)
+      float hx[CPL], hy[CPL], hz[CPL], hxm[CPL], hzm[CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        const int i = ib + c;
-        const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
-        const int s = sl[c];
-        const float kx = s >= 0 ? sSlab[5][s] : 1.0f;
-        if (on && i < I - 1) {
-          hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - ezm[c]) + (nym[c] - eym[c]));
-          hzm[c] = Chh * hzm[c] + Che * (-(eym1 - eym[c]) * kx + (ex[c] - exm[c]));
+        const float ey1 = c + 1 < CPL ? ey[c + 1 < CPL ? c + 1 : 0] : ey_n;
+        const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
+        hx[c] = Lp.hx[c]; hy[c] = Lp.hy[c]; hz[c] = Lp.hz[c];
+        if (!EDGE || bit(f, FS * c + F_H)) {
+          const float dyz = EDGE ? (ez1 - ez[c]) * kH[c] : ez1 - ez[c];
+          const float dzy = EDGE ? (ey1 - ey[c]) * kH[c] : ey1 - ey[c];
+          hx[c] = Chh * hx[c] + Che * (-(Lp.ezq[c] - ez[c]) + (Lp.ny[c] - ey[c]));
+          hy[c] = Chh * hy[c] + Che * (-(Lp.nx[c] - ex[c]) + dyz);
+          hz[c] = Chh * hz[c] + Che * (-dzy + (Lp.exq[c] - ex[c]));
         }
-        if (c == xh_c && dhzm) { const float v = Che * vhzm; hzm[c] = xh_lo ? hzm[c] + v : hzm[c] - v; }
-        if (yf_hm && on && i >= bi0 && i <= bi1) {
-          if ((act_h & (1 << 3)) && k <= bk1 - 1) {
-            const float v = Che * ldo(yrowm, (uint32_t)i * 4u);
-            hxm[c] = yf_lom ? hxm[c] + v : hxm[c] - v;
-          }
-          if ((act_h & (1 << 5)) && i <= bi1 - 1) {
-            const float v = Che * ldo(yrowm + yS, (uint32_t)i * 4u);
-            hzm[c] = yf_lom ? hzm[c] - v : hzm[c] + v;
-          }
-        }
-        if (zf_hm && on && i >= bi0 && i <= bi1) {
-          const float v = Che * ldo(zrowm, (uint32_t)i * 4u);
-          hxm[c] = zf_lo ? hxm[c] - v : hxm[c] + v;
-        }
-        if (s >= 0 && s < P2 - 1) {
-          const float pzx = sSlab[2][s] * pHzm[c] + sSlab[3][s] * (eym1 - eym[c]) / dx;
-          hzm[c] = hzm[c] - Che * pzx * dx;
+        if (EDGE && bit(f, FS * c + F_XH)) {
+          if ((ah & 2) && xj0 && xk0) hy[c] = hy[c] + (Che * Lp.vhy) * sxh;
+          if ((ah & 4) && xj1 && xk1) hz[c] = hz[c] - (Che * Lp.vhz) * sxh;
         }
       }
-    }
-
-    // ---- H at i-1 of the lane's first column; segment exchange ----
-    float hy_p = __shfl_up(hy[CPL - 1], 1), hz_p = __shfl_up(hz[CPL - 1], 1);
-    if (S > 1) {
-      if (lane == 63) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
-      if (lane == 0) { sX[par][wv][0] = ny[0]; sX[par][wv][1] = nz[0]; sX[par][wv][2] = nym[0]; }
-      lds_barrier();
-      if (lane == 0 && seg > 0) { hy_p = sX[par][wv - 1][3]; hz_p = sX[par][wv - 1][4]; }
-      if (lane == 63 && seg + 1 < S) { bey = sX[par][wv + 1][0]; bez = sX[par][wv + 1][1]; beym = sX[par][wv + 1][2]; }
-      par ^= 1;
-    }
-
-    if (on && own) {
-      Fn.st(o1, c3, hx);
-      Fn.st(o1, c4, hy);
-      Fn.st(o1, c5, hz);
-      if (slab_lane) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c)
-          if (sl[c] >= 0) {
-            Pn.st1(so[c], gHy + pq, pHy[c]);
-            Pn.st1(so[c], gHz + pq, pHz[c]);
-          }
-      }
-      // ---- E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML X (src/gpu.h:765-868) ----
-      float ce[3][CPL], ch[3][CPL];
-      if (IDX) {
+      // TF/SF y faces of H (rows j0-1, j1): edge waves only (plain waves never sit on them)
+      if (EDGE && ah && (j == bj0 - 1 || j == bj1) && k >= bk0 && k <= bk1) {
+        const AS4 StepArgs *q = opaque_c(pa);
+        const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+        const size_t yS = (size_t)ni * nk;
+        const bool lo = j == bj0 - 1;
+        const float *yr = q->xs + 8 * (size_t)nj * nk + (lo ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * ni - bi0);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-          const float *rw = sCoef + 6 * mt[c];
-          const bool last = ib + c == I - 1;  // den_ex = 0 on the last x plane
-          ce[0][c] = rw[0]; ce[1][c] = rw[1]; ce[2][c] = rw[2];
-          ch[0][c] = last ? 0.0f : rw[3];
-          ch[1][c] = last ? 0.0f : rw[4];
-          ch[2][c] = last ? 0.0f : rw[5];
-        }
-      } else {
-        const O cq = (O)(IJK * 4);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          Cf.ld(o1, (O)q * cq, ce[q]);
-          Cf.ld(o1, (O)(3 + q) * cq, ch[q]);
+          const int i = ib + c;
+          if (bit(f, F_ON) && i >= bi0 && i <= bi1) {
+            if ((ah & (1 << 3)) && k <= bk1 - 1) {
+              const float v = Che * ldo(yr, (uint32_t)i * 4u);
+              hx[c] = lo ? hx[c] + v : hx[c] - v;
+            }
+            if ((ah & (1 << 5)) && i <= bi1 - 1) {
+              const float v = Che * ldo(yr + yS, (uint32_t)i * 4u);
+              hz[c] = lo ? hz[c] - v : hz[c] + v;
+            }
+          }
         }
       }
-      // TF/SF y face (rows j0, j1) and z face (planes k0, k1) of E
-      const bool yf_e = act_e && (j == bj0 || j == bj1) && k >= bk0 && k <= bk1;
-      const bool zf_e = act_e && (k == bk0 || k == bk1) && j >= bj0 && j <= bj1;
-      const bool yf_loe = j == bj0, zf_loe = k == bk0;
-      const float *yrowe = ys + (yf_loe ? 4 : 6) * yS + ((ptrdiff_t)(k - bk0) * xni - bi0);
-      const float *zrowe = zs + (zf_loe ? 4 : 6) * zS + ((ptrdiff_t)(j - bj0) * xni - bi0);
-      float ux[CPL], uy[CPL], uz[CPL];
+      // TF/SF z faces of H (planes k0-1, k1)
+      const bool zf_h = ah && (k == bk0 - 1 || k == bk1);
+      if (zf_h && zrow) {
+        const AS4 StepArgs *q = opaque_c(pa);
+        const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+        const size_t zS = (size_t)ni * nj;
+        const bool lo = k == bk0 - 1;
+        const float *zr = q->xs + 8 * ((size_t)nj * nk + (size_t)ni * nk) + (lo ? 0 : 2 * zS) +
+                          ((ptrdiff_t)(j - bj0) * ni - bi0);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const int i = ib + c;
+          if (lon && i >= bi0 && i <= bi1) {
+            if ((ah & (1 << 6)) && j <= bj1 - 1) {
+              const float v = Che * ldo(zr, (uint32_t)i * 4u);
+              hx[c] = lo ? hx[c] - v : hx[c] + v;
+            }
+            if ((ah & (1 << 7)) && i <= bi1 - 1) {
+              const float v = Che * ldo(zr + zS, (uint32_t)i * 4u);
+              hy[c] = lo ? hy[c] + v : hy[c] - v;
+            }
+          }
+        }
+      }
+      if (EDGE && bit(f, F_SLAB)) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          if (bit(f, FS * c + F_SLH)) {
+            const float ey1 = c + 1 < CPL ? ey[c + 1 < CPL ? c + 1 : 0] : ey_n;
+            const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
+            const float pyx = bH[c] * Lp.pHy[c] + cH[c] * (ez1 - ez[c]) / dx;
+            hy[c] = hy[c] + Che * pyx * dx;
+            const float pzx = bH[c] * Lp.pHz[c] + cH[c] * (ey1 - ey[c]) / dx;
+            hz[c] = hz[c] - Che * pzx * dx;
+            Lp.pHy[c] = pyx;
+            Lp.pHz[c] = pzx;
+          }
+        }
+      }
+
+      // Hx, Hz at (i, j-1, k), recomputed the same way
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) { hxm[c] = Lp.hxm[c]; hzm[c] = Lp.hzm[c]; }
+      if (jm_C) {
+        const int jm = j - 1;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
+          if (!EDGE || bit(f, FS * c + F_H)) {
+            const float dzy = EDGE ? (eym1 - eym[c]) * kH[c] : eym1 - eym[c];
+            hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - Lp.ezm[c]) + (Lp.nym[c] - eym[c]));
+            hzm[c] = Chh * hzm[c] + Che * (-dzy + (ex[c] - Lp.exm[c]));
+          }
+          if (EDGE && bit(f, FS * c + F_XH) && (ah & 4) && xjm1 && xk1) hzm[c] = hzm[c] - (Che * Lp.vhzm) * sxh;
+        }
+        if (EDGE && ah && (jm == bj0 - 1 || jm == bj1) && k >= bk0 && k <= bk1) {
+          const AS4 StepArgs *q = opaque_c(pa);
+          const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+          const size_t yS = (size_t)ni * nk;
+          const bool lo = jm == bj0 - 1;
+          const float *yr = q->xs + 8 * (size_t)nj * nk + (lo ? 0 : 2 * yS) + ((ptrdiff_t)(k - bk0) * ni - bi0);
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int i = ib + c;
+            if (bit(f, F_ON) && i >= bi0 && i <= bi1) {
+              if ((ah & (1 << 3)) && k <= bk1 - 1) {
+                const float v = Che * ldo(yr, (uint32_t)i * 4u);
+                hxm[c] = lo ? hxm[c] + v : hxm[c] - v;
+              }
+              if ((ah & (1 << 5)) && i <= bi1 - 1) {
+                const float v = Che * ldo(yr + yS, (uint32_t)i * 4u);
+                hzm[c] = lo ? hzm[c] - v : hzm[c] + v;
+              }
+            }
+          }
+        }
+        if (zf_h && (ah & (1 << 6)) && zrow_m) {
+          const AS4 StepArgs *q = opaque_c(pa);
+          const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+          const size_t zS = (size_t)ni * nj;
+          const bool lo = k == bk0 - 1;
+          const float *zr = q->xs + 8 * ((size_t)nj * nk + (size_t)ni * nk) + (lo ? 0 : 2 * zS) +
+                            ((ptrdiff_t)(jm - bj0) * ni - bi0);
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int i = ib + c;
+            if (lon && i >= bi0 && i <= bi1) {
+              const float v = Che * ldo(zr, (uint32_t)i * 4u);
+              hxm[c] = lo ? hxm[c] - v : hxm[c] + v;
+            }
+          }
+        }
+        if (EDGE && bit(f, F_SLAB)) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_SLH)) {
+              const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
+              const float pzx = bH[c] * Lp.pHzm[c] + cH[c] * (eym1 - eym[c]) / dx;
+              hzm[c] = hzm[c] - Che * pzx * dx;
+            }
+          }
+        }
+      }
+
+      // H at i-1 of the lane's first column; segment exchange
+      float hy_p = __shfl_up(hy[CPL - 1], 1), hz_p = __shfl_up(hz[CPL - 1], 1);
+      if (S > 1) {
+        if (lane == 63) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
+        if (lane == 0) { sX[par][wv][0] = Lp.ny[0]; sX[par][wv][1] = Lp.nz[0]; sX[par][wv][2] = Lp.nym[0]; }
+        lds_barrier();
+        if (lane == 0 && seg > 0) { hy_p = sX[par][wv - 1][3]; hz_p = sX[par][wv - 1][4]; }
+        if (lane == 63 && seg + 1 < S) { bey = sX[par][wv + 1][0]; bez = sX[par][wv + 1][1]; beym = sX[par][wv + 1][2]; }
+        par ^= 1;
+      }
+
+      if (own) {
+        if (lon) {
+          Fn.st(o1, c3, hx);
+          Fn.st(o1, c4, hy);
+          Fn.st(o1, c5, hz);
+          if (EDGE && bit(f, F_SLAB)) {
+            const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              if (slab_col(f, c)) {
+                Pn.st1(so[c], gHy + pq, Lp.pHy[c]);
+                Pn.st1(so[c], gHz + pq, Lp.pHz[c]);
+              }
+          }
+        }
+        // E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML X (src/gpu.h:765-868)
+        float ce[3][CPL], ch[3][CPL];
+        if (IDX) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const float *rw = sCoef + 6 * Lp.mt[c];
+            ce[0][c] = rw[0]; ce[1][c] = rw[1]; ce[2][c] = rw[2];
+            ch[0][c] = rw[3]; ch[1][c] = rw[4]; ch[2][c] = rw[5];
+            if (EDGE && bit(f, FS * c + F_LAST)) ch[0][c] = ch[1][c] = ch[2][c] = 0.0f;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) ce[0][c] = ce[1][c] = ce[2][c] = ch[0][c] = ch[1][c] = ch[2][c] = 0.0f;
+          if (lon) {
+            const O cq = (O)(IJK * 4);
+#pragma unroll
+            for (int qq = 0; qq < 3; ++qq) {
+              Cf.ld(o1, (O)qq * cq, ce[qq]);
+              Cf.ld(o1, (O)(3 + qq) * cq, ch[qq]);
+            }
+          }
+        }
+        float ux[CPL], uy[CPL], uz[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          const float hym1 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_p;
+          const float hzm1 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_p;
+          ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) - (hy[c] - hpy[c]));
+          uy[c] = ey[c];
+          uz[c] = ez[c];
+          if (!EDGE || bit(f, FS * c + F_E)) {
+            const float dzx = EDGE ? (hz[c] - hzm1) * kE[c] : hz[c] - hzm1;
+            const float dyx = EDGE ? (hy[c] - hym1) * kE[c] : hy[c] - hym1;
+            uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) - dzx);
+            uz[c] = ce[2][c] * ez[c] + ch[2][c] * (dyx - (hx[c] - hxm[c]));
+          }
+          if (EDGE && bit(f, FS * c + F_XE)) {
+            if ((ae & 4) && xj0 && xk0) uz[c] = uz[c] + (ch[2][c] * Lp.vez) * sxe;
+            if ((ae & 2) && xj1 && xk1) uy[c] = uy[c] - (ch[1][c] * Lp.vey) * sxe;
+          }
+        }
+        // TF/SF y faces of E (rows j0, j1): edge waves only
+        if (EDGE && ae && (j == bj0 || j == bj1) && k >= bk0 && k <= bk1) {
+          const AS4 StepArgs *q = opaque_c(pa);
+          const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+          const size_t yS = (size_t)ni * nk;
+          const bool lo = j == bj0;
+          const float *yr = q->xs + 8 * (size_t)nj * nk + (lo ? 4 : 6) * yS + ((ptrdiff_t)(k - bk0) * ni - bi0);
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int i = ib + c;
+            if (i >= bi0 && i <= bi1) {
+              if ((ae & (1 << 3)) && i <= bi1 - 1) {
+                const float v = ch[0][c] * ldo(yr, (uint32_t)i * 4u);
+                ux[c] = lo ? ux[c] - v : ux[c] + v;
+              }
+              if ((ae & (1 << 5)) && k <= bk1 - 1) {
+                const float v = ch[2][c] * ldo(yr + yS, (uint32_t)i * 4u);
+                uz[c] = lo ? uz[c] + v : uz[c] - v;
+              }
+            }
+          }
+        }
+        // TF/SF z faces of E (planes k0, k1)
+        if (ae && (k == bk0 || k == bk1) && zrow) {
+          const AS4 StepArgs *q = opaque_c(pa);
+          const int ni = q->xni, nj = q->xnj, nk = q->xnk;
+          const size_t zS = (size_t)ni * nj;
+          const bool lo = k == bk0;
+          const float *zr = q->xs + 8 * ((size_t)nj * nk + (size_t)ni * nk) + (lo ? 4 : 6) * zS +
+                            ((ptrdiff_t)(j - bj0) * ni - bi0);
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            const int i = ib + c;
+            if (i >= bi0 && i <= bi1) {
+              if ((ae & (1 << 6)) && i <= bi1 - 1) {
+                const float v = ch[0][c] * ldo(zr, (uint32_t)i * 4u);
+                ux[c] = lo ? ux[c] + v : ux[c] - v;
+              }
+              if ((ae & (1 << 7)) && j <= bj1 - 1) {
+                const float v = ch[1][c] * ldo(zr + zS, (uint32_t)i * 4u);
+                uy[c] = lo ? uy[c] - v : uy[c] + v;
+              }
+            }
+          }
+        }
+        if (EDGE && bit(f, F_SLAB)) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_SLE)) {
+              const float hym1 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_p;
+              const float hzm1 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_p;
+              const float pyx = bE[c] * Lp.pEy[c] + cE[c] * (hz[c] - hzm1) / dx;
+              uy[c] = uy[c] - ch[1][c] * pyx * dx;
+              const float pzx = bE[c] * Lp.pEz[c] + cE[c] * (hy[c] - hym1) / dx;
+              uz[c] = uz[c] + ch[2][c] * pzx * dx;
+              Lp.pEy[c] = pyx;
+              Lp.pEz[c] = pzx;
+            }
+          }
+        }
+        if (lon) {
+          Fn.st(o1, 0, ux);
+          Fn.st(o1, c1, uy);
+          Fn.st(o1, c2, uz);
+          if (EDGE && bit(f, F_SLAB)) {
+            const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              if (slab_col(f, c)) {
+                Pn.st1(so[c], pq, Lp.pEy[c]);
+                Pn.st1(so[c], gEz + pq, Lp.pEz[c]);
+              }
+          }
+        }
+      }
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
-        const int i = ib + c;
-        const float hym1 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_p;
-        const float hzm1 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_p;
-        const int s = sl[c];
-        const float kx = s >= 0 ? sSlab[4][s] : 1.0f;
-        ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) - (hy[c] - hpy[c]));
-        uy[c] = ey[c];
-        uz[c] = ez[c];
-        if (i != 0) {
-          uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) - (hz[c] - hzm1) * kx);
-          uz[c] = ce[2][c] * ez[c] + ch[2][c] * ((hy[c] - hym1) * kx - (hx[c] - hxm[c]));
-        }
-        if (c == xe_c) {
-          if (dez) { const float v = ch[2][c] * vez; uz[c] = xe_lo ? uz[c] - v : uz[c] + v; }
-          if (dey) { const float v = ch[1][c] * vey; uy[c] = xe_lo ? uy[c] + v : uy[c] - v; }
-        }
-        if (yf_e && i >= bi0 && i <= bi1) {
-          if ((act_e & (1 << 3)) && i <= bi1 - 1) {
-            const float v = ch[0][c] * ldo(yrowe, (uint32_t)i * 4u);
-            ux[c] = yf_loe ? ux[c] - v : ux[c] + v;
-          }
-          if ((act_e & (1 << 5)) && k <= bk1 - 1) {
-            const float v = ch[2][c] * ldo(yrowe + yS, (uint32_t)i * 4u);
-            uz[c] = yf_loe ? uz[c] + v : uz[c] - v;
-          }
-        }
-        if (zf_e && i >= bi0 && i <= bi1) {
-          if ((act_e & (1 << 6)) && i <= bi1 - 1) {
-            const float v = ch[0][c] * ldo(zrowe, (uint32_t)i * 4u);
-            ux[c] = zf_loe ? ux[c] + v : ux[c] - v;
-          }
-          if ((act_e & (1 << 7)) && j <= bj1 - 1) {
-            const float v = ch[1][c] * ldo(zrowe + zS, (uint32_t)i * 4u);
-            uy[c] = zf_loe ? uy[c] - v : uy[c] + v;
-          }
-        }
-        if (s >= 1) {
-          const float bE = sSlab[0][s], cE = sSlab[1][s];
-          const float pyx = bE * pEy[c] + cE * (hz[c] - hzm1) / dx;
-          uy[c] = uy[c] - ch[1][c] * pyx * dx;
-          const float pzx = bE * pEz[c] + cE * (hy[c] - hym1) / dx;
-          uz[c] = uz[c] + ch[2][c] * pzx * dx;
-          pEy[c] = pyx;
-          pEz[c] = pzx;
-        }
+        hpx[c] = hx[c];
+        hpy[c] = hy[c];
+        ex[c] = Lp.nx[c]; ey[c] = Lp.ny[c]; ez[c] = Lp.nz[c]; eym[c] = Lp.nym[c];
       }
-      Fn.st(o1, 0, ux);
-      Fn.st(o1, c1, uy);
-      Fn.st(o1, c2, uz);
-      if (slab_lane) {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c)
-          if (sl[c] >= 0) {
-            Pn.st1(so[c], pq, pEy[c]);
-            Pn.st1(so[c], gEz + pq, pEz[c]);
-          }
-      }
-    }
+    };
+
+    RowPlane<CPL> A;
+    // lanes outside the grid (edge waves) never load: their plane values stay zero
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-      hpx[c] = hx[c];
-      hpy[c] = hy[c];
-      ex[c] = nx[c]; ey[c] = ny[c]; ez[c] = nz[c]; eym[c] = nym[c];
+      A.nx[c] = A.ny[c] = A.nz[c] = A.nym[c] = A.exm[c] = A.ezm[c] = A.exq[c] = A.ezq[c] = 0.0f;
+      A.hx[c] = A.hy[c] = A.hz[c] = A.hxm[c] = A.hzm[c] = 0.0f;
+      A.pHy[c] = A.pHz[c] = A.pHzm[c] = A.pEy[c] = A.pEz[c] = 0.0f;
+      A.mt[c] = 0;
     }
-  }
+    for (int it = 0; it < n_planes; ++it) {
+      load(A, it);
+      compute(A, it);
+    }
+  };
+
+  if (uni((int)plain)) march(std::false_type{});
+  else march(std::true_type{});
 }
 
 template <bool IDX, bool BUF, int CPL>
