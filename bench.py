@@ -13,7 +13,9 @@ oracle timed on a bounded sample of the same workload on this host's cores.
 
 --copy-calibration runs a plain device copy of known size (the scale of the HBM byte counters).
 --workload batch runs the 96-grid workload through the batched engine and, for comparison, each
-grid alone (a secondary measurement; the default line is the 256^3 workload).
+grid alone; --workload serve runs a Poisson stream of solve requests through the request server
+and, for comparison, one after another through the plain path (secondary measurements; the
+default line is the 256^3 workload).
 """
 import argparse
 import json
@@ -154,6 +156,119 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_serve(args):
+    """Served workload: a Poisson stream of solve requests (edges 48..320, 500..3000 steps, spheres
+    or layered ground, own CPML and plane wave each) through the request server, against the same
+    requests run one after another through the plain path.  The arrival rate is sized from the
+    plain path's rate to keep the GPU near saturation (0.9 of it).  Reports aggregate
+    Gcell-updates/s, p50 / p99 time to completion, and whether every served probe record equals the
+    plain run's."""
+    import numpy as np
+
+    from maxwell_solver_amd import Server, Simulation
+    from maxwell_solver_amd import scenarios as S
+    reqs = S.config5(args.requests)
+    total = sum(sc.cells * steps for sc, steps, _ in reqs)
+    # warm the kernels
+    sc0, _, _ = reqs[0]
+    g = Simulation()
+    S.setup(g, sc0)
+    g.run(0, 2, sc0.mode)
+    g.synchronize()
+    g.close()
+    # plain path, one request after another (set-up on the GPU, steps, completion)
+    rec = []
+    t_seq = 0.0
+    lat_seq = []
+    for sc, steps, _ in reqs:
+        t0 = time.perf_counter()
+        g = Simulation()
+        S.setup(g, sc)
+        g.run(0, steps, sc.mode)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        t_seq += dt
+        lat_seq.append(t_seq)
+        rec.append(g.probes().copy())
+        g.close()
+    rate_plain = total / t_seq
+    mean_work = total / len(reqs)
+    scale = mean_work / (0.9 * rate_plain)
+    arrivals = np.cumsum([gap for _, _, gap in reqs]) * scale
+    arrivals -= arrivals[0]
+
+    def serve(arr):
+        srv = Server(hbm_budget=int(args.hbm_budget_gb * (1 << 30)), max_active=args.max_active,
+                     steps_per_round=args.steps_per_round)
+        sims = []
+        for sc, steps, _ in reqs:  # host-side set-up, as the clients that submit them
+            sim = Simulation()
+            S.setup_host(sim, sc)
+            sims.append(sim)
+        ids = {}
+        lat, same = [], True
+        nxt = 0
+        t_start = srv.now()
+        t_end = t_start
+        while len(lat) < len(reqs):
+            now = srv.now() - t_start
+            while nxt < len(reqs) and arr[nxt] <= now:
+                sc, steps, _ = reqs[nxt]
+                ids[srv.submit(sims[nxt], steps, sc.mode, source=sc.source, t_arrive=t_start + arr[nxt])] = nxt
+                nxt += 1
+            active, queued = srv.round()
+            for rid, ta, tadm, tdone in srv.collect():
+                q = ids[rid]
+                same = same and np.array_equal(srv.simulation(rid).probes(), rec[q])
+                lat.append(tdone - ta)
+                t_end = max(t_end, tdone)
+                srv.release(rid)
+            if not active and not queued and nxt < len(reqs):
+                time.sleep(max(0.0, min(0.05, arr[nxt] - (srv.now() - t_start))))
+        peak = srv.stats()[1]
+        srv.close()
+        return t_end - t_start, np.asarray(lat), same, peak
+
+    span_open, lat_open, same_open, peak_open = serve(arrivals)
+    span_closed, lat_closed, same_closed, _ = serve(np.zeros(len(reqs)))
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(total / span_open / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": int(sum(steps for _, steps, _ in reqs)),
+        "warmup": 2,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded request stream: sizes, materials, plane waves, Poisson arrivals)",
+        "config": {
+            "workload": f"{len(reqs)} solve requests (edges 48..320, 500..3000 steps, spheres or layered ground), "
+                        "Poisson arrivals at 0.9 of the plain path's rate, request server with continuous batching",
+            "requests": len(reqs),
+            "cell_updates": total,
+            "p50_s": round(float(np.percentile(lat_open, 50)), 3),
+            "p99_s": round(float(np.percentile(lat_open, 99)), 3),
+            "span_s": round(span_open, 3),
+            "peak_hbm_gb": round(peak_open / (1 << 30), 2),
+            "closed_value": round(total / span_closed / 1e9, 3),
+            "closed_p50_s": round(float(np.percentile(lat_closed, 50)), 3),
+            "closed_p99_s": round(float(np.percentile(lat_closed, 99)), 3),
+            "sequential_value": round(rate_plain / 1e9, 3),
+            "sequential_s": round(t_seq, 3),
+            "sequential_p50_s": round(float(np.percentile(lat_seq, 50)), 3),
+            "sequential_p99_s": round(float(np.percentile(lat_seq, 99)), 3),
+            "closed_speedup_vs_sequential": round(t_seq / span_closed, 3),
+            "served_equals_sequential": bool(same_open and same_closed),
+            "max_active": args.max_active,
+            "steps_per_round": args.steps_per_round,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(sc, budget_s=15.0, max_steps=1000):
     """The CPU oracle (float32, OpenMP over k planes) on the same workload, bounded to ~budget_s."""
     import numpy as np
@@ -193,8 +308,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", choices=["grid", "batch"], default="grid",
-                    help="grid: the 256^3 workload (default); batch: 96 independent grids, batched engine")
+    ap.add_argument("--workload", choices=["grid", "batch", "serve"], default="grid",
+                    help="grid: the 256^3 workload (default); batch: 96 independent grids, batched engine; "
+                         "serve: a stream of solve requests through the request server")
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--max-active", type=int, default=8)
+    ap.add_argument("--steps-per-round", type=int, default=64)
+    ap.add_argument("--hbm-budget-gb", type=float, default=288.0)
     ap.add_argument("--grids", type=int, default=96)
     ap.add_argument("--min-edge", type=float, default=32)
     ap.add_argument("--max-edge", type=float, default=192)
@@ -208,6 +328,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
 
     from maxwell_solver_amd import Simulation

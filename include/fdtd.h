@@ -151,6 +151,39 @@ int fdtd_batch_run(fdtd_batch *b);
  * steps queued on its own stream); 0 (default) = merged launches for the grids under 2^18 cells
  * and one stream each for the larger ones, both at once.  Results are the same either way. */
 int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy);
+/* Request server (no counterpart in the reference; the served form of the solver).  Solves arrive
+ * one at a time and are advanced together with continuous batching on one GPU.
+ *   fdtd_server_create: hbm_budget = device bytes the admitted requests may hold together,
+ *     max_active = requests advanced at once, steps_per_round = steps per round.
+ *   fdtd_server_submit: queue a solve.  s has its host set-up done (SetConst, source, materials or
+ *     material index, probe locations and components, CPML / TF/SF / plane-wave options) and is not
+ *     ready; the server owns it from here on (do not destroy it).  At admission the server runs
+ *     ReadySimulation and, for mode != 0, Init1D(rate, pw_source) (pw_source: T samples, copied
+ *     here), then steps n = 0 .. count-1 in plane-wave mode `mode`.  t_arrive: arrival time on the
+ *     server clock (fdtd_server_now), or < 0 for now.  A request is charged an upper bound of its
+ *     device bytes from admission to release; one above the budget alone is refused
+ *     (FDTD_ERR_NOMEM).
+ *   fdtd_server_round: one round: admit queued requests in arrival order while slots and budget
+ *     allow, advance every admitted request by up to steps_per_round steps (batched engine), retire
+ *     the ones that reached their last step.  Returns after the round's GPU work is done.
+ *   fdtd_server_collect: up to max completed requests: ids[k] and times[3k..3k+2] = arrival,
+ *     admission and completion times (server clock, seconds).  Their results stay readable through
+ *     the simulation handle (OutputAnt, OutputVis, fdtd_read_array) until fdtd_server_release,
+ *     which destroys the simulation and returns its bytes to the budget.
+ * Each request's fields, psi and probe record equal those of fdtd_run of the same simulation alone. */
+typedef struct fdtd_server fdtd_server;
+int fdtd_server_create(fdtd_server **out, uint64_t hbm_budget, int32_t max_active, int32_t steps_per_round);
+int fdtd_server_destroy(fdtd_server *srv);
+int fdtd_server_now(fdtd_server *srv, double *t);
+int fdtd_server_submit(fdtd_server *srv, fdtd_sim *s, float rate, const float *pw_source, int32_t count, int32_t mode,
+                       double t_arrive, int64_t *id);
+int fdtd_server_round(fdtd_server *srv, int32_t *active, int32_t *queued);
+int fdtd_server_collect(fdtd_server *srv, int64_t *ids, double *times, int32_t max, int32_t *n);
+int fdtd_server_release(fdtd_server *srv, int64_t id);
+/* Device bytes a request is charged while admitted (its upper bound, fixed at submission). */
+int fdtd_server_request_bytes(fdtd_server *srv, int64_t id, uint64_t *bytes);
+/* Bytes charged to admitted requests now and at most so far; cell updates done. */
+int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_peak, int64_t *cell_updates);
 /* Time the fused step at time index n, plane-wave mode `mode`, for each available columns-per-lane
  * and z-chunk choice (reps steps each) and keep the fastest (*best_ms: its average step time).
  * The steps are run dry: the fields, psi, probe record and time step are unchanged. */

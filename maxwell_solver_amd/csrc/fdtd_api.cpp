@@ -7,6 +7,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -130,6 +132,7 @@ struct fdtd_sim {
   double kernel_ms = 0, rows_ms = 0;  // summed step / row-kernel event times of the last timed run
   int64_t kernel_launches = 0;
   std::vector<void *> allocs;
+  size_t dev_bytes = 0;  // device bytes allocated so far (snapshots excluded)
 
   ~fdtd_sim() {
     if (stream) hipStreamSynchronize(stream);
@@ -146,8 +149,29 @@ struct fdtd_sim {
     hipError_t e = hipMalloc(&q, n * sizeof(Tp));
     if (e != hipSuccess) return fail(FDTD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     allocs.push_back(q);
+    dev_bytes += n * sizeof(Tp);
     *p = (Tp *)q;
     return 0;
+  }
+
+  // Upper bound of the device bytes the simulation holds once ready, with its 1D line and after
+  // fused steps (two field and psi sets, materials, tables, line histories, face sheets, probe
+  // record), from the set-up alone (no device call): the request server admits against it.
+  size_t device_bytes_bound(int mode) const {
+    const size_t P = (size_t)cp.P;
+    const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
+    size_t b = 2 * ((6 * (IJK + kCompSkew) + kSetSkew) + 4 * (sx + sy + sz)) * sizeof(float);
+    b += have_index ? IJK + 256 * 6 * sizeof(float) : 6 * IJK * sizeof(float);
+    b += (8 * P + 2 * ((size_t)I + J + K)) * sizeof(float);
+    if (mode) {
+      const size_t N = (size_t)I + J + K + 64;                // line nodes, oblique included
+      b += 2 * (2 * (size_t)T * N + 4 * N + T) * sizeof(float);  // two lines: histories + uploads
+      b += 6 * 4 * ((size_t)I + J + K) * sizeof(float);       // plane-wave tables
+      const size_t ni = I, nj = J, nk = K;
+      b += 8 * (nj * nk + ni * nk + ni * nj) * sizeof(float);  // face sheets
+    }
+    if (init_antenna) b += (probe_comps.size() * (size_t)T * nAnt + nAnt + probe_comps.size()) * sizeof(float);
+    return b + ((size_t)1 << 20);  // argument blocks and small tables
   }
   template <typename Tp>
   int upload(Tp **p, const std::vector<Tp> &v) {
@@ -911,6 +935,91 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
 }
 
 // ------------------------------------------------------------------------------------------
+// Request server: solves arrive one at a time and are advanced together (continuous batching).
+// A round admits queued requests in arrival order while the active slots and the HBM budget allow
+// (ReadySimulation and Init1D run at admission, so a queued request holds no device memory),
+// advances every active request by up to steps_per_round steps through the batched engine, and
+// retires the requests that reached their last step.  Requests admitted in one round join the
+// running ones at the next round boundary; each request's result is that of fdtd_run alone.
+// ------------------------------------------------------------------------------------------
+struct fdtd_server {
+  struct Req {
+    int64_t id = 0;
+    fdtd_sim *s = nullptr;
+    float rate = 0;
+    std::vector<float> pw;
+    int count = 0, mode = 0, done = 0;
+    size_t bytes = 0;
+    double t_arrive = 0, t_admit = -1, t_done = -1;
+  };
+  size_t budget = 0, in_use = 0, peak = 0;
+  int max_active = 0, steps_per_round = 0;
+  int64_t next_id = 1;
+  std::deque<Req *> pending;        // arrival order
+  std::vector<Req *> active;
+  std::deque<Req *> finished;       // completed, not yet collected
+  std::map<int64_t, Req *> live;    // every request not yet released
+  fdtd_batch batch;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  int64_t cell_updates = 0;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+  ~fdtd_server() {
+    for (auto &kv : live) {
+      delete kv.second->s;
+      delete kv.second;
+    }
+  }
+
+  int admit(Req *r) {
+    fdtd_sim *s = r->s;
+    if (int rc = s->ready()) return rc;
+    if (r->mode)
+      if (int rc = s->init_1d(r->rate, r->pw.data())) return rc;
+    r->pw.clear();
+    r->pw.shrink_to_fit();
+    in_use += r->bytes;
+    peak = std::max(peak, in_use);
+    r->t_admit = now();
+    active.push_back(r);
+    return 0;
+  }
+
+  int round() {
+    while (!pending.empty() && (int)active.size() < max_active && in_use + pending.front()->bytes <= budget) {
+      Req *r = pending.front();
+      pending.pop_front();
+      if (int rc = admit(r)) return rc;
+    }
+    if (active.empty()) return 0;
+    batch.items.clear();
+    for (Req *r : active) {
+      const int k = std::min(steps_per_round, r->count - r->done);
+      batch.items.push_back({r->s, r->done, k, r->mode});
+    }
+    if (int rc = batch_run(&batch)) return rc;
+    const double t = now();
+    std::vector<Req *> still;
+    for (size_t q = 0; q < active.size(); ++q) {
+      Req *r = active[q];
+      const int k = batch.items[q].count;
+      r->done += k;
+      cell_updates += (int64_t)r->s->IJK * k;
+      if (r->done >= r->count) {
+        r->t_done = t;
+        finished.push_back(r);
+      } else {
+        still.push_back(r);
+      }
+    }
+    active.swap(still);
+    batch.items.clear();
+    return 0;
+  }
+};
+
+// ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
 #define NEED(s)                                                     \
@@ -1184,6 +1293,116 @@ int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy) {
 int fdtd_batch_run(fdtd_batch *b) {
   if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
   return batch_run(b);
+}
+
+int fdtd_server_create(fdtd_server **out, uint64_t hbm_budget, int32_t max_active, int32_t steps_per_round) {
+  if (!out) return fail(FDTD_ERR_INVALID, "null output pointer");
+  if (max_active < 1 || steps_per_round < 1 || hbm_budget == 0)
+    return fail(FDTD_ERR_INVALID, "server: budget, slots and steps per round must be positive");
+  fdtd_server *srv = new fdtd_server();
+  srv->budget = (size_t)hbm_budget;
+  srv->max_active = max_active;
+  srv->steps_per_round = steps_per_round;
+  *out = srv;
+  return 0;
+}
+
+int fdtd_server_destroy(fdtd_server *srv) {
+  delete srv;
+  return 0;
+}
+
+int fdtd_server_now(fdtd_server *srv, double *t) {
+  if (!srv || !t) return fail(FDTD_ERR_INVALID, "null server handle or output");
+  *t = srv->now();
+  return 0;
+}
+
+int fdtd_server_submit(fdtd_server *srv, fdtd_sim *s, float rate, const float *pw_source, int32_t count, int32_t mode,
+                       double t_arrive, int64_t *id) {
+  if (!srv || !id) return fail(FDTD_ERR_INVALID, "null server handle or output");
+  NEED(s);
+  if (s->init_ready) return fail(FDTD_ERR_STATE, "server: the simulation must not be ready yet (admission readies it)");
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "server: SetConst has not been called");
+  if (count < 1) return fail(FDTD_ERR_INVALID, "server: step count must be positive");
+  if (mode < 0 || mode > 5) return fail(FDTD_ERR_INVALID, "server: plane-wave mode must be 0..5");
+  if (mode && !pw_source) return fail(FDTD_ERR_INVALID, "server: a plane-wave mode needs its source samples");
+  if (mode && count > s->T) return fail(FDTD_ERR_INVALID, "server: steps beyond the source length T");
+  if (mode == 5 && s->pw_mode != 5) return fail(FDTD_ERR_STATE, "server: mode 5 needs the oblique plane-wave set-up");
+  for (const auto &kv : srv->live)
+    if (kv.second->s == s) return fail(FDTD_ERR_INVALID, "server: simulation already submitted");
+  auto *r = new fdtd_server::Req();
+  r->id = srv->next_id++;
+  r->s = s;
+  r->rate = rate;
+  if (mode) r->pw.assign(pw_source, pw_source + s->T);
+  r->count = count;
+  r->mode = mode;
+  r->bytes = s->device_bytes_bound(mode);
+  r->t_arrive = t_arrive >= 0 ? t_arrive : srv->now();
+  if (r->bytes > srv->budget) {
+    delete r;
+    return fail(FDTD_ERR_NOMEM, "server: the request alone exceeds the HBM budget");
+  }
+  srv->pending.push_back(r);
+  srv->live[r->id] = r;
+  *id = r->id;
+  return 0;
+}
+
+int fdtd_server_round(fdtd_server *srv, int32_t *active, int32_t *queued) {
+  if (!srv) return fail(FDTD_ERR_INVALID, "null server handle");
+  const int rc = srv->round();
+  if (active) *active = (int32_t)srv->active.size();
+  if (queued) *queued = (int32_t)srv->pending.size();
+  return rc;
+}
+
+int fdtd_server_collect(fdtd_server *srv, int64_t *ids, double *times, int32_t max, int32_t *n) {
+  if (!srv || !n || (max > 0 && (!ids || !times))) return fail(FDTD_ERR_INVALID, "null server handle or output");
+  int k = 0;
+  while (k < max && !srv->finished.empty()) {
+    fdtd_server::Req *r = srv->finished.front();
+    srv->finished.pop_front();
+    ids[k] = r->id;
+    times[3 * k] = r->t_arrive;
+    times[3 * k + 1] = r->t_admit;
+    times[3 * k + 2] = r->t_done;
+    ++k;
+  }
+  *n = k;
+  return 0;
+}
+
+int fdtd_server_release(fdtd_server *srv, int64_t id) {
+  if (!srv) return fail(FDTD_ERR_INVALID, "null server handle");
+  auto it = srv->live.find(id);
+  if (it == srv->live.end()) return fail(FDTD_ERR_INVALID, "server: unknown request id");
+  fdtd_server::Req *r = it->second;
+  if (r->t_done < 0) return fail(FDTD_ERR_STATE, "server: request not completed");
+  for (const auto &f : srv->finished)
+    if (f == r) return fail(FDTD_ERR_STATE, "server: request not collected");
+  delete r->s;
+  srv->in_use -= r->bytes;
+  srv->live.erase(it);
+  delete r;
+  return 0;
+}
+
+int fdtd_server_request_bytes(fdtd_server *srv, int64_t id, uint64_t *bytes) {
+  if (!srv || !bytes) return fail(FDTD_ERR_INVALID, "null server handle or output");
+  auto it = srv->live.find(id);
+  if (it == srv->live.end()) return fail(FDTD_ERR_INVALID, "server: unknown request id");
+  *bytes = it->second->bytes;
+  return 0;
+}
+
+int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_peak, int64_t *cell_updates) {
+  if (!srv) return fail(FDTD_ERR_INVALID, "null server handle");
+  if (bytes_in_use) *bytes_in_use = srv->in_use;
+  if (bytes_peak) *bytes_peak = srv->peak;
+  if (cell_updates) *cell_updates = srv->cell_updates;
+  return 0;
 }
 
 int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms) {

@@ -274,6 +274,89 @@ def config4(n_grids: int = 96, edge_lo: float = 32, edge_hi: float = 192) -> lis
     return out
 
 
+def _layered(rng, I, J, K, P, T, n_layers=5):
+    """Horizontal lossy layers under air as in workload 3, thicknesses scaled to the grid height
+    (U[20, 80] cells at K = 256), dx for >= 10 cells per wavelength in the most conductive layer at
+    twice the pulse centre frequency.  Returns (dx, dt, fc, index, table, ground surface plane)."""
+    fc = rng.uniform(1e6, 100e6)
+    layers = []
+    for _ in range(n_layers):
+        th = max(2, int(round(rng.randint(20, 80) * K / 256.0)))
+        layers.append((th, rng.log_uniform(1e-4, 1.0), rng.uniform(4.0, 30.0)))
+    w = 2 * math.pi * 2 * fc
+    lam = []
+    for _, sg, er in layers:
+        kk = w * (MU0 * complex(er * EPS0, -sg / w)) ** 0.5
+        lam.append(2 * math.pi / abs(kk.real))
+    dx = min(lam) / 10.0
+    idx = np.zeros((K, J, I), dtype=np.uint8)
+    table = [[0, 0, 0, 1, 1, 1]]
+    ztop = K // 2
+    z = ztop
+    for m, (th, sg, er) in enumerate(layers):
+        zl = max(0, z - th)
+        idx[zl:z] = m + 1
+        table.append([sg, sg, sg, er, er, er])
+        z = zl
+        if z == 0:
+            break
+    idx[:z] = idx[z] if z < K else 0
+    return dx, _dt(dx), fc, idx, np.asarray(table, dtype=np.float32), ztop
+
+
+def config5(n_requests: int = 2000, edge_lo: float = 48, edge_hi: float = 320, steps_lo: int = 500,
+            steps_hi: int = 3000) -> list:
+    """Served workload: a stream of solve requests.  Per request: base edge log-uniform in
+    [edge_lo, edge_hi] with per-axis aspect factors in U[1, 1.25] normalised to a unit product
+    (edges multiples of 4); steps uniform in [steps_lo, steps_hi]; materials with probability 0.5
+    from the sphere generator of workload 2 (0..40 spheres, 10-cell CPML, oblique plane wave at
+    seeded angles, Gaussian-modulated sine f0 in [1, 3] GHz, 8 probes Ez / Hy) and otherwise from
+    the layered generator of workload 3 (12-cell CPML, vertical x-polarised Gaussian pulse through
+    a layered 1D line, 8 probes Ex / Hy on the ground surface).  Returns (Scenario, steps, gap)
+    triples: gap is an exponential inter-arrival time of unit mean (a Poisson stream once scaled by
+    the mean request time the arrival rate is sized for)."""
+    seed = 0x5EED0000 + 5
+    rng = SplitMix64(seed)
+    out = []
+    for q in range(n_requests):
+        base = rng.log_uniform(edge_lo, edge_hi)
+        a = [rng.uniform(1.0, 1.25) for _ in range(3)]
+        gm = (a[0] * a[1] * a[2]) ** (1.0 / 3.0)
+        I, J, K = (max(48, 4 * int(round(base * x / gm / 4))) for x in a)
+        steps = rng.randint(steps_lo, steps_hi)
+        gap = -math.log(max(1e-12, 1.0 - rng.uniform()))
+        if rng.uniform() < 0.5:
+            P = 10
+            cpml = (P, 3.0, 5.0, 0.05, 1.0, 0.8)
+            dx = 0.01
+            dt = _dt(dx)
+            f0 = rng.uniform(1e9, 3e9)
+            inset = P + 5
+            box = (inset, I - inset - 1, inset, J - inset - 1, inset, K - inset - 1)
+            half = min(box[1] - box[0], box[3] - box[2], box[5] - box[4]) / 2
+            rmax = min(20.0, half - 3)
+            n_sph = rng.randint(0, 40) if rmax >= 2.0 else 0
+            idx = tab = None
+            if n_sph:
+                idx, tab = spheres(rng, I, J, K, box, n_sph, min(4.0, rmax), rmax, (2.0, 12.0), (0.0, 0.05))
+            theta, phi, psi = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
+            sc = Scenario(f"serve_{q}_spheres", I, J, K, steps, dx, dt, f0, cpml, inset, 5, (theta, phi, psi),
+                          gauss_sine(steps, dt, f0), mat_index=idx, mat_table=tab,
+                          probes=_probes(rng, 8, I, J, K, P), probe_comps=(3, 5), seed=seed + q)
+        else:
+            P = 12
+            cpml = (P, 3.0, 5.0, 0.05, 1.0, 0.8)
+            dx, dt, fc, idx, tab, ztop = _layered(rng, I, J, K, P, steps)
+            zp = max(0, min(K - 2 * P - 1, ztop - P + 1))
+            probes = np.array([[rng.randint(0, I - 2 * P - 1), rng.randint(0, J - 2 * P - 1), zp] for _ in range(8)],
+                              dtype=np.int32)
+            sc = Scenario(f"serve_{q}_layers", I, J, K, steps, dx, dt, fc, cpml, P + 5, 2, (1.0,),
+                          gauss_pulse(steps, dt, fc), mat_index=idx, mat_table=tab, probes=probes,
+                          probe_comps=(1, 5), seed=seed + q)
+        out.append((sc, steps, gap))
+    return out
+
+
 def small_variant(sc: Scenario, N: int, T: int) -> Scenario:
     """The same physics on an N^3 grid (materials regenerated, probes clipped) for parity tests."""
     d = dataclasses.replace(sc, I=N, J=N, K=N, T=T, source=sc.source[:T].copy())
@@ -289,6 +372,28 @@ def small_variant(sc: Scenario, N: int, T: int) -> Scenario:
         d.probes = _probes(rng, min(8, len(sc.probes)), N, N, N, P)
     d.snap_every = 0
     return d
+
+
+def setup_host(backend, sc: Scenario, use_arrays: bool = False):
+    """The host part of setup(): every op before ReadySimulation (what a request submitted to the
+    server carries; the server runs ReadySimulation and Init1D at admission)."""
+    nAnt = 0 if sc.probes is None else len(sc.probes)
+    backend.set_const([sc.fctr, sc.dx, sc.dt], [sc.T, sc.I, sc.J, sc.K, nAnt])
+    backend.set_cpml(list(sc.cpml))
+    backend.set_tfsf(sc.inset)
+    backend.set_source(sc.source)
+    if nAnt:
+        backend.set_probe_components(list(sc.probe_comps))
+        backend.set_antenna_locations(np.asarray(sc.probes, dtype=np.int32).T.reshape(-1))
+    if sc.mat_index is not None:
+        if use_arrays or not hasattr(backend, "set_material_index"):
+            arrs = sc.material_arrays()
+            for f in range(6):
+                backend.update_material(f + 1, arrs[f])
+        else:
+            backend.set_material_index(sc.mat_index, sc.mat_table)
+    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4) and sc.pw_params and sc.pw_params[0]):
+        backend.set_plane_wave(sc.mode, sc.pw_params)
 
 
 def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = True):

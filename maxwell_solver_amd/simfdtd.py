@@ -28,6 +28,8 @@ EXPORTS = [
     "fdtd_output_snapshots", "fdtd_synchronize", "fdtd_read_array", "fdtd_prepare",
     "fdtd_read_setup", "fdtd_set_tuning", "fdtd_autotune", "fdtd_batch_create", "fdtd_batch_destroy",
     "fdtd_batch_add", "fdtd_batch_clear", "fdtd_batch_run", "fdtd_batch_set_strategy", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
+    "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
+    "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
 ]
 
 
@@ -69,6 +71,14 @@ def lib():
             "fdtd_batch_create": [C.POINTER(vp)], "fdtd_batch_destroy": [vp],
             "fdtd_batch_add": [vp, vp, i32, i32, i32], "fdtd_batch_clear": [vp], "fdtd_batch_run": [vp],
             "fdtd_batch_set_strategy": [vp, i32],
+            "fdtd_server_create": [C.POINTER(vp), u64, i32, i32], "fdtd_server_destroy": [vp],
+            "fdtd_server_now": [vp, dp],
+            "fdtd_server_submit": [vp, vp, C.c_float, fp, i32, i32, C.c_double, C.POINTER(C.c_int64)],
+            "fdtd_server_round": [vp, ip, ip],
+            "fdtd_server_collect": [vp, C.POINTER(C.c_int64), dp, i32, ip],
+            "fdtd_server_release": [vp, C.c_int64],
+            "fdtd_server_stats": [vp, u64p, u64p, C.POINTER(C.c_int64)],
+            "fdtd_server_request_bytes": [vp, C.c_int64, u64p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -119,9 +129,9 @@ class Simulation:
             raise SimFDTDError(L.fdtd_last_error_id().decode(), L.fdtd_last_error().decode())
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owned", True):
             lib().fdtd_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -323,6 +333,82 @@ class Batch:
         if getattr(self, "_h", None):
             lib().fdtd_batch_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server (fdtd_server_*): solves submitted one at a time, advanced together with
+    continuous batching.  A submitted Simulation is owned by the server until release()."""
+
+    def __init__(self, hbm_budget: int, max_active: int = 8, steps_per_round: int = 64):
+        h = C.c_void_p()
+        Simulation._chk(lib().fdtd_server_create(C.byref(h), int(hbm_budget), int(max_active), int(steps_per_round)))
+        self._h = h
+        self._reqs = {}
+
+    def now(self) -> float:
+        t = C.c_double()
+        Simulation._chk(lib().fdtd_server_now(self._h, C.byref(t)))
+        return t.value
+
+    def submit(self, sim, count, mode, source=None, rate=1.0, t_arrive=-1.0) -> int:
+        """Queue `count` steps of `sim` (host set-up done, not ready) in plane-wave mode `mode`;
+        `source`: the Init1D samples (mode != 0).  Returns the request id."""
+        rid = C.c_int64()
+        src = _f32(source) if source is not None else None
+        Simulation._chk(lib().fdtd_server_submit(self._h, sim._h, float(rate),
+                                                  _p(src, C.c_float) if src is not None else None,
+                                                  int(count), int(mode), float(t_arrive), C.byref(rid)))
+        sim._owned = False
+        self._reqs[rid.value] = sim
+        return rid.value
+
+    def round(self):
+        """One round; returns (active, queued) after it."""
+        a, q = C.c_int32(), C.c_int32()
+        Simulation._chk(lib().fdtd_server_round(self._h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def collect(self, max_n: int = 1024):
+        """Completed requests: list of (id, t_arrive, t_admit, t_done)."""
+        ids = np.zeros(max_n, np.int64)
+        ts = np.zeros(3 * max_n, np.float64)
+        n = C.c_int32()
+        Simulation._chk(lib().fdtd_server_collect(self._h, _p(ids, C.c_int64), _p(ts, C.c_double), int(max_n),
+                                                   C.byref(n)))
+        return [(int(ids[k]), float(ts[3 * k]), float(ts[3 * k + 1]), float(ts[3 * k + 2])) for k in range(n.value)]
+
+    def simulation(self, rid: int):
+        return self._reqs[rid]
+
+    def release(self, rid: int):
+        Simulation._chk(lib().fdtd_server_release(self._h, int(rid)))
+        sim = self._reqs.pop(rid)
+        sim._h = None
+
+    def request_bytes(self, rid: int) -> int:
+        b = C.c_uint64()
+        Simulation._chk(lib().fdtd_server_request_bytes(self._h, int(rid), C.byref(b)))
+        return b.value
+
+    def stats(self):
+        """(bytes in use, peak bytes, cell updates)."""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_int64()
+        Simulation._chk(lib().fdtd_server_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fdtd_server_destroy(self._h)
+            self._h = None
+            for sim in self._reqs.values():
+                sim._h = None
+            self._reqs = {}
 
     def __del__(self):
         try:
