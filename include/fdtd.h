@@ -130,7 +130,9 @@ int fdtd_prepare(fdtd_sim *s);
 int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
 /* Launch tuning of the fused step's row kernel: rows per workgroup (1, 2 or 4; 0 = automatic),
  * z-chunk depth in planes (0 = automatic) and columns per lane (1, 2 or 4; 0 = automatic; the
- * grid's I must be a multiple of it and a row at most 256 of it).  An unsupported choice makes
+ * grid's I must be a multiple of it and a row at most 256 of it).  With 1 column per lane and I a
+ * multiple of 64 the row's two ends share one wave (split edge segment); -1 = 1 column per lane
+ * without that split.  An unsupported choice makes
  * the next step fail with FDTD_ERR_INVALID. */
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
 /* Batched engine (no counterpart in the reference; for many independent grids, each much smaller

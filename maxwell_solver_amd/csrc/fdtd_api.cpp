@@ -684,7 +684,7 @@ namespace {
 struct BatchPlan {
   // one epoch: the batch steps [t0, t1) with a fixed set of running grids
   struct Launch {
-    int kind;  // 0 slab H, 1 rows, 2 slab E
+    int kind;  // 0 TF/SF face sheets, 1 row kernel part 0 (interior), 2 row kernel part 1 (slabs)
     RowsKey key;
     bool idx = false, narrow = true;
     size_t tab = 0;  // element offset of the table in the device buffer
@@ -734,7 +734,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
   struct Prep {
     fdtd_batch::Item it;
     const void *rows_arg[2];
-    int rows_nwg;
+    int rows_nwg, rows_nwg1;
     RowsKey key;
     const void *slab_arg[2];
     unsigned nb_slab, nb_sheets;
@@ -764,7 +764,8 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
         break;
       }
       pr.rows_arg[w] = slot->dptr;
-      pr.rows_nwg = slot->nwg;
+      pr.rows_nwg = rows_part_nwg(slot->host.data(), 0);
+      pr.rows_nwg1 = rows_part_nwg(slot->host.data(), 1);
       pr.key = rows_key(slot->host.data());
       make_slab_args(f, w ? h1.data() : h0.data(), &pr.nb_slab, &pr.nb_sheets, &pr.narrow);
       pr.idx = f.mat != nullptr;
@@ -837,9 +838,9 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
           bg.off = off;
           bg.cur0 = pr->it.s->cur ^ (t0 & 1);
           bg.n0 = pr->it.n0 + t0;
-          for (int w = 0; w < 2; ++w) bg.arg[w] = kind == 1 ? pr->rows_arg[w] : pr->slab_arg[w];
+          for (int w = 0; w < 2; ++w) bg.arg[w] = kind == 0 ? pr->slab_arg[w] : pr->rows_arg[w];
           tabs.push_back(bg);
-          off += kind == 0 ? (int)(pr->nb_slab + pr->nb_sheets) : kind == 1 ? pr->rows_nwg : (int)pr->nb_slab;
+          off += kind == 0 ? (int)(pr->nb_slab + pr->nb_sheets) : kind == 1 ? pr->rows_nwg : pr->rows_nwg1;
         }
         BatchGrid end;
         end.off = off;
@@ -851,12 +852,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       std::map<RowsKey, std::vector<const Prep *>> by_key;
       for (const Prep *pr : run) by_key[pr->key].push_back(pr);
       for (auto &kv : by_key) add_table(1, kv.second, kv.first, false);
-      for (int idx = 0; idx < 2; ++idx) {
-        std::vector<const Prep *> grp;
-        for (const Prep *pr : run)
-          if (pr->idx == (idx == 1)) grp.push_back(pr);
-        if (!grp.empty()) add_table(2, grp, RowsKey(), idx == 1);
-      }
+      for (auto &kv : by_key) add_table(2, kv.second, kv.first, false);
       ep.probes = jobs.size();
       for (const Prep *pr : run) {
         fdtd_sim *s = pr->it.s;
@@ -913,9 +909,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       for (const auto &L : ep.launches)
         if (L.kind == 0) HIPCHK(launch_slab_h_batch(st, L.narrow, dt + L.tab, L.ng, L.nwg, step));
       for (const auto &L : ep.launches)
-        if (L.kind == 1) HIPCHK(launch_rows_batch(st, L.key, dt + L.tab, L.ng, L.nwg, step));
-      for (const auto &L : ep.launches)
-        if (L.kind == 2) HIPCHK(launch_slab_e_batch(st, L.idx, L.narrow, dt + L.tab, L.ng, L.nwg, step));
+        if (L.kind >= 1) HIPCHK(launch_rows_batch(st, L.key, L.kind - 1, dt + L.tab, L.ng, L.nwg, step));
       HIPCHK(launch_probes_batch(st, dj + ep.probes, ep.nprobe, ep.max_ant, step));
     }
   }
@@ -1630,11 +1624,12 @@ int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_
   if (rows_per_group != 0 && rows_per_group != 1 && rows_per_group != 2 && rows_per_group != 4)
     return fail(FDTD_ERR_INVALID, "rows_per_group must be 0, 1, 2 or 4");
   if (z_chunk < 0) return fail(FDTD_ERR_INVALID, "z_chunk must be >= 0");
-  if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4)
-    return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2 or 4");
+  if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4 && cols_per_lane != -1)
+    return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2, 4 or -1");
   s->tune.tile_rows = rows_per_group;
   s->tune.z_chunk = z_chunk;
-  s->tune.cpl = cols_per_lane;
+  s->tune.cpl = cols_per_lane < 0 ? 1 : cols_per_lane;
+  s->tune.no_split = cols_per_lane < 0;
   s->args_gen++;
   return 0;
 }

@@ -574,7 +574,11 @@ struct StepArgs {
   int i0, i1, j0, j1, k0, k1;
   int S, R, CPL, nTy;  // row segments per row, rows per workgroup, columns per lane, row tiles
   int KC, ntab, act_h, act_e;
-  int xni, xnj, xnk, pad;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
+  int KCS, nz1, nz2, pad1;  // z chunks: nz1 of KCS planes over [0, P), nz2 of KC over [P, K-P), nz1 over [K-P, K)
+  int nTlo, nThi, nTmid;    // row tiles of the bands [0, P], [J-P, J) and [P+1, J-P)
+  int nwg0, nwg1;           // workgroups of the two parts (host information)
+  int xni, xnj, xnk;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
+  int spl;            // split edge segment (1 column per lane, I a multiple of 64; see rows_body)
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -835,9 +839,11 @@ struct RowPlane {
   int mt[CPL];                                   // material index (index mode)
   float pHy[CPL], pHz[CPL], pHzm[CPL], pEy[CPL], pEz[CPL];  // x-slab psi (edge waves)
   float vhy, vhz, vhzm, vez, vey;                // x-face incident values (edge waves)
+  float qHxy[CPL], qHzy[CPL], qExy[CPL], qEzy[CPL], qHxym[CPL], qHzym[CPL];  // y-slab psi (slab waves)
+  float qHxz[CPL], qHyz[CPL], qExz[CPL], qEyz[CPL], qHxzm[CPL];              // z-slab psi
 };
 
-template <bool IDX, bool BUF, int CPL>
+template <bool IDX, bool BUF, int CPL, int PART>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
   // it is loaded with scalar loads into scalar registers (also where the block's address was
@@ -878,41 +884,99 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   // offsets and flags in scalar registers (and branch on them) instead of per-lane values
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int seg = wv % S, r = wv / S;
-  const int ty = L % nTy, tz = L / nTy;
-  const int j = uni(P + R * ty + r);
-  const int kb = P + tz * KC, ke = min(kb + KC, K - P);
-  const bool row_ok = j < J - P;
+  // Part 0: the interior band of rows [P+1, J-P) over the interior chunks of planes [P, K-P).
+  // Part 1: the rest -- the bands [0, P] and [J-P, J) over every chunk, and the interior band over
+  // the z-slab chunks.  The two parts read only the old state and write disjoint cells.
+  int row0, row_end, tz;
+  {
+    const int nTlo = pa->nTlo, nThi = pa->nThi, nTmid = pa->nTmid, n1 = pa->nz1, n2 = pa->nz2;
+    if (PART == 0) {
+      row0 = P + 1 + R * (L % nTmid); row_end = J - P; tz = n1 + L / nTmid;
+    } else {
+      const int nb = nTlo + nThi, nA = nb * (2 * n1 + n2);
+      if (L < nA) {
+        const int t = L % nb;
+        tz = L / nb;
+        if (t < nTlo) { row0 = R * t; row_end = P + 1; }
+        else { row0 = J - P + R * (t - nTlo); row_end = J; }
+      } else {
+        const int q = L - nA, zz = q / nTmid;
+        row0 = P + 1 + R * (q % nTmid); row_end = J - P; tz = zz < n1 ? zz : n1 + n2 + (zz - n1);
+      }
+    }
+    (void)nTy;
+  }
+  const int j = uni(row0 + r);
+  // z chunk of the wave; chunks do not straddle the z-slab boundaries P and K-P
+  int kb, ke;
+  {
+    const int n1 = pa->nz1, n2 = pa->nz2, KCS = pa->KCS;
+    if (tz < n1) { kb = tz * KCS; ke = min(kb + KCS, P); }
+    else if (tz < n1 + n2) { kb = P + (tz - n1) * KC; ke = min(kb + KC, K - P); }
+    else { kb = K - P + (tz - n1 - n2) * KCS; ke = min(kb + KCS, K); }
+    kb = uni(kb); ke = uni(ke);
+  }
+  const bool row_ok = j < row_end;
   if (S == 1 && !row_ok) return;  // whole wave, and no barrier follows when S == 1
-  const int ib = seg * W + CPL * lane;  // first column of the lane
+  // Columns of the lane.  Plain layout: segment g holds the columns [64 CPL g, 64 CPL (g+1)).
+  // Split layout (1 column per lane, I a multiple of 64): segment 0 holds both ends of the row,
+  // lanes 0-31 the columns [0, 32) and lanes 32-63 the columns [I-32, I), and segment g >= 1 the
+  // columns [64 g - 32, 64 g + 32); the x CPML slabs and (usually) the x faces of the TF/SF box
+  // then fall in one wave per row instead of two, and every other wave runs the plain march.
+  const bool spl = CPL == 1 && pa->spl;
+  const int ib = spl ? (seg == 0 ? (lane < 32 ? lane : I - 64 + lane) : 64 * seg - 32 + lane) : seg * W + CPL * lane;
+  // Lanes and waves of the exchange with the neighbouring segments: lane e_pub's E (its column's
+  // left neighbour is in wave w_prev) and lane h_pub's new H (its right neighbour is in wave
+  // w_next) go through LDS to lane e_rd of wave w_prev and lane h_rd of wave w_next.
+  const int wrow = wv - seg;
+  const int w_next = wrow + (spl ? (seg + 1) % S : seg + 1), w_prev = wrow + (spl ? (seg + S - 1) % S : seg - 1);
+  const bool has_next = spl || seg + 1 < S, has_prev = spl || seg > 0;
+  const int e_pub = spl && seg == 0 ? 32 : 0, h_pub = spl && seg == 0 ? 31 : 63;
+  const int e_rd = h_pub, h_rd = e_pub;
   const bool on = row_ok && ib < I;     // I is a multiple of CPL: all CPL columns are in the grid
-  const bool jm_C = j - 1 >= P;         // row j-1 is recomputed (else: a y-slab row, read new)
+  const bool has_jm = j >= 1;           // row j-1 exists (its Hx, Hz are recomputed from the old state)
 
-  // arrays and uniform offsets (bytes): component c at c*cs, row j-1 / j / j+1 at 0 / I / 2I,
-  // plane k+1 at IJ
+  // arrays and uniform offsets (bytes): component c at c*cs; the lane offset o0 addresses row
+  // max(j-1, 0), row j is at r1 and row j+1 at r2 from it (on the first row, row j-1 is row 0 and
+  // on the last, row j+1 is row j: values read there are not used); plane k+1 at IJ
   const Acc<BUF> Fc(pa->Fc, 6 * cs * 4), Fn(pa->Fn, 6 * cs * 4);
-  // H of row j-1: the old state where it is recomputed, else the new state
-  const Acc<BUF> Fm(uni(jm_C ? pa->Fc : (const float *)pa->Fn), 6 * cs * 4);
   const Acc<BUF> Mt(pa->mat, IDX ? IJK : 0), Cf(pa->coef, IDX ? 0 : 24 * IJK);
   const O c1 = (O)(cs * 4), c2 = 2 * c1, c3 = 3 * c1, c4 = 4 * c1, c5 = 5 * c1;
-  const O r1 = (O)(I * 4), r2 = 2 * r1, pz = (O)(IJ * 4);
+  const O r1 = uni(has_jm ? (O)(I * 4) : (O)0), r2 = uni(r1 + (j + 1 < J ? (O)(I * 4) : (O)0)), pz = (O)(IJ * 4);
 
-  const int k_first = uni(kb > P ? kb - 1 : kb), n_planes = uni(ke - k_first);
-  const size_t col = (on ? (size_t)ib : 0) + (size_t)(j - 1) * I;
+  const int k_first = uni(kb > 0 ? kb - 1 : kb), n_planes = uni(ke - k_first);
+  const size_t col = (on ? (size_t)ib : 0) + (size_t)(has_jm ? j - 1 : 0) * I;
   const O o0 = (O)((col + (size_t)k_first * IJ) * 4);
 
   // plain wave (see above); the z faces of the box are applied on their planes under uniform
   // branches in both marches
-  bool plain = row_ok && seg * W >= P && (seg + 1) * W <= I - P;
+  // slab wave: a row in a y CPML slab or next to one (its row j-1 is recomputed with the slab's psi),
+  // or a chunk of z-slab planes
+  const bool slabw = j <= P || j >= J - P || kb < P || ke > K - P;
+  const int lo = spl ? 64 * seg - 32 : seg * W, hi = lo + W;
+  bool plain = row_ok && !slabw && !(spl && seg == 0) && lo >= P && hi <= I - P;
   if (act_h | act_e) {
-    const int lo = seg * W, hi = lo + W;
     plain = plain && !(bi0 - 1 >= lo && bi0 - 1 < hi) && !(bi0 >= lo && bi0 < hi) && !(bi1 >= lo && bi1 < hi);
     plain = plain && j != bj0 - 1 && j != bj0 && j != bj1 && j != bj1 + 1;
   }
   const bool zrow = j >= bj0 && j <= bj1, zrow_m = j - 1 >= bj0 && j - 1 <= bj1 - 1;
 
-  auto march = [&](auto edge_tag) {
-    constexpr bool EDGE = decltype(edge_tag)::value;
+  // y-slab rows: slab index and 1/kappa_y of rows j and j-1 (1 off the slabs)
+  const AS4 float *prmc = (const AS4 float *)pa->prm;
+  const bool ys = j < P || j >= J - P, ysm = has_jm && (j - 1 < P || j - 1 >= J - P);
+  const int sy = j < P ? j : j - (J - P2), sym = j - 1 < P ? j - 1 : j - 1 - (J - P2);
+  const int ikE0 = 4 * P2, ikH0 = 4 * P2 + I + J + K;  // 1/kappa tables of E and H: x, y, z
+  // Waves run one of three marches: 0 plain, 1 edge (x-slab columns, TF/SF faces, partial
+  // segments), 2 slab (y-slab rows and the rows above them, z-slab chunks; also the edge work).
+  auto march = [&](auto kind_tag) {
+    constexpr int KIND = decltype(kind_tag)::value;
+    constexpr bool EDGE = KIND >= 1, SLAB = KIND == 2;
     const bool lon = !EDGE || on;
+    const float kyH = SLAB ? prmc[ikH0 + I + j] : 1.0f, kyE = SLAB ? prmc[ikE0 + I + j] : 1.0f;
+    const float kyHm = SLAB && has_jm ? prmc[ikH0 + I + j - 1] : 1.0f;
+    const int syc = ys ? sy : 0, symc = ysm ? sym : 0;
+    const float bHy = sSlab[2][syc], cHy = sSlab[3][syc], bEy = sSlab[0][syc], cEy = sSlab[1][syc];
+    const float bHym = sSlab[2][symc], cHym = sSlab[3][symc];
 
     // ---- per-lane constants of the march ----
     // Predicates are bits of one register (FS per column c, then lane bits), re-read through an
@@ -956,11 +1020,17 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     // and the rows the z faces of the box cover (j, and j-1 for the recomputed Hx)
     const int jfl = uni((j >= bj0 && j <= bj1 ? 1 : 0) | (j >= bj0 && j <= bj1 - 1 ? 2 : 0) |
                         (j - 1 >= bj0 && j - 1 <= bj1 - 1 ? 4 : 0) | (zrow ? 8 : 0) | (zrow_m ? 16 : 0));
-    // x-slab psi groups [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k
-    const size_t sx = pa->sx;
-    const Acc<BUF> Pc(pa->Pc, EDGE ? 16 * (sx + pa->sy + pa->sz) : 0);
-    const Acc<BUF> Pn(pa->Pn, EDGE ? 16 * (sx + pa->sy + pa->sz) : 0);
+    // psi groups (bytes): x [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k;
+    // y [Exy Ezy Hxy Hzy] of 2P I K, element i + I s + 2P I k; z [Exz Eyz Hxz Hyz] of 2P I J,
+    // element i + I j + I J s
+    const size_t sx = pa->sx, syy = pa->sy, szz = pa->sz;
+    const Acc<BUF> Pc(pa->Pc, 16 * (sx + syy + szz));
+    const Acc<BUF> Pn(pa->Pn, 16 * (sx + syy + szz));
     const O gEz = (O)(sx * 4), gHy = 2 * gEz, gHz = 3 * gEz;
+    const O gExy = 4 * gEz, gEzy = gExy + (O)(syy * 4), gHxy = gExy + (O)(2 * syy * 4), gHzy = gExy + (O)(3 * syy * 4);
+    const O gExz = gExy + (O)(4 * syy * 4), gEyz = gExz + (O)(szz * 4), gHxz = gExz + (O)(2 * szz * 4),
+            gHyz = gExz + (O)(3 * szz * 4);
+    const O oc = (O)((on ? (size_t)ib : 0) * 4);  // lane offset of the y- and z-slab psi (column i)
     O so[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 4);
@@ -972,40 +1042,58 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     if (lon) {
       Fc.ld(o0 + r1, 0, ex); Fc.ld(o0 + r1, c1, ey); Fc.ld(o0 + r1, c2, ez);
       Fc.ld(o0, c1, eym);
-      if (kb == P) {  // plane P-1 is a z-slab plane: its new H is already written
-        Fn.ld(o0 + r1 - pz, c3, hpx);
-        Fn.ld(o0 + r1 - pz, c4, hpy);
-      }
     }
     // E at the first column of the next segment (plane k), for the lane's last column
     float bey = 0, bez = 0, beym = 0;
     int par = 0;
     if (S > 1) {
-      if (lane == 0) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
+      if (lane == e_pub) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
       lds_barrier();
-      if (lane == 63 && seg + 1 < S) { bey = sX[0][wv + 1][0]; bez = sX[0][wv + 1][1]; beym = sX[0][wv + 1][2]; }
+      if (lane == e_rd && has_next) { bey = sX[0][w_next][0]; bez = sX[0][w_next][1]; beym = sX[0][w_next][2]; }
       par = 1;
     }
 
     // ---- loads of plane index it ----
-    auto load = [&](RowPlane<CPL> &Lp, int it) {
-      const int itc = uni(min(it, n_planes - 1));
-      const int k = k_first + itc;
-      const O o = o0 + (O)itc * pz;
+    auto load = [&](RowPlane<CPL> &Lp, int it, auto own_tag) {
+      constexpr bool own = decltype(own_tag)::value;
+      const int k = uni(k_first + it);
+      const O o = o0 + (O)it * pz;
       const O o1 = o + r1, o2 = o + r2;
+      // plane k+1 (the last plane of the grid reads itself: its H is not updated)
+      const O pn1 = SLAB ? uni(k + 1 < K ? pz : (O)0) : pz;
       unsigned f = lf;
       int ah = act_h, ae = act_e, jf = jfl;
       if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
       const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4;
       if (!EDGE || bit(f, F_ON)) {
-        Fc.ld(o1, pz, Lp.nx); Fc.ld(o1, c1 + pz, Lp.ny); Fc.ld(o1, c2 + pz, Lp.nz);
-        Fc.ld(o, c1 + pz, Lp.nym);
+        Fc.ld(o1, pn1, Lp.nx); Fc.ld(o1, c1 + pn1, Lp.ny); Fc.ld(o1, c2 + pn1, Lp.nz);
+        Fc.ld(o, c1 + pn1, Lp.nym);
         Fc.ld(o, 0, Lp.exm); Fc.ld(o, c2, Lp.ezm);
         Fc.ld(o2, 0, Lp.exq); Fc.ld(o2, c2, Lp.ezq);
         Fc.ld(o1, c3, Lp.hx); Fc.ld(o1, c4, Lp.hy); Fc.ld(o1, c5, Lp.hz);
-        Fm.ld(o, c3, Lp.hxm);
-        Fm.ld(o, c5, Lp.hzm);
-        if (IDX) Mt.ldm(o1 >> 2, 0, Lp.mt);
+        Fc.ld(o, c3, Lp.hxm);
+        Fc.ld(o, c5, Lp.hzm);
+        if (IDX && own) Mt.ldm(o1 >> 2, 0, Lp.mt);
+        // y-slab psi of rows j and j-1
+        if (SLAB && ys) {
+          const O u = uni((O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 4));
+          Pc.ld(oc, gHxy + u, Lp.qHxy); Pc.ld(oc, gHzy + u, Lp.qHzy);
+          if (own) { Pc.ld(oc, gExy + u, Lp.qExy); Pc.ld(oc, gEzy + u, Lp.qEzy); }
+        }
+        if (SLAB && own && ysm) {
+          const O u = uni((O)((size_t)I * ((size_t)sym + (size_t)P2 * k) * 4));
+          Pc.ld(oc, gHxy + u, Lp.qHxym); Pc.ld(oc, gHzy + u, Lp.qHzym);
+        }
+        // z-slab psi of plane k (also on the plane below an interior chunk, when that is a slab plane)
+        if ((SLAB || !own) && (k < P || k >= K - P)) {
+          const int sz = k < P ? k : k - (K - P2);
+          const O u = uni((O)(((size_t)I * j + IJ * (size_t)sz) * 4));
+          Pc.ld(oc, gHxz + u, Lp.qHxz); Pc.ld(oc, gHyz + u, Lp.qHyz);
+          if (own) {
+            Pc.ld(oc, gExz + u, Lp.qExz); Pc.ld(oc, gEyz + u, Lp.qEyz);
+            if (has_jm) Pc.ld(oc, gHxz + u - (O)(I * 4), Lp.qHxzm);
+          }
+        }
         if (EDGE && bit(f, F_SLAB)) {
           const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
 #pragma unroll
@@ -1013,7 +1101,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             if (slab_col(f, c)) {
               Lp.pHy[c] = Pc.ld1(so[c], gHy + pq);
               Lp.pHz[c] = Pc.ld1(so[c], gHz + pq);
-              if (jm_C) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
+              if (has_jm) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
               Lp.pEy[c] = Pc.ld1(so[c], pq);
               Lp.pEz[c] = Pc.ld1(so[c], gEz + pq);
             }
@@ -1027,7 +1115,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         if (bit(f, F_XHL)) {
           if ((ah & 2) && xj0 && xk0) Lp.vhy = ldo(xs, (xh_sel + xo) * 4u);
           if ((ah & 4) && xj1 && xk1) Lp.vhz = ldo(xs + xS, (xh_sel + xo) * 4u);
-          if (jm_C && (ah & 4) && xjm1 && xk1) Lp.vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
+          if (has_jm && (ah & 4) && xjm1 && xk1) Lp.vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
         }
         if (bit(f, F_XEL)) {
           if ((ae & 4) && xj0 && xk0) Lp.vez = ldo(xs, (xe_sel + xo) * 4u);
@@ -1037,9 +1125,9 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     };
 
     // ---- plane index it: H(k) of row j, Hx, Hz(k) of row j-1, then E(k) of row j ----
-    auto compute = [&](RowPlane<CPL> &Lp, int it) {
+    auto compute = [&](RowPlane<CPL> &Lp, int it, auto own_tag) {
       const int k = uni(k_first + it);
-      const bool own = k >= kb;  // planes of the chunk (the plane below only feeds H(k-1))
+      constexpr bool own = decltype(own_tag)::value;  // a plane of the chunk (the plane below only feeds H(k-1))
       const O o1 = o0 + (O)it * pz + r1;
       const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
       unsigned f = lf;
@@ -1050,21 +1138,31 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
       // E at i+1 of the lane's last column
       float ey_n = __shfl_down(ey[0], 1), ez_n = __shfl_down(ez[0], 1), eym_n = __shfl_down(eym[0], 1);
-      if (S > 1 && lane == 63) { ey_n = bey; ez_n = bez; eym_n = beym; }
+      if (S > 1 && lane == e_rd) { ey_n = bey; ez_n = bez; eym_n = beym; }
 
-      // H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML X (src/gpu.h:604-725)
+      // z-slab plane: slab index, profile and 1/kappa_z (the plane below an interior chunk may be
+      // a slab plane too); off the slabs 1/kappa_z is exactly 1 and its factor is omitted
+      constexpr bool ZC = SLAB || !own;
+      const bool zs = ZC && (k < P || k >= K - P);
+      const int sz = zs ? (k < P ? k : k - (K - P2)) : 0;
+      const float kzH = ZC ? prmc[ikH0 + I + J + k] : 1.0f, kzE = SLAB ? prmc[ikE0 + I + J + k] : 1.0f;
+      const float bHz = sSlab[2][sz], cHz = sSlab[3][sz], bEz = sSlab[0][sz], cEz = sSlab[1][sz];
+      // uniform parts of the update masks (slab waves reach the grid's first and last rows / planes)
+      const bool h_jk = !SLAB || (j < J - 1 && k < K - 1);
+
+      // H(i, j, k): K1 GPU_updateH (src/gpu.h:592-601), TF/SF, CPML Y-X-Z (src/gpu.h:604-725)
       float hx[CPL], hy[CPL], hz[CPL], hxm[CPL], hzm[CPL];
 #pragma unroll
       for (int c = 0; c < CPL; ++c) {
         const float ey1 = c + 1 < CPL ? ey[c + 1 < CPL ? c + 1 : 0] : ey_n;
         const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
         hx[c] = Lp.hx[c]; hy[c] = Lp.hy[c]; hz[c] = Lp.hz[c];
-        if (!EDGE || bit(f, FS * c + F_H)) {
+        if (h_jk && (!EDGE || bit(f, FS * c + F_H))) {
           const float dyz = EDGE ? (ez1 - ez[c]) * kH[c] : ez1 - ez[c];
           const float dzy = EDGE ? (ey1 - ey[c]) * kH[c] : ey1 - ey[c];
-          hx[c] = Chh * hx[c] + Che * (-(Lp.ezq[c] - ez[c]) + (Lp.ny[c] - ey[c]));
-          hy[c] = Chh * hy[c] + Che * (-(Lp.nx[c] - ex[c]) + dyz);
-          hz[c] = Chh * hz[c] + Che * (-dzy + (Lp.exq[c] - ex[c]));
+          hx[c] = Chh * hx[c] + Che * (-(Lp.ezq[c] - ez[c]) * kyH + (Lp.ny[c] - ey[c]) * kzH);
+          hy[c] = Chh * hy[c] + Che * (-(Lp.nx[c] - ex[c]) * kzH + dyz);
+          hz[c] = Chh * hz[c] + Che * (-dzy + (Lp.exq[c] - ex[c]) * kyH);
         }
         if (EDGE && bit(f, FS * c + F_XH)) {
           if ((ah & 2) && xj0 && xk0) hy[c] = hy[c] + (Che * Lp.vhy) * sxh;
@@ -1117,7 +1215,22 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           }
         }
       }
-      if (EDGE && bit(f, F_SLAB)) {
+      // CPML Y (K5: rows of the y slabs, s < 2P-1, i < I-1, k < K-1)
+      if (SLAB && ys && sy < P2 - 1 && k < K - 1) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          if (bit(f, FS * c + F_H)) {
+            const float pxy = bHy * Lp.qHxy[c] + cHy * (Lp.ezq[c] - ez[c]) / dx;
+            hx[c] = hx[c] - Che * pxy * dx;
+            const float pzy = bHy * Lp.qHzy[c] + cHy * (Lp.exq[c] - ex[c]) / dx;
+            hz[c] = hz[c] + Che * pzy * dx;
+            Lp.qHxy[c] = pxy;
+            Lp.qHzy[c] = pzy;
+          }
+        }
+      }
+      // CPML X (K4: x-slab columns, s < 2P-1, j < J-1, k < K-1)
+      if (EDGE && h_jk && bit(f, F_SLAB)) {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           if (bit(f, FS * c + F_SLH)) {
@@ -1132,19 +1245,34 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           }
         }
       }
+      // CPML Z (K6: z-slab planes, s < 2P-1, i < I-1, j < J-1)
+      if (zs && sz < P2 - 1 && (!SLAB || j < J - 1)) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+          if (!EDGE || bit(f, FS * c + F_H)) {
+            const float pxz = bHz * Lp.qHxz[c] + cHz * (Lp.ny[c] - ey[c]) / dx;
+            hx[c] = hx[c] + Che * pxz * dx;
+            const float pyz = bHz * Lp.qHyz[c] + cHz * (Lp.nx[c] - ex[c]) / dx;
+            hy[c] = hy[c] - Che * pyz * dx;
+            Lp.qHxz[c] = pxz;
+            Lp.qHyz[c] = pyz;
+          }
+        }
+      }
 
-      // Hx, Hz at (i, j-1, k), recomputed the same way
+      // Hx, Hz at (i, j-1, k), recomputed the same way (not needed on the plane below the chunk)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) { hxm[c] = Lp.hxm[c]; hzm[c] = Lp.hzm[c]; }
-      if (jm_C) {
+      if (own && has_jm) {
         const int jm = j - 1;
+        const bool hm_k = !SLAB || k < K - 1;
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
-          if (!EDGE || bit(f, FS * c + F_H)) {
+          if (hm_k && (!EDGE || bit(f, FS * c + F_H))) {
             const float dzy = EDGE ? (eym1 - eym[c]) * kH[c] : eym1 - eym[c];
-            hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - Lp.ezm[c]) + (Lp.nym[c] - eym[c]));
-            hzm[c] = Chh * hzm[c] + Che * (-dzy + (ex[c] - Lp.exm[c]));
+            hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - Lp.ezm[c]) * kyHm + (Lp.nym[c] - eym[c]) * kzH);
+            hzm[c] = Chh * hzm[c] + Che * (-dzy + (ex[c] - Lp.exm[c]) * kyHm);
           }
           if (EDGE && bit(f, FS * c + F_XH) && (ah & 4) && xjm1 && xk1) hzm[c] = hzm[c] - (Che * Lp.vhzm) * sxh;
         }
@@ -1185,7 +1313,19 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             }
           }
         }
-        if (EDGE && bit(f, F_SLAB)) {
+        // CPML Y of row j-1 (a y-slab row)
+        if (SLAB && ysm && sym < P2 - 1 && k < K - 1) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_H)) {
+              const float pxy = bHym * Lp.qHxym[c] + cHym * (ez[c] - Lp.ezm[c]) / dx;
+              hxm[c] = hxm[c] - Che * pxy * dx;
+              const float pzy = bHym * Lp.qHzym[c] + cHym * (ex[c] - Lp.exm[c]) / dx;
+              hzm[c] = hzm[c] + Che * pzy * dx;
+            }
+          }
+        }
+        if (EDGE && hm_k && bit(f, F_SLAB)) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             if (bit(f, FS * c + F_SLH)) {
@@ -1195,16 +1335,26 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             }
           }
         }
+        // CPML Z of row j-1 (Hx only)
+        if (SLAB && zs && sz < P2 - 1) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_H)) {
+              const float pxz = bHz * Lp.qHxzm[c] + cHz * (Lp.nym[c] - eym[c]) / dx;
+              hxm[c] = hxm[c] + Che * pxz * dx;
+            }
+          }
+        }
       }
 
       // H at i-1 of the lane's first column; segment exchange
       float hy_p = __shfl_up(hy[CPL - 1], 1), hz_p = __shfl_up(hz[CPL - 1], 1);
       if (S > 1) {
-        if (lane == 63) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
-        if (lane == 0) { sX[par][wv][0] = Lp.ny[0]; sX[par][wv][1] = Lp.nz[0]; sX[par][wv][2] = Lp.nym[0]; }
+        if (lane == h_pub) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
+        if (lane == e_pub) { sX[par][wv][0] = Lp.ny[0]; sX[par][wv][1] = Lp.nz[0]; sX[par][wv][2] = Lp.nym[0]; }
         lds_barrier();
-        if (lane == 0 && seg > 0) { hy_p = sX[par][wv - 1][3]; hz_p = sX[par][wv - 1][4]; }
-        if (lane == 63 && seg + 1 < S) { bey = sX[par][wv + 1][0]; bez = sX[par][wv + 1][1]; beym = sX[par][wv + 1][2]; }
+        if (lane == h_rd && has_prev) { hy_p = sX[par][w_prev][3]; hz_p = sX[par][w_prev][4]; }
+        if (lane == e_rd && has_next) { bey = sX[par][w_next][0]; bez = sX[par][w_next][1]; beym = sX[par][w_next][2]; }
         par ^= 1;
       }
 
@@ -1223,7 +1373,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
               }
           }
         }
-        // E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML X (src/gpu.h:765-868)
+        // E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML Y-Z-X (src/gpu.h:765-868)
         float ce[3][CPL], ch[3][CPL];
         if (IDX) {
 #pragma unroll
@@ -1250,14 +1400,16 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         for (int c = 0; c < CPL; ++c) {
           const float hym1 = c > 0 ? hy[c > 0 ? c - 1 : 0] : hy_p;
           const float hzm1 = c > 0 ? hz[c > 0 ? c - 1 : 0] : hz_p;
-          ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) - (hy[c] - hpy[c]));
+          ux[c] = ex[c];
           uy[c] = ey[c];
           uz[c] = ez[c];
+          if (!SLAB || (j != 0 && k != 0))
+            ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) * kyE - (hy[c] - hpy[c]) * kzE);
           if (!EDGE || bit(f, FS * c + F_E)) {
             const float dzx = EDGE ? (hz[c] - hzm1) * kE[c] : hz[c] - hzm1;
             const float dyx = EDGE ? (hy[c] - hym1) * kE[c] : hy[c] - hym1;
-            uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) - dzx);
-            uz[c] = ce[2][c] * ez[c] + ch[2][c] * (dyx - (hx[c] - hxm[c]));
+            if (!SLAB || k != 0) uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) * kzE - dzx);
+            if (!SLAB || j != 0) uz[c] = ce[2][c] * ez[c] + ch[2][c] * (dyx - (hx[c] - hxm[c]) * kyE);
           }
           if (EDGE && bit(f, FS * c + F_XE)) {
             if ((ae & 4) && xj0 && xk0) uz[c] = uz[c] + (ch[2][c] * Lp.vez) * sxe;
@@ -1309,7 +1461,36 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             }
           }
         }
-        if (EDGE && bit(f, F_SLAB)) {
+        // CPML Y (K8: rows of the y slabs, s >= 1, 0 < i < I-1, 0 < k < K-1)
+        if (SLAB && ys && sy >= 1 && k >= 1 && k < K - 1) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_E) && bit(f, FS * c + F_H)) {
+              const float pxy = bEy * Lp.qExy[c] + cEy * (hz[c] - hzm[c]) / dx;
+              ux[c] = ux[c] + ch[0][c] * pxy * dx;
+              const float pzy = bEy * Lp.qEzy[c] + cEy * (hx[c] - hxm[c]) / dx;
+              uz[c] = uz[c] - ch[2][c] * pzy * dx;
+              Lp.qExy[c] = pxy;
+              Lp.qEzy[c] = pzy;
+            }
+          }
+        }
+        // CPML Z (K9: z-slab planes, s >= 1, 0 < i < I-1, 0 < j < J-1)
+        if (SLAB && zs && sz >= 1 && j >= 1 && j < J - 1) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            if (bit(f, FS * c + F_E) && bit(f, FS * c + F_H)) {
+              const float pxz = bEz * Lp.qExz[c] + cEz * (hy[c] - hpy[c]) / dx;
+              ux[c] = ux[c] - ch[0][c] * pxz * dx;
+              const float pyz = bEz * Lp.qEyz[c] + cEz * (hx[c] - hpx[c]) / dx;
+              uy[c] = uy[c] + ch[1][c] * pyz * dx;
+              Lp.qExz[c] = pxz;
+              Lp.qEyz[c] = pyz;
+            }
+          }
+        }
+        // CPML X (K7: x-slab columns, s >= 1, 0 < j < J-1, 0 < k < K-1)
+        if (EDGE && (!SLAB || (j >= 1 && j < J - 1 && k >= 1 && k < K - 1)) && bit(f, F_SLAB)) {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             if (bit(f, FS * c + F_SLE)) {
@@ -1337,6 +1518,17 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
                 Pn.st1(so[c], gEz + pq, Lp.pEz[c]);
               }
           }
+          // y- and z-slab psi: every element written each step (updated, or copied outside the masks)
+          if (SLAB && ys) {
+            const O u = uni((O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 4));
+            Pn.st(oc, gHxy + u, Lp.qHxy); Pn.st(oc, gHzy + u, Lp.qHzy);
+            Pn.st(oc, gExy + u, Lp.qExy); Pn.st(oc, gEzy + u, Lp.qEzy);
+          }
+          if (SLAB && zs) {
+            const O u = uni((O)(((size_t)I * j + IJ * (size_t)sz) * 4));
+            Pn.st(oc, gHxz + u, Lp.qHxz); Pn.st(oc, gHyz + u, Lp.qHyz);
+            Pn.st(oc, gExz + u, Lp.qExz); Pn.st(oc, gEyz + u, Lp.qEyz);
+          }
         }
       }
 #pragma unroll
@@ -1354,21 +1546,39 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       A.nx[c] = A.ny[c] = A.nz[c] = A.nym[c] = A.exm[c] = A.ezm[c] = A.exq[c] = A.ezq[c] = 0.0f;
       A.hx[c] = A.hy[c] = A.hz[c] = A.hxm[c] = A.hzm[c] = 0.0f;
       A.pHy[c] = A.pHz[c] = A.pHzm[c] = A.pEy[c] = A.pEz[c] = 0.0f;
+      A.qHxy[c] = A.qHzy[c] = A.qExy[c] = A.qEzy[c] = A.qHxym[c] = A.qHzym[c] = 0.0f;
+      A.qHxz[c] = A.qHyz[c] = A.qExz[c] = A.qEyz[c] = A.qHxzm[c] = 0.0f;
       A.mt[c] = 0;
     }
-    for (int it = 0; it < n_planes; ++it) {
-      load(A, it);
-      compute(A, it);
+    // The plane below the chunk (H only) is peeled off the march, so that every load of a march
+    // iteration is consumed within it: a load left in flight across the back edge (the material
+    // byte of a plane whose E is not updated) made the compiler wait for all memory operations,
+    // the previous plane's stores included, before the next plane's loads.
+    int it0 = 0;
+    if (k_first < kb) {
+      load(A, 0, std::false_type{});
+      compute(A, 0, std::false_type{});
+      it0 = 1;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int it = it0; it < n_planes; ++it) {
+      load(A, it, std::true_type{});
+      compute(A, it, std::true_type{});
     }
   };
 
-  if (uni((int)plain)) march(std::false_type{});
-  else march(std::true_type{});
+  if constexpr (PART == 0) {
+    if (uni((int)plain)) march(std::integral_constant<int, 0>{});
+    else march(std::integral_constant<int, 1>{});
+  } else {
+    (void)slabw;
+    march(std::integral_constant<int, 2>{});
+  }
 }
 
-template <bool IDX, bool BUF, int CPL>
+template <bool IDX, bool BUF, int CPL, int PART>
 __global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
-  rows_body<IDX, BUF, CPL>(pa, xcd_remap(blockIdx.x, gridDim.x));
+  rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Batched launches cover the workgroups of many grids: tab[g].off is grid g's first workgroup
@@ -1386,12 +1596,12 @@ __device__ __forceinline__ int find_grid(const BatchGrid *__restrict__ tab, int 
 
 // Blocks of one grid whose local index agrees mod 8 run on the same XCD (blocks are dealt to the
 // XCDs round-robin), so the per-grid remap is xcd_remap of the local index.
-template <bool IDX, bool BUF, int CPL>
+template <bool IDX, bool BUF, int CPL, int PART>
 __global__ void __launch_bounds__(256) k_rows_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
   const int b = blockIdx.x, g = find_grid(tab, ng, b);
   const int par = (tab[g].cur0 + step) & 1;
   const int off = tab[g].off, nb = tab[g + 1].off - off;
-  rows_body<IDX, BUF, CPL>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
+  rows_body<IDX, BUF, CPL, PART>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
 }
 
 
@@ -1662,11 +1872,12 @@ static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
   return best_waste >= 0;
 }
 
-template <bool IDX, bool BUF>
+template <bool IDX, bool BUF, int PART>
 static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const StepArgs *d, int n) {
-  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4>), grid, blk, 0, st, d, n);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2>), grid, blk, 0, st, d, n);
-  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1>), grid, blk, 0, st, d, n);
+  if (grid.x == 0) return;
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, 0, st, d, n);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, 0, st, d, n);
+  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, 0, st, d, n);
 }
 
 static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int nwg, int n) {
@@ -1674,13 +1885,14 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
-  const dim3 grid(nwg), blk(64 * a.S * a.R);
+  const dim3 g0(a.nwg0), g1(a.nwg1), blk(64 * a.S * a.R);
+  (void)nwg;
   if (a.mat) {
-    if (buf) launch_rows_w<true, true>(st, a.CPL, grid, blk, dargs, n);
-    else launch_rows_w<true, false>(st, a.CPL, grid, blk, dargs, n);
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, blk, dargs, n); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, blk, dargs, n); }
   } else {
-    if (buf) launch_rows_w<false, true>(st, a.CPL, grid, blk, dargs, n);
-    else launch_rows_w<false, false>(st, a.CPL, grid, blk, dargs, n);
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, blk, dargs, n); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, blk, dargs, n); }
   }
   return hipGetLastError();
 }
@@ -1702,8 +1914,10 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   }
   int R = 4 / S;
   if (tune.tile_rows > 0 && tune.tile_rows < R) R = tune.tile_rows;
-  const int cJ = g.J - 2 * g.P, cK = g.K - 2 * g.P;
+  // every row and plane of the grid; z chunks do not straddle the z-slab boundaries
+  const int cJ = g.J, cK = g.K - 2 * g.P;
   const int nTy = (cJ + R - 1) / R;
+  const int nTlo = (g.P + 1 + R - 1) / R, nThi = (g.P + R - 1) / R, nTmid = (g.J - 2 * g.P - 1 + R - 1) / R;
   int kc = tune.z_chunk;
   if (kc <= 0) {
     // 8 planes per chunk (one recomputed plane below each chunk: 1/8 extra reads), fewer when
@@ -1713,13 +1927,20 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
     while (kc > 2 && layer * ((cK + kc - 1) / kc) < 16384) kc /= 2;
     if (kc > cK) kc = cK;
   }
-  const int nKc = (cK + kc - 1) / kc;
+  const int kcs = g.P <= 2 * kc ? g.P : kc;  // z-slab chunks: the whole slab when it is thin
+  const int nz1 = (g.P + kcs - 1) / kcs, nz2 = (cK + kc - 1) / kc;
+  const int nKc = 2 * nz1 + nz2;
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
   a.i0 = g.i0; a.i1 = g.i1; a.j0 = g.j0; a.j1 = g.j1; a.k0 = g.k0; a.k1 = g.k1;
   a.S = S; a.R = R; a.CPL = cpl; a.nTy = nTy;
+  a.spl = (cpl == 1 && g.I % 64 == 0 && !tune.no_split) ? 1 : 0;
   a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
+  a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
+  a.nTlo = nTlo; a.nThi = nThi; a.nTmid = nTmid;
+  a.nwg0 = nTmid * nz2;
+  a.nwg1 = (nTlo + nThi) * nKc + nTmid * 2 * nz1;
   a.xni = g.i1 - g.i0 + 1;
   a.xnj = g.j1 - g.j0 + 1;
   a.xnk = g.k1 - g.k0 + 1;
@@ -1731,7 +1952,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.xs = f.xs;
   a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
   std::memcpy(out, &a, sizeof(a));
-  return (cJ > 0 && cK > 0) ? nTy * nKc : 0;
+  return a.nwg0 + a.nwg1;
 }
 
 static SlabArgs slab_args(const FusedDev &f, int n) {
@@ -1801,10 +2022,8 @@ RowsKey rows_key(const void *hargs) {
 size_t slab_args_size() { return sizeof(SlabArgs); }
 
 void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow) {
-  const GridDev &g = f.g;
   SlabArgs sa = slab_args(f, 0);
-  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P);
-  const unsigned nb = cdiv(nslab, 256);
+  const unsigned nb = 0;  // the row kernel covers the slab cells; this launch fills the face sheets
   sa.nbs = nb;
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
@@ -1814,24 +2033,37 @@ void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *n
   *narrow = (long)nb * 256 < (1L << 31);
 }
 
-template <bool IDX, bool BUF>
+template <bool IDX, bool BUF, int PART>
 static void launch_rows_batch_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const BatchGrid *tab, int ng, int step) {
-  if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4>), grid, blk, 0, st, tab, ng, step);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2>), grid, blk, 0, st, tab, ng, step);
-  else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1>), grid, blk, 0, st, tab, ng, step);
+  if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4, PART>), grid, blk, 0, st, tab, ng, step);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2, PART>), grid, blk, 0, st, tab, ng, step);
+  else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1, PART>), grid, blk, 0, st, tab, ng, step);
 }
 
-hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, const BatchGrid *tab, int ng, int nwg, int step) {
+template <int PART>
+static void launch_rows_batch_p(hipStream_t st, const RowsKey &key, dim3 grid, dim3 blk, const BatchGrid *tab, int ng,
+                                int step) {
+  if (key.idx) {
+    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, tab, ng, step);
+    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, tab, ng, step);
+  } else {
+    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, tab, ng, step);
+    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, tab, ng, step);
+  }
+}
+
+hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
+                             int step) {
   if (nwg <= 0) return hipSuccess;
   const dim3 grid(nwg), blk(key.threads);
-  if (key.idx) {
-    if (key.buf) launch_rows_batch_w<true, true>(st, key.cpl, grid, blk, tab, ng, step);
-    else launch_rows_batch_w<true, false>(st, key.cpl, grid, blk, tab, ng, step);
-  } else {
-    if (key.buf) launch_rows_batch_w<false, true>(st, key.cpl, grid, blk, tab, ng, step);
-    else launch_rows_batch_w<false, false>(st, key.cpl, grid, blk, tab, ng, step);
-  }
+  if (part == 0) launch_rows_batch_p<0>(st, key, grid, blk, tab, ng, step);
+  else launch_rows_batch_p<1>(st, key, grid, blk, tab, ng, step);
   return hipGetLastError();
+}
+
+int rows_part_nwg(const void *hargs, int part) {
+  const StepArgs &a = *(const StepArgs *)hargs;
+  return part == 0 ? a.nwg0 : a.nwg1;
 }
 
 hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step) {
@@ -1859,21 +2091,16 @@ hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, 
   return hipGetLastError();
 }
 
-// One time step: A (slab H + x-face sheets), B (rows), C (slab E).  ev_rows (optional): two
-// events recorded around B.
+// One time step: the TF/SF face sheets of step n (when a plane wave is active), then the row
+// kernel over every cell.  ev_rows (optional): two events recorded around the row kernel.
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows) {
-  const GridDev &g = f.g;
   SlabArgs sa = slab_args(f, n);
-  const long nslab = 2L * g.P * g.I * g.J + 2L * g.P * g.I * (g.K - 2 * g.P);
-  const unsigned nb = cdiv(nslab, 256);
-  sa.nbs = nb;
+  sa.nbs = 0;
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
   const unsigned nbx = sheets ? cdiv(nsheet, 256) : 0;
-  const bool narrow = (long)nb * 256 < (1L << 31);
-  if (narrow) hipLaunchKernelGGL(k_slab_h<uint32_t>, dim3(nb + nbx), dim3(256), 0, st, sa);
-  else hipLaunchKernelGGL(k_slab_h<uint64_t>, dim3(nb + nbx), dim3(256), 0, st, sa);
+  if (nbx) hipLaunchKernelGGL(k_slab_h<uint32_t>, dim3(nbx), dim3(256), 0, st, sa);
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[0], st);
     if (e != hipSuccess) return e;
@@ -1885,13 +2112,6 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[1], st);
     if (e != hipSuccess) return e;
-  }
-  if (f.mat) {
-    if (narrow) hipLaunchKernelGGL((k_slab_e<true, uint32_t>), dim3(nb), dim3(256), 0, st, sa);
-    else hipLaunchKernelGGL((k_slab_e<true, uint64_t>), dim3(nb), dim3(256), 0, st, sa);
-  } else {
-    if (narrow) hipLaunchKernelGGL((k_slab_e<false, uint32_t>), dim3(nb), dim3(256), 0, st, sa);
-    else hipLaunchKernelGGL((k_slab_e<false, uint64_t>), dim3(nb), dim3(256), 0, st, sa);
   }
   return hipGetLastError();
 }

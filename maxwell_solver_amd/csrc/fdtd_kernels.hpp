@@ -57,6 +57,7 @@ struct StepTuning {
   int tile_rows = 0;  // rows per workgroup of the row kernel (1, 2, 4; 0: automatic)
   int z_chunk = 0;    // planes per chunk (0: automatic)
   int cpl = 0;        // columns per lane (1, 2, 4; 0: automatic)
+  bool no_split = false;  // 1 column per lane without the split edge segment
 };
 struct FusedDev {
   GridDev g;
@@ -115,7 +116,10 @@ RowsKey rows_key(const void *hargs);
 // slab cells, and the face-sheet workgroups k_slab_h adds.
 size_t slab_args_size();
 void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow);
-hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, const BatchGrid *tab, int ng, int nwg, int step);
+hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
+                             int step);
+// Workgroups of part 0 (interior rows and chunks) or part 1 (the slab rows and chunks) of the row kernel.
+int rows_part_nwg(const void *hargs, int part);
 hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
 hipError_t launch_slab_e_batch(hipStream_t st, bool idx, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
 hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, int max_ant, int step);
