@@ -775,19 +775,20 @@ __device__ __forceinline__ const AS4 T *opaque_c(const AS4 T *p) {
 //   z  E_y  j<j1 ->Hx  E_x  i<i1 ->Hy  H_y  i<i1 ->Ex   H_x  j<j1 ->Ey
 // H sheets sit on the face planes i0|i1 (j0|j1, k0|k1), E sheets on i0-1|i1 (j0-1|j1, k0-1|k1).
 // Entries outside their range are not written (and not read).
+// (32-bit index arithmetic: the sheets of a grid hold far fewer than 2^32 entries.)
 __device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, const float *rowE, const float *rowH,
-                                           float *xs, int ni, int nj, int nk, long t) {
-  const long Sx = (long)nj * nk, Sy = (long)ni * nk, Sz = (long)ni * nj;
+                                           float *xs, int ni, int nj, int nk, uint32_t t) {
+  const uint32_t Sx = (uint32_t)nj * nk, Sy = (uint32_t)ni * nk, Sz = (uint32_t)ni * nj;
   int a;
-  long r;
+  uint32_t r;
   if (t < 8 * Sx) { a = 0; r = t; }
   else if (t < 8 * (Sx + Sy)) { a = 1; r = t - 8 * Sx; }
   else { a = 2; r = t - 8 * (Sx + Sy); }
-  const long Sa = a == 0 ? Sx : (a == 1 ? Sy : Sz);
+  const uint32_t Sa = a == 0 ? Sx : (a == 1 ? Sy : Sz);
   const int q = (int)(r / Sa);
-  const long e = r - q * Sa;
+  const uint32_t e = r - (uint32_t)q * Sa;
   const int eh = q >> 2, hi = (q >> 1) & 1, u = q & 1;
-  const int nf = a == 0 ? nj : ni;
+  const uint32_t nf = a == 0 ? nj : ni;
   const int fa = (int)(e % nf), sa = (int)(e / nf);
   // bit, incident component and the axis whose range stops one short, per [eh][a][u]
   constexpr int kBit[2][3][2] = {{{1, 2}, {3, 5}, {6, 7}}, {{2, 1}, {3, 5}, {6, 7}}};
@@ -804,7 +805,7 @@ __device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, co
   const int c3[3] = {i, j, k};
   if (c3[sh] > hi0[sh] - 1) return;
   const float v = eh == 0 ? inc_e(pw, rowE, comp, i, j, k) : inc_h(pw, rowH, comp, i, j, k);
-  const long off = a == 0 ? 0 : (a == 1 ? 8 * Sx : 8 * (Sx + Sy));
+  const uint32_t off = a == 0 ? 0 : (a == 1 ? 8 * Sx : 8 * (Sx + Sy));
   xs[off + r] = v;
 }
 
@@ -1653,8 +1654,8 @@ __device__ __forceinline__ void load_slab_args(SlabArgs &dst, const void *src) {
 template <typename IT>
 __device__ __forceinline__ void slab_h_body(const SlabArgs &a, unsigned blk, int n) {
   if (blk >= a.nbs) {
-    const long t = (long)(blk - a.nbs) * 256 + threadIdx.x;
-    if (t < 8L * ((long)a.xnj * a.xnk + (long)a.xni * a.xnk + (long)a.xni * a.xnj)) {
+    const uint32_t t = (blk - a.nbs) * 256u + threadIdx.x;
+    if (t < 8u * ((uint32_t)a.xnj * a.xnk + (uint32_t)a.xni * a.xnk + (uint32_t)a.xni * a.xnj)) {
       const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
       const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
       tfsf_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xni, a.xnj, a.xnk, t);
@@ -1880,13 +1881,13 @@ static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const St
   else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, 0, st, d, n);
 }
 
-static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int nwg, int n) {
+// part: 0 the interior part, 1 the slab part, -1 both (one after the other)
+static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int part, int n) {
   // buffer addressing when every array the kernel touches (a state set, a psi set, the material
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
-  const dim3 g0(a.nwg0), g1(a.nwg1), blk(64 * a.S * a.R);
-  (void)nwg;
+  const dim3 g0(part == 1 ? 0 : a.nwg0), g1(part == 0 ? 0 : a.nwg1), blk(64 * a.S * a.R);
   if (a.mat) {
     if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, blk, dargs, n); }
     else { launch_rows_w<true, false, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, blk, dargs, n); }
@@ -2106,7 +2107,7 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
     if (e != hipSuccess) return e;
   }
   if (nwg > 0) {
-    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, nwg, n);
+    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, -1, n);
     if (e != hipSuccess) return e;
   }
   if (ev_rows) {
