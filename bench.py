@@ -3,12 +3,12 @@
 Workload (default): 256^3 FP32 grid, 10-cell CPML (m=3, kappa_max=5, alpha_max=0.05), 40 seeded
 dielectric spheres stored as a uint8 material index, oblique TF/SF plane wave at seeded angles,
 64 probes recording Ez and Hy every step, Ez snapshot every 250 steps.  A step is one full time
-step of the grid (H update, TF/SF, CPML, E update, TF/SF, CPML, probes): three launches, the
-y/z-slab H kernel, the row kernel (every other row, the step's dominant launch) and the y/z-slab
-E kernel, plus the probe gather.
+step of the grid (H update, TF/SF, CPML, E update, TF/SF, CPML, probes): the TF/SF face-sheet
+fill, the row kernel's interior part (rows P+1 .. J-P-1 of the planes P .. K-P-1, the step's
+dominant launch), its slab part (the other rows and planes, y/z CPML inline) and the probe gather.
 
 Prints one JSON line: Gcell-updates/s over exactly --steps timed steps (inputs resident in HBM),
-the row kernel's HBM roofline fraction from HIP-event times around its launches, and the CPU
+the interior part's HBM roofline fraction from HIP-event times around its launches, and the CPU
 oracle timed on a bounded sample of the same workload on this host's cores.
 
 --copy-calibration runs a plain device copy of known size (the scale of the HBM byte counters).
@@ -37,11 +37,12 @@ def bytes_per_cell(sc, material_bytes=1):
 
 
 def row_kernel_bytes(sc, material_bytes=1):
-    """Compulsory HBM bytes of one row-kernel launch: the rows j in [P, J-P) of the planes k in
-    [P, K-P), every column: 6 components read + written and the material byte per cell, plus the
-    x-slab psi (Hyx, Hzx, Eyx, Ezx read + written) in the 2P x-slab columns of each row."""
+    """Compulsory HBM bytes of one launch of the row kernel's interior part: the rows j in
+    [P+1, J-P) of the planes k in [P, K-P), every column: 6 components read + written and the
+    material byte per cell, plus the x-slab psi (Hyx, Hzx, Eyx, Ezx read + written) in the 2P
+    x-slab columns of each row."""
     P = sc.P
-    rows = (sc.J - 2 * P) * (sc.K - 2 * P)
+    rows = (sc.J - 2 * P - 1) * (sc.K - 2 * P)
     return rows * (sc.I * (48.0 + material_bytes) + 2 * P * 32.0)
 
 
@@ -365,7 +366,7 @@ def main():
     achieved = rows_bytes / (rows_ms * 1e-3) / 1e9 if rows_ms > 0 else 0.0
     step_achieved = bpc * sc.cells / (step_ms * 1e-3) / 1e9 if step_ms > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_k_rows_{sc.I}.json")  # rocprofv3 FETCH/WRITE, copy-calibrated
+    tpath = os.path.join(ROOT, "profiles", f"traffic_k_rows_c1_{sc.I}.json")  # rocprofv3 FETCH/WRITE, copy-calibrated
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")
@@ -390,11 +391,11 @@ def main():
             "grid": [sc.I, sc.J, sc.K],
             "cells": sc.cells,
             "time_steps": args.steps,
-            "kernels": "k_slab_h (y/z-slab H + TF/SF x-face sheets), k_rows (all other rows), k_slab_e (y/z-slab E), "
-                       "k_probes",
+            "kernels": "k_sheets (TF/SF face sheets), k_rows_c1 (row kernel, interior part), k_rows<..., 1> "
+                       "(row kernel, slab part: y/z CPML rows and planes), k_probes",
         },
         "roofline": {
-            "kernel": "k_rows",
+            "kernel": "k_rows_c1 (row kernel, interior part)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

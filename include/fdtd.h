@@ -103,8 +103,9 @@ int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *tabl
 /* Probe components (1..6), recorded after every step; default {3} (Ez). */
 int fdtd_set_probe_components(fdtd_sim *s, const int32_t *comps, int32_t n);
 /* One full time step n (H update, plane wave, CPML, E update, plane wave, CPML, probes) as the
- * fused step (three kernel launches over a ping-ponged state: y/z-slab H, every other row, y/z-slab
- * E), or the per-op kernels for grids outside its layouts; mode 0 = no plane wave. */
+ * fused step (over a ping-ponged state: the TF/SF face sheets, then H and E of every cell in one
+ * pass of the row kernel, interior and CPML-slab parts), or the per-op kernels for grids outside
+ * its layouts; mode 0 = no plane wave. */
 int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode);
 /* Steps n0 .. n0+count-1 (fused).  If snap_every > 0, component snap_comp (1..6) is copied to a
  * device snapshot buffer after every snap_every-th step (counted since ResetFieldAll). */
@@ -128,7 +129,8 @@ int fdtd_prepare(fdtd_sim *s);
  * 19 material id per cell; 20 + 4*axis + {0,1,2,3} oblique tables (E integer, E half, H integer,
  * H half); 32 polarisation [eE x,y,z, eH x,y,z].  *count receives the element count. */
 int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
-/* Launch tuning of the fused step's row kernel: rows per workgroup (1, 2 or 4; 0 = automatic),
+/* Launch tuning of the fused step's row kernel: rows per workgroup (1, 2, 4, 8 or 16, at most 1024
+ * threads; rows of a workgroup exchange their neighbour values through LDS; 0 = automatic),
  * z-chunk depth in planes (0 = automatic) and columns per lane (1, 2 or 4; 0 = automatic; the
  * grid's I must be a multiple of it and a row at most 256 of it).  With 1 column per lane and I a
  * multiple of 64 the row's two ends share one wave (split edge segment); -1 = 1 column per lane
@@ -194,8 +196,8 @@ int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *be
  * (HIP events around each launch on the simulation's stream). */
 int fdtd_set_timing(fdtd_sim *s, int32_t on);
 int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches);
-/* Average device time of the row kernel alone (the rows j in [P, J-P) of the planes k in
- * [P, K-P), the step's dominant launch) over the same run. */
+/* Average device time of the row kernel's interior part alone (the rows j in [P+1, J-P) of the
+ * planes k in [P, K-P), the step's dominant launch) over the same run. */
 int fdtd_row_kernel_time(fdtd_sim *s, double *avg_ms);
 
 #ifdef __cplusplus

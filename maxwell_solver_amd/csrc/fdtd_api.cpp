@@ -645,8 +645,8 @@ struct fdtd_sim {
 
 // ------------------------------------------------------------------------------------------
 // Batched engine: many independent grids advanced together.  Every batch step launches each
-// kernel of the fused step once for all the grids still running (slab H + face sheets, the row
-// kernel per row-kernel variant, slab E per material mode, probes), so small grids fill the
+// kernel of the fused step once for all the grids still running (face sheets, the row kernel's
+// interior and slab parts per row-kernel variant, probes), so small grids fill the
 // chip together and the launch count per step does not grow with the number of grids.  Grids
 // whose layout the fused step does not cover run on their own afterwards.
 // ------------------------------------------------------------------------------------------
@@ -907,7 +907,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       const int step = t - ep.t0;
       hipStream_t st = b->streams[q];
       for (const auto &L : ep.launches)
-        if (L.kind == 0) HIPCHK(launch_slab_h_batch(st, L.narrow, dt + L.tab, L.ng, L.nwg, step));
+        if (L.kind == 0) HIPCHK(launch_sheets_batch(st, dt + L.tab, L.ng, L.nwg, step));
       for (const auto &L : ep.launches)
         if (L.kind >= 1) HIPCHK(launch_rows_batch(st, L.key, L.kind - 1, dt + L.tab, L.ng, L.nwg, step));
       HIPCHK(launch_probes_batch(st, dj + ep.probes, ep.nprobe, ep.max_ant, step));
@@ -1621,8 +1621,9 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
 
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane) {
   NEED(s);
-  if (rows_per_group != 0 && rows_per_group != 1 && rows_per_group != 2 && rows_per_group != 4)
-    return fail(FDTD_ERR_INVALID, "rows_per_group must be 0, 1, 2 or 4");
+  if (rows_per_group != 0 && rows_per_group != 1 && rows_per_group != 2 && rows_per_group != 4 && rows_per_group != 8 &&
+      rows_per_group != 16)
+    return fail(FDTD_ERR_INVALID, "rows_per_group must be 0, 1, 2, 4, 8 or 16");
   if (z_chunk < 0) return fail(FDTD_ERR_INVALID, "z_chunk must be >= 0");
   if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4 && cols_per_lane != -1)
     return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2, 4 or -1");

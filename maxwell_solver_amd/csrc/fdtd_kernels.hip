@@ -3,11 +3,11 @@
 // Two families:
 //  * per-op kernels, one per op of the reference MEX gateway, updating the state in place with
 //    the reference's masks and operation order (src/gpu.h);
-//  * the fused time step: three launches per step over a ping-ponged state (read the current
-//    set, write the next): k_slab_h (H of the y/z CPML slabs, plus the TF/SF face sheets of the
-//    step), k_rows (H and E of every other row: one wave per aligned row segment marching up a
-//    chunk of planes, the x CPML slabs and the TF/SF corrections inline), k_slab_e (E of the y/z
-//    slabs).  Each field is read once and written once per step outside the recomputed halo.
+//  * the fused time step over a ping-ponged state (read the current set, write the next):
+//    k_sheets (the TF/SF face sheets of the step), then the row kernel, H and E of every cell in
+//    one pass: one wave per aligned row segment marching up a chunk of planes, all CPML slabs and
+//    TF/SF corrections inline, in two launches (the interior rows and planes; the slab rows and
+//    planes).  Each field is read once and written once per step outside the recomputed halo.
 //
 // All arithmetic is plain IEEE float with no contraction (-ffp-contract=off); expressions keep the
 // reference's association order so results equal the CPU oracle bit for bit.
@@ -126,23 +126,6 @@ __device__ __forceinline__ void pw_e_face(const GridDev &g, const float ch[3], c
   }
 }
 
-// All TF/SF corrections of one H cell, faces in axis order x, y, z.
-__device__ __forceinline__ void pw_h_cell(const GridDev &g, float Che, const PWDev &pw, const float *row, int i, int j,
-                                          int k, float &hx, float &hy, float &hz) {
-  if (i < g.i0 - 1 || i > g.i1 || j < g.j0 - 1 || j > g.j1 || k < g.k0 - 1 || k > g.k1) return;
-  if (i == g.i0 - 1 || i == g.i1) pw_h_face<0>(g, Che, pw, row, i, j, k, i == g.i0 - 1, hx, hy, hz);
-  if (j == g.j0 - 1 || j == g.j1) pw_h_face<1>(g, Che, pw, row, i, j, k, j == g.j0 - 1, hx, hy, hz);
-  if (k == g.k0 - 1 || k == g.k1) pw_h_face<2>(g, Che, pw, row, i, j, k, k == g.k0 - 1, hx, hy, hz);
-}
-
-__device__ __forceinline__ void pw_e_cell(const GridDev &g, const float ch[3], const PWDev &pw, const float *row, int i,
-                                          int j, int k, float &ex, float &ey, float &ez) {
-  if (i < g.i0 || i > g.i1 || j < g.j0 || j > g.j1 || k < g.k0 || k > g.k1) return;
-  if (i == g.i0 || i == g.i1) pw_e_face<0>(g, ch, pw, row, i, j, k, i == g.i0, ex, ey, ez);
-  if (j == g.j0 || j == g.j1) pw_e_face<1>(g, ch, pw, row, i, j, k, j == g.j0, ex, ey, ez);
-  if (k == g.k0 || k == g.k1) pw_e_face<2>(g, ch, pw, row, i, j, k, k == g.k0, ex, ey, ez);
-}
-
 // Material coefficients of cell id (column i).  Index mode: table row of the cell's material,
 // with Ch = 0 on the last x plane (den_ex = 0 there, src/simulation.h:887-912).
 template <bool IDX>
@@ -158,146 +141,6 @@ __device__ __forceinline__ void cell_coef(const CoefDev &co, const float *tab, s
   } else {
     ce[0] = co.Ce[0][id]; ce[1] = co.Ce[1][id]; ce[2] = co.Ce[2][id];
     ch[0] = co.Ch[0][id]; ch[1] = co.Ch[1][id]; ch[2] = co.Ch[2][id];
-  }
-}
-
-// ------------------------------------------------------------------------------------------
-// H-cell and E-cell updates shared by the fused kernel (main update, TF/SF, CPML in the
-// reference's order).  Neighbour values are passed in; psi is read from pc and, when `own`, the
-// new psi (or an unchanged copy for slab cells outside the update masks) is written to pn.
-// ------------------------------------------------------------------------------------------
-struct HNb {
-  float ex, ey, ez;         // E(i,j,k)
-  float ex_j1, ez_j1;       // E(i,j+1,k)
-  float ey_i1, ez_i1;       // E(i+1,j,k)
-  float ex_k1, ey_k1;       // E(i,j,k+1)
-};
-struct ENb {
-  float hx, hy, hz;         // H(i,j,k) at n+1/2
-  float hx_jm, hz_jm;       // H(i,j-1,k)
-  float hy_im, hz_im;       // H(i-1,j,k)
-  float hx_km, hy_km;       // H(i,j,k-1)
-};
-
-__device__ __forceinline__ void h_cell(const GridDev &g, const CoefDev &co, const PWDev &pw, const float *rowE,
-                                       const PsiSet &pc, const PsiSet &pn, bool own, int i, int j, int k, size_t IJ,
-                                       const HNb &e, float &hx, float &hy, float &hz) {
-  const int I = g.I, J = g.J, K = g.K, P = g.P;
-  const float Che = co.Che, dx = co.dx;
-  // K1 GPU_updateH, src/gpu.h:592-601
-  if (i < I - 1 && j < J - 1 && k < K - 1) {
-    const float kx = co.ikH[0][i], ky = co.ikH[1][j], kz = co.ikH[2][k];
-    hx = co.Chh * hx + Che * (-(e.ez_j1 - e.ez) * ky + (e.ey_k1 - e.ey) * kz);
-    hy = co.Chh * hy + Che * (-(e.ex_k1 - e.ex) * kz + (e.ez_i1 - e.ez) * kx);
-    hz = co.Chh * hz + Che * (-(e.ey_i1 - e.ey) * kx + (e.ex_j1 - e.ex) * ky);
-  }
-  // updateH_PlaneWave (src/simulation.h:330-349)
-  if (pw.active_h) pw_h_cell(g, Che, pw, rowE, i, j, k, hx, hy, hz);
-  // updateH_PML: Y (K5), X (K4), Z (K6), src/simulation.h:392-403
-  if (j < P || j >= J - P) {
-    const int s = j < P ? j : j - (J - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
-    if (s < 2 * P - 1 && i < I - 1 && k < K - 1) {
-      const float pxy = co.bH[s] * pc.p[P_HXY][t] + co.cH[s] * (e.ez_j1 - e.ez) / dx;
-      hx = hx - Che * pxy * dx;
-      const float pzy = co.bH[s] * pc.p[P_HZY][t] + co.cH[s] * (e.ex_j1 - e.ex) / dx;
-      hz = hz + Che * pzy * dx;
-      if (own) { pn.p[P_HXY][t] = pxy; pn.p[P_HZY][t] = pzy; }
-    } else if (own) {
-      pn.p[P_HXY][t] = pc.p[P_HXY][t];
-      pn.p[P_HZY][t] = pc.p[P_HZY][t];
-    }
-  }
-  if (i < P || i >= I - P) {
-    const int s = i < P ? i : i - (I - 2 * P);
-    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
-    if (s < 2 * P - 1 && j < J - 1 && k < K - 1) {
-      const float pyx = co.bH[s] * pc.p[P_HYX][t] + co.cH[s] * (e.ez_i1 - e.ez) / dx;
-      hy = hy + Che * pyx * dx;
-      const float pzx = co.bH[s] * pc.p[P_HZX][t] + co.cH[s] * (e.ey_i1 - e.ey) / dx;
-      hz = hz - Che * pzx * dx;
-      if (own) { pn.p[P_HYX][t] = pyx; pn.p[P_HZX][t] = pzx; }
-    } else if (own) {
-      pn.p[P_HYX][t] = pc.p[P_HYX][t];
-      pn.p[P_HZX][t] = pc.p[P_HZX][t];
-    }
-  }
-  if (k < P || k >= K - P) {
-    const int s = k < P ? k : k - (K - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
-    if (s < 2 * P - 1 && i < I - 1 && j < J - 1) {
-      const float pxz = co.bH[s] * pc.p[P_HXZ][t] + co.cH[s] * (e.ey_k1 - e.ey) / dx;
-      hx = hx + Che * pxz * dx;
-      const float pyz = co.bH[s] * pc.p[P_HYZ][t] + co.cH[s] * (e.ex_k1 - e.ex) / dx;
-      hy = hy - Che * pyz * dx;
-      if (own) { pn.p[P_HXZ][t] = pxz; pn.p[P_HYZ][t] = pyz; }
-    } else if (own) {
-      pn.p[P_HXZ][t] = pc.p[P_HXZ][t];
-      pn.p[P_HYZ][t] = pc.p[P_HYZ][t];
-    }
-  }
-}
-
-__device__ __forceinline__ void e_cell(const GridDev &g, const CoefDev &co, const PWDev &pw, const float *rowH,
-                                       const PsiSet &pc, const PsiSet &pn, int i, int j, int k, size_t IJ,
-                                       const float ce[3], const float ch[3], const ENb &h, float &ex, float &ey,
-                                       float &ez) {
-  const int I = g.I, J = g.J, K = g.K, P = g.P;
-  const float dx = co.dx;
-  // K3 GPU_updateE, src/gpu.h:745-762
-  {
-    const float kx = co.ikE[0][i], ky = co.ikE[1][j], kz = co.ikE[2][k];
-    if (j != 0 && k != 0) ex = ce[0] * ex + ch[0] * ((h.hz - h.hz_jm) * ky - (h.hy - h.hy_km) * kz);
-    if (i != 0 && k != 0) ey = ce[1] * ey + ch[1] * ((h.hx - h.hx_km) * kz - (h.hz - h.hz_im) * kx);
-    if (i != 0 && j != 0) ez = ce[2] * ez + ch[2] * ((h.hy - h.hy_im) * kx - (h.hx - h.hx_jm) * ky);
-  }
-  // updateE_PlaneWave (src/simulation.h:350-369)
-  if (pw.active_e) pw_e_cell(g, ch, pw, rowH, i, j, k, ex, ey, ez);
-  // updateE_PML: Y (K8), Z (K9), X (K7), src/simulation.h:404-418
-  if (j < P || j >= J - P) {
-    const int s = j < P ? j : j - (J - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
-    if (s >= 1 && i >= 1 && i < I - 1 && k >= 1 && k < K - 1) {
-      const float pxy = co.bE[s] * pc.p[P_EXY][t] + co.cE[s] * (h.hz - h.hz_jm) / dx;
-      ex = ex + ch[0] * pxy * dx;
-      const float pzy = co.bE[s] * pc.p[P_EZY][t] + co.cE[s] * (h.hx - h.hx_jm) / dx;
-      ez = ez - ch[2] * pzy * dx;
-      pn.p[P_EXY][t] = pxy;
-      pn.p[P_EZY][t] = pzy;
-    } else {
-      pn.p[P_EXY][t] = pc.p[P_EXY][t];
-      pn.p[P_EZY][t] = pc.p[P_EZY][t];
-    }
-  }
-  if (k < P || k >= K - P) {
-    const int s = k < P ? k : k - (K - 2 * P);
-    const size_t t = (size_t)i + (size_t)I * j + IJ * s;
-    if (s >= 1 && i >= 1 && i < I - 1 && j >= 1 && j < J - 1) {
-      const float pxz = co.bE[s] * pc.p[P_EXZ][t] + co.cE[s] * (h.hy - h.hy_km) / dx;
-      ex = ex - ch[0] * pxz * dx;
-      const float pyz = co.bE[s] * pc.p[P_EYZ][t] + co.cE[s] * (h.hx - h.hx_km) / dx;
-      ey = ey + ch[1] * pyz * dx;
-      pn.p[P_EXZ][t] = pxz;
-      pn.p[P_EYZ][t] = pyz;
-    } else {
-      pn.p[P_EXZ][t] = pc.p[P_EXZ][t];
-      pn.p[P_EYZ][t] = pc.p[P_EYZ][t];
-    }
-  }
-  if (i < P || i >= I - P) {
-    const int s = i < P ? i : i - (I - 2 * P);
-    const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
-    if (s >= 1 && j >= 1 && j < J - 1 && k >= 1 && k < K - 1) {
-      const float pyx = co.bE[s] * pc.p[P_EYX][t] + co.cE[s] * (h.hz - h.hz_im) / dx;
-      ey = ey - ch[1] * pyx * dx;
-      const float pzx = co.bE[s] * pc.p[P_EZX][t] + co.cE[s] * (h.hy - h.hy_im) / dx;
-      ez = ez + ch[2] * pzx * dx;
-      pn.p[P_EYX][t] = pyx;
-      pn.p[P_EZX][t] = pzx;
-    } else {
-      pn.p[P_EYX][t] = pc.p[P_EYX][t];
-      pn.p[P_EZX][t] = pc.p[P_EZX][t];
-    }
   }
 }
 
@@ -556,25 +399,23 @@ __global__ void k_probes(FieldSet F, const int *idx, int nAnt, const int *comps,
 }
 
 // ------------------------------------------------------------------------------------------
-// fused time step: three launches per step, reading the cur state and writing the nxt state
+// fused time step, reading the cur state and writing the nxt state
 //
-//  A  k_slab_h  one thread per H cell of the y and z CPML slabs (rows j < P or j >= J-P of every
-//               plane, and the planes k < P or k >= K-P): main update, TF/SF, CPML Y-X-Z with psi,
-//               in the reference order.  Further workgroups of the same launch fill the x-face
-//               sheets of the TF/SF box for this time step.
-//  B  k_rows    every other row: j in [P, J-P) of the planes k in [P, K-P), all columns i (the
-//               x CPML slabs included).  One wave per aligned row segment of 64 CPL columns, CPL
-//               consecutive columns per lane, marching k up a chunk of planes; fused H and E.
-//  C  k_slab_e  one thread per E cell of the y and z slabs: main update, TF/SF, CPML Y-Z-X with
-//               psi, reading the new H of A and B.
-// All three apply per cell exactly the operations of the per-op kernels, in the same order.
+//  k_sheets   the incident values the six faces of the TF/SF box need at step n (tfsf_sheet).
+//  k_rows     every cell: one wave per aligned row segment of 64 CPL columns (CPL consecutive
+//             columns per lane), marching k up a chunk of planes; fused H and E with the TF/SF
+//             corrections and the CPML of all three axes inline.  Part 0 (k_rows_c1 for one
+//             column per lane): the rows j in [P+1, J-P) of the planes k in [P, K-P); part 1: the
+//             other rows and planes (y-slab rows and the row above the low slab, z-slab planes).
+// Per cell both apply exactly the operations of the per-op kernels, in the same order.
 // ------------------------------------------------------------------------------------------
 struct StepArgs {
   int I, J, K, P;
   int i0, i1, j0, j1, k0, k1;
   int S, R, CPL, nTy;  // row segments per row, rows per workgroup, columns per lane, row tiles
   int KC, ntab, act_h, act_e;
-  int KCS, nz1, nz2, pad1;  // z chunks: nz1 of KCS planes over [0, P), nz2 of KC over [P, K-P), nz1 over [K-P, K)
+  int KCS, nz1, nz2;  // z chunks: nz1 of KCS planes over [0, P), nz2 of KC over [P, K-P), nz1 over [K-P, K)
+  int Wrow;           // LDS row stride of the row exchange: 64 CPL S columns
   int nTlo, nThi, nTmid;    // row tiles of the bands [0, P], [J-P, J) and [P+1, J-P)
   int nwg0, nwg1;           // workgroups of the two parts (host information)
   int xni, xnj, xnk;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
@@ -844,6 +685,9 @@ struct RowPlane {
   float qHxz[CPL], qHyz[CPL], qExz[CPL], qEyz[CPL], qHxzm[CPL];              // z-slab psi
 };
 
+// Workgroup size bound of the row kernel per columns per lane: wider lanes hold more registers.
+constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
+
 template <bool IDX, bool BUF, int CPL, int PART>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
@@ -862,7 +706,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
   __shared__ float sCoef[IDX ? 256 * 6 : 6];
   __shared__ float sSlab[6][128];   // per x-slab index s: bE cE bH cH, 1/kappa_x of E and of H
-  __shared__ float sX[2][4][5];     // per plane parity and wave: E (lane 0) ey ez eym, H (lane 63) hy hz
+  __shared__ float sX[16][5];       // per wave: E ey ez eym (lane e_pub), new H hy hz (lane h_pub)
+  extern __shared__ float sRow[];   // per row of the tile: Ex, Ez (plane k, old), Hx, Hz (new), Wrow each
   {
     const auto prm = gptr(pa->prm);
     if (IDX) {
@@ -918,7 +763,13 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     kb = uni(kb); ke = uni(ke);
   }
   const bool row_ok = j < row_end;
-  if (S == 1 && !row_ok) return;  // whole wave, and no barrier follows when S == 1
+  if (S == 1 && R == 1 && !row_ok) return;  // a one-wave workgroup: no barrier follows
+  // Rows of a tile exchange through LDS: row j reads the old Ex, Ez of row j+1 and the new Hx, Hz of
+  // row j-1 there.  The tile's first row recomputes Hx, Hz of the row below it from the old state,
+  // and its last row (or the band's) reads row j+1 from memory.
+  const bool firstr = uni(r) == 0, lastq = uni((int)(r == R - 1 || j + 1 >= row_end)) != 0;
+  const bool xsync = S > 1 || R > 1;
+  const int Wrow = pa->Wrow;
   // Columns of the lane.  Plain layout: segment g holds the columns [64 CPL g, 64 CPL (g+1)).
   // Split layout (1 column per lane, I a multiple of 64): segment 0 holds both ends of the row,
   // lanes 0-31 the columns [0, 32) and lanes 32-63 the columns [I-32, I), and segment g >= 1 the
@@ -1044,15 +895,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       Fc.ld(o0 + r1, 0, ex); Fc.ld(o0 + r1, c1, ey); Fc.ld(o0 + r1, c2, ez);
       Fc.ld(o0, c1, eym);
     }
-    // E at the first column of the next segment (plane k), for the lane's last column
-    float bey = 0, bez = 0, beym = 0;
-    int par = 0;
-    if (S > 1) {
-      if (lane == e_pub) { sX[0][wv][0] = ey[0]; sX[0][wv][1] = ez[0]; sX[0][wv][2] = eym[0]; }
-      lds_barrier();
-      if (lane == e_rd && has_next) { bey = sX[0][w_next][0]; bez = sX[0][w_next][1]; beym = sX[0][w_next][2]; }
-      par = 1;
-    }
+    float *const sRq = sRow + (size_t)(4 * r) * Wrow + (on ? ib : 0);  // this row's LDS slots, column ib
 
     // ---- loads of plane index it ----
     auto load = [&](RowPlane<CPL> &Lp, int it, auto own_tag) {
@@ -1068,12 +911,14 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4;
       if (!EDGE || bit(f, F_ON)) {
         Fc.ld(o1, pn1, Lp.nx); Fc.ld(o1, c1 + pn1, Lp.ny); Fc.ld(o1, c2 + pn1, Lp.nz);
-        Fc.ld(o, c1 + pn1, Lp.nym);
-        Fc.ld(o, 0, Lp.exm); Fc.ld(o, c2, Lp.ezm);
-        Fc.ld(o2, 0, Lp.exq); Fc.ld(o2, c2, Lp.ezq);
         Fc.ld(o1, c3, Lp.hx); Fc.ld(o1, c4, Lp.hy); Fc.ld(o1, c5, Lp.hz);
-        Fc.ld(o, c3, Lp.hxm);
-        Fc.ld(o, c5, Lp.hzm);
+        if (firstr) {  // row j-1, recomputed by the tile's first row
+          Fc.ld(o, c1 + pn1, Lp.nym);
+          Fc.ld(o, 0, Lp.exm); Fc.ld(o, c2, Lp.ezm);
+          Fc.ld(o, c3, Lp.hxm);
+          Fc.ld(o, c5, Lp.hzm);
+        }
+        if (lastq) { Fc.ld(o2, 0, Lp.exq); Fc.ld(o2, c2, Lp.ezq); }
         if (IDX && own) Mt.ldm(o1 >> 2, 0, Lp.mt);
         // y-slab psi of rows j and j-1
         if (SLAB && ys) {
@@ -1081,7 +926,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           Pc.ld(oc, gHxy + u, Lp.qHxy); Pc.ld(oc, gHzy + u, Lp.qHzy);
           if (own) { Pc.ld(oc, gExy + u, Lp.qExy); Pc.ld(oc, gEzy + u, Lp.qEzy); }
         }
-        if (SLAB && own && ysm) {
+        if (SLAB && own && ysm && firstr) {
           const O u = uni((O)((size_t)I * ((size_t)sym + (size_t)P2 * k) * 4));
           Pc.ld(oc, gHxy + u, Lp.qHxym); Pc.ld(oc, gHzy + u, Lp.qHzym);
         }
@@ -1092,7 +937,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           Pc.ld(oc, gHxz + u, Lp.qHxz); Pc.ld(oc, gHyz + u, Lp.qHyz);
           if (own) {
             Pc.ld(oc, gExz + u, Lp.qExz); Pc.ld(oc, gEyz + u, Lp.qEyz);
-            if (has_jm) Pc.ld(oc, gHxz + u - (O)(I * 4), Lp.qHxzm);
+            if (has_jm && firstr) Pc.ld(oc, gHxz + u - (O)(I * 4), Lp.qHxzm);
           }
         }
         if (EDGE && bit(f, F_SLAB)) {
@@ -1102,7 +947,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             if (slab_col(f, c)) {
               Lp.pHy[c] = Pc.ld1(so[c], gHy + pq);
               Lp.pHz[c] = Pc.ld1(so[c], gHz + pq);
-              if (has_jm) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
+              if (has_jm && firstr) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
               Lp.pEy[c] = Pc.ld1(so[c], pq);
               Lp.pEz[c] = Pc.ld1(so[c], gEz + pq);
             }
@@ -1116,7 +961,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         if (bit(f, F_XHL)) {
           if ((ah & 2) && xj0 && xk0) Lp.vhy = ldo(xs, (xh_sel + xo) * 4u);
           if ((ah & 4) && xj1 && xk1) Lp.vhz = ldo(xs + xS, (xh_sel + xo) * 4u);
-          if (has_jm && (ah & 4) && xjm1 && xk1) Lp.vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
+          if (has_jm && firstr && (ah & 4) && xjm1 && xk1) Lp.vhzm = ldo(xs + xS, (xh_sel + xo - 1u) * 4u);
         }
         if (bit(f, F_XEL)) {
           if ((ae & 4) && xj0 && xk0) Lp.vez = ldo(xs, (xe_sel + xo) * 4u);
@@ -1137,9 +982,22 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4, zrow = jf & 8, zrow_m = jf & 16;
       const bool lon = !EDGE || bit(f, F_ON);
 
-      // E at i+1 of the lane's last column
+      // exchange 1 (old E of plane k): Ex, Ez of each row for the row below it, and Ey, Ez (and Ey
+      // of row j-1) at the first column of each segment for the segment on its left
       float ey_n = __shfl_down(ey[0], 1), ez_n = __shfl_down(ez[0], 1), eym_n = __shfl_down(eym[0], 1);
-      if (S > 1 && lane == e_rd) { ey_n = bey; ez_n = bez; eym_n = beym; }
+      if (xsync) {
+        if (R > 1 && lon) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) { sRq[c] = ex[c]; sRq[Wrow + c] = ez[c]; }
+        }
+        if (S > 1 && lane == e_pub) { sX[wv][0] = ey[0]; sX[wv][1] = ez[0]; sX[wv][2] = eym[0]; }
+        lds_barrier();
+        if (!lastq && lon) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) { Lp.exq[c] = sRq[4 * Wrow + c]; Lp.ezq[c] = sRq[5 * Wrow + c]; }
+        }
+        if (S > 1 && lane == e_rd && has_next) { ey_n = sX[w_next][0]; ez_n = sX[w_next][1]; eym_n = sX[w_next][2]; }
+      }
 
       // z-slab plane: slab index, profile and 1/kappa_z (the plane below an interior chunk may be
       // a slab plane too); off the slabs 1/kappa_z is exactly 1 and its factor is omitted
@@ -1264,7 +1122,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       // Hx, Hz at (i, j-1, k), recomputed the same way (not needed on the plane below the chunk)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) { hxm[c] = Lp.hxm[c]; hzm[c] = Lp.hzm[c]; }
-      if (own && has_jm) {
+      if (own && has_jm && firstr) {
         const int jm = j - 1;
         const bool hm_k = !SLAB || k < K - 1;
 #pragma unroll
@@ -1348,15 +1206,22 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         }
       }
 
-      // H at i-1 of the lane's first column; segment exchange
+      // exchange 2 (new H of plane k): Hx, Hz of each row for the row above it, and Hy, Hz at the last
+      // column of each segment for the segment on its right (the plane below the chunk only
+      // synchronises, so that its reads of exchange 1 end before the next plane's writes)
       float hy_p = __shfl_up(hy[CPL - 1], 1), hz_p = __shfl_up(hz[CPL - 1], 1);
-      if (S > 1) {
-        if (lane == h_pub) { sX[par][wv][3] = hy[CPL - 1]; sX[par][wv][4] = hz[CPL - 1]; }
-        if (lane == e_pub) { sX[par][wv][0] = Lp.ny[0]; sX[par][wv][1] = Lp.nz[0]; sX[par][wv][2] = Lp.nym[0]; }
+      if (xsync) {
+        if (own && R > 1 && lon) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) { sRq[2 * Wrow + c] = hx[c]; sRq[3 * Wrow + c] = hz[c]; }
+        }
+        if (own && S > 1 && lane == h_pub) { sX[wv][3] = hy[CPL - 1]; sX[wv][4] = hz[CPL - 1]; }
         lds_barrier();
-        if (lane == h_rd && has_prev) { hy_p = sX[par][w_prev][3]; hz_p = sX[par][w_prev][4]; }
-        if (lane == e_rd && has_next) { bey = sX[par][w_next][0]; bez = sX[par][w_next][1]; beym = sX[par][w_next][2]; }
-        par ^= 1;
+        if (own && !firstr && lon) {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) { hxm[c] = sRq[-2 * Wrow + c]; hzm[c] = sRq[-Wrow + c]; }
+        }
+        if (own && S > 1 && lane == h_rd && has_prev) { hy_p = sX[w_prev][3]; hz_p = sX[w_prev][4]; }
       }
 
       if (own) {
@@ -1578,8 +1443,15 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 }
 
 template <bool IDX, bool BUF, int CPL, int PART>
-__global__ void __launch_bounds__(256) k_rows(const StepArgs *__restrict__ pa, int n) {
+__global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n) {
   rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(blockIdx.x, gridDim.x));
+}
+// The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
+// (measured 4% faster at 256^3 than the 68 the compiler picks freely).
+template <bool IDX, bool BUF>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_rows_c1(const StepArgs *__restrict__ pa, int n) {
+  rows_body<IDX, BUF, 1, 0>(pa, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Batched launches cover the workgroups of many grids: tab[g].off is grid g's first workgroup
@@ -1598,7 +1470,7 @@ __device__ __forceinline__ int find_grid(const BatchGrid *__restrict__ tab, int 
 // Blocks of one grid whose local index agrees mod 8 run on the same XCD (blocks are dealt to the
 // XCDs round-robin), so the per-grid remap is xcd_remap of the local index.
 template <bool IDX, bool BUF, int CPL, int PART>
-__global__ void __launch_bounds__(256) k_rows_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
+__global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
   const int b = blockIdx.x, g = find_grid(tab, ng, b);
   const int par = (tab[g].cur0 + step) & 1;
   const int off = tab[g].off, nb = tab[g + 1].off - off;
@@ -1606,31 +1478,7 @@ __global__ void __launch_bounds__(256) k_rows_batch(const BatchGrid *__restrict_
 }
 
 
-// Decode thread t of the slab kernels into a cell of the y and z slabs (z slabs, then the y slabs
-// of the remaining planes).  32-bit arithmetic when the slab cell count allows it.
-template <typename IT>
-__device__ __forceinline__ bool slab_cell(IT t, int I, int J, int K, int P, int &i, int &j, int &k) {
-  const IT IJ = (IT)I * J;
-  const IT nz = (IT)2 * P * IJ, ny = (IT)2 * P * I * (K - 2 * P);
-  if (t < nz) {
-    const IT pl = t / IJ, rem = t - pl * IJ;
-    k = pl < (IT)P ? (int)pl : K - 2 * P + (int)pl;
-    j = (int)(rem / I);
-    i = (int)(rem - (IT)j * I);
-    return true;
-  }
-  t -= nz;
-  if (t < ny) {
-    const IT q = t / I;
-    i = (int)(t - q * I);
-    const int js = (int)(q % (2 * P));
-    k = P + (int)(q / (2 * P));
-    j = js < P ? js : J - 2 * P + js;
-    return true;
-  }
-  return false;
-}
-
+// Argument block of the TF/SF face-sheet fill (k_sheets), one per (state parity, plane-wave mode).
 struct SlabArgs {
   GridDev g;
   FieldSet cur, nxt;
@@ -1638,109 +1486,38 @@ struct SlabArgs {
   CoefDev co;
   PWDev pw;
   int n;
-  unsigned nbs;  // workgroups of the slab cells; k_slab_h's further workgroups fill the face sheets
+  unsigned nbs;  // workgroups before the sheet fill (none: the row kernel covers every cell)
   float *xs;
   int xni, xnj, xnk;
 };
 
-// The argument block of a batched slab workgroup, copied to LDS once (it is read field by field).
+// The argument block of a batched workgroup, copied to LDS once (it is read field by field).
 __device__ __forceinline__ void load_slab_args(SlabArgs &dst, const void *src) {
   const uint32_t *s = (const uint32_t *)src;
   for (int q = threadIdx.x; q < (int)(sizeof(SlabArgs) / 4); q += blockDim.x) ((uint32_t *)&dst)[q] = s[q];
   __syncthreads();
 }
 
-// A: H cells of the y and z CPML slabs (src/gpu.h:575-602 + plane wave + src/gpu.h:604-725)
-template <typename IT>
-__device__ __forceinline__ void slab_h_body(const SlabArgs &a, unsigned blk, int n) {
-  if (blk >= a.nbs) {
-    const uint32_t t = (blk - a.nbs) * 256u + threadIdx.x;
-    if (t < 8u * ((uint32_t)a.xnj * a.xnk + (uint32_t)a.xni * a.xnk + (uint32_t)a.xni * a.xnj)) {
-      const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
-      const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
-      tfsf_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xni, a.xnj, a.xnk, t);
-    }
-    return;
+// The TF/SF face sheets of time step n (one thread per sheet entry).
+__device__ __forceinline__ void sheets_body(const SlabArgs &a, unsigned blk, int n) {
+  if (blk < a.nbs) return;
+  const uint32_t t = (blk - a.nbs) * 256u + threadIdx.x;
+  if (t < 8u * ((uint32_t)a.xnj * a.xnk + (uint32_t)a.xni * a.xnk + (uint32_t)a.xni * a.xnj)) {
+    const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
+    const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
+    tfsf_sheet(a.g, a.pw, rowE, rowH, a.xs, a.xni, a.xnj, a.xnk, t);
   }
-  int i, j, k;
-  if (!slab_cell<IT>((IT)blk * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
-  const int I = a.g.I, J = a.g.J, K = a.g.K;
-  const size_t IJ = (size_t)I * J;
-  const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
-  const float *Ex = a.cur.f[EX], *Ey = a.cur.f[EY], *Ez = a.cur.f[EZ];
-  HNb e;
-  e.ex = Ex[id]; e.ey = Ey[id]; e.ez = Ez[id];
-  const bool in = i + 1 < I, jn = j + 1 < J, kn = k + 1 < K;
-  e.ey_i1 = in ? Ey[id + 1] : 0.0f;
-  e.ez_i1 = in ? Ez[id + 1] : 0.0f;
-  e.ex_j1 = jn ? Ex[id + I] : 0.0f;
-  e.ez_j1 = jn ? Ez[id + I] : 0.0f;
-  e.ex_k1 = kn ? Ex[id + IJ] : 0.0f;
-  e.ey_k1 = kn ? Ey[id + IJ] : 0.0f;
-  float hx = a.cur.f[HX][id], hy = a.cur.f[HY][id], hz = a.cur.f[HZ][id];
-  const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
-  h_cell(a.g, a.co, a.pw, rowE, a.pcur, a.pnxt, true, i, j, k, IJ, e, hx, hy, hz);
-  a.nxt.f[HX][id] = hx;
-  a.nxt.f[HY][id] = hy;
-  a.nxt.f[HZ][id] = hz;
 }
 
-template <typename IT>
-__global__ void __launch_bounds__(256) k_slab_h(const SlabArgs a) {
-  slab_h_body<IT>(a, blockIdx.x, a.n);
-}
+__global__ void __launch_bounds__(256) k_sheets(const SlabArgs a) { sheets_body(a, blockIdx.x, a.n); }
 
-template <typename IT>
-__global__ void __launch_bounds__(256) k_slab_h_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
+__global__ void __launch_bounds__(256) k_sheets_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
   __shared__ __align__(16) char raw[sizeof(SlabArgs)];
   SlabArgs &sa = *reinterpret_cast<SlabArgs *>(raw);
   const int b = blockIdx.x, g = find_grid(tab, ng, b);
   const int par = (tab[g].cur0 + step) & 1;
   load_slab_args(sa, tab[g].arg[par]);
-  slab_h_body<IT>(sa, b - tab[g].off, tab[g].n0 + step);
-}
-
-// C: E cells of the y and z CPML slabs (src/gpu.h:727-763 + plane wave + src/gpu.h:765-868),
-// reading the new H of kernels A and B.
-template <bool IDX, typename IT>
-__device__ __forceinline__ void slab_e_body(const SlabArgs &a, unsigned blk, int n) {
-  int i, j, k;
-  if (!slab_cell<IT>((IT)blk * 256 + threadIdx.x, a.g.I, a.g.J, a.g.K, a.g.P, i, j, k)) return;
-  const int I = a.g.I, J = a.g.J;
-  const size_t IJ = (size_t)I * J;
-  const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
-  const float *Hx = a.nxt.f[HX], *Hy = a.nxt.f[HY], *Hz = a.nxt.f[HZ];
-  ENb h;
-  h.hx = Hx[id]; h.hy = Hy[id]; h.hz = Hz[id];
-  h.hy_im = i > 0 ? Hy[id - 1] : 0.0f;
-  h.hz_im = i > 0 ? Hz[id - 1] : 0.0f;
-  h.hx_jm = j > 0 ? Hx[id - I] : 0.0f;
-  h.hz_jm = j > 0 ? Hz[id - I] : 0.0f;
-  h.hx_km = k > 0 ? Hx[id - IJ] : 0.0f;
-  h.hy_km = k > 0 ? Hy[id - IJ] : 0.0f;
-  float ce[3], ch[3];
-  cell_coef<IDX>(a.co, a.co.tab, id, i, I, ce, ch);
-  float ex = a.cur.f[EX][id], ey = a.cur.f[EY][id], ez = a.cur.f[EZ][id];
-  const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
-  e_cell(a.g, a.co, a.pw, rowH, a.pcur, a.pnxt, i, j, k, IJ, ce, ch, h, ex, ey, ez);
-  a.nxt.f[EX][id] = ex;
-  a.nxt.f[EY][id] = ey;
-  a.nxt.f[EZ][id] = ez;
-}
-
-template <bool IDX, typename IT>
-__global__ void __launch_bounds__(256) k_slab_e(const SlabArgs a) {
-  slab_e_body<IDX, IT>(a, blockIdx.x, a.n);
-}
-
-template <bool IDX, typename IT>
-__global__ void __launch_bounds__(256) k_slab_e_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
-  __shared__ __align__(16) char raw[sizeof(SlabArgs)];
-  SlabArgs &sa = *reinterpret_cast<SlabArgs *>(raw);
-  const int b = blockIdx.x, g = find_grid(tab, ng, b);
-  const int par = (tab[g].cur0 + step) & 1;
-  load_slab_args(sa, tab[g].arg[par]);
-  slab_e_body<IDX, IT>(sa, b - tab[g].off, tab[g].n0 + step);
+  sheets_body(sa, b - tab[g].off, tab[g].n0 + step);
 }
 
 // Probe gather of many grids after a batch step: job blockIdx.y, probes blockIdx.x * 64 + lane,
@@ -1873,12 +1650,16 @@ static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
   return best_waste >= 0;
 }
 
+// LDS bytes of the row exchange: four rows of Wrow floats per row of the tile (none for one row)
+static unsigned rows_lds(const StepArgs &a) { return a.R > 1 ? 16u * a.R * a.Wrow : 0u; }
+
 template <bool IDX, bool BUF, int PART>
-static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const StepArgs *d, int n) {
+static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, unsigned lds, const StepArgs *d, int n) {
   if (grid.x == 0) return;
-  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, 0, st, d, n);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, 0, st, d, n);
-  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, 0, st, d, n);
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, lds, st, d, n);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, lds, st, d, n);
+  else if (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF>), grid, blk, lds, st, d, n);
+  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, lds, st, d, n);
 }
 
 // part: 0 the interior part, 1 the slab part, -1 both (one after the other)
@@ -1888,12 +1669,13 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const size_t lim = ((size_t)1 << 32) - 1;
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
   const dim3 g0(part == 1 ? 0 : a.nwg0), g1(part == 0 ? 0 : a.nwg1), blk(64 * a.S * a.R);
+  const unsigned L = rows_lds(a);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, blk, dargs, n); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, blk, dargs, n); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, blk, L, dargs, n); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, blk, L, dargs, n); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, blk, dargs, n); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, blk, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, blk, dargs, n); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, blk, L, dargs, n); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, blk, L, dargs, n); }
   }
   return hipGetLastError();
 }
@@ -1913,8 +1695,14 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   if ((f.act_h | f.act_e) && g.i1 - g.i0 < cpl) {
     if (tune.cpl || !row_layout(g.I, 1, cpl, S)) return -1;
   }
-  int R = 4 / S;
-  if (tune.tile_rows > 0 && tune.tile_rows < R) R = tune.tile_rows;
+  // rows per tile (workgroup of R S waves, at most 1024 threads): 2 for 3-4 segments, 4 for 1-2
+  const int max_waves = kRowsMaxThreads[cpl] / 64;
+  // One row per tile where a row takes several segments (a tile waits for its slowest wave, and
+  // the 4-wave tiles keep the most workgroups resident), two rows for one-segment rows.
+  int R = S == 1 ? 2 : 1;
+  while (R > 1 && S * R > max_waves) R /= 2;
+  if (tune.tile_rows > 0) R = tune.tile_rows;
+  if (S * R > max_waves) return -1;
   // every row and plane of the grid; z chunks do not straddle the z-slab boundaries
   const int cJ = g.J, cK = g.K - 2 * g.P;
   const int nTy = (cJ + R - 1) / R;
@@ -1939,6 +1727,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.spl = (cpl == 1 && g.I % 64 == 0 && !tune.no_split) ? 1 : 0;
   a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
+  a.Wrow = 64 * cpl * S;
   a.nTlo = nTlo; a.nThi = nThi; a.nTmid = nTmid;
   a.nwg0 = nTmid * nz2;
   a.nwg1 = (nTlo + nThi) * nKc + nTmid * 2 * nz1;
@@ -2017,6 +1806,7 @@ RowsKey rows_key(const void *hargs) {
   k.buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
   k.cpl = a.CPL;
   k.threads = 64 * a.S * a.R;
+  k.lds = rows_lds(a);
   return k;
 }
 
@@ -2035,21 +1825,22 @@ void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *n
 }
 
 template <bool IDX, bool BUF, int PART>
-static void launch_rows_batch_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, const BatchGrid *tab, int ng, int step) {
-  if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4, PART>), grid, blk, 0, st, tab, ng, step);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2, PART>), grid, blk, 0, st, tab, ng, step);
-  else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1, PART>), grid, blk, 0, st, tab, ng, step);
+static void launch_rows_batch_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, unsigned lds, const BatchGrid *tab, int ng,
+                                int step) {
+  if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4, PART>), grid, blk, lds, st, tab, ng, step);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2, PART>), grid, blk, lds, st, tab, ng, step);
+  else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1, PART>), grid, blk, lds, st, tab, ng, step);
 }
 
 template <int PART>
 static void launch_rows_batch_p(hipStream_t st, const RowsKey &key, dim3 grid, dim3 blk, const BatchGrid *tab, int ng,
                                 int step) {
   if (key.idx) {
-    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, tab, ng, step);
-    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
+    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
   } else {
-    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, tab, ng, step);
-    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
+    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
   }
 }
 
@@ -2067,22 +1858,9 @@ int rows_part_nwg(const void *hargs, int part) {
   return part == 0 ? a.nwg0 : a.nwg1;
 }
 
-hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step) {
+hipError_t launch_sheets_batch(hipStream_t st, const BatchGrid *tab, int ng, int nwg, int step) {
   if (nwg <= 0) return hipSuccess;
-  if (narrow) hipLaunchKernelGGL(k_slab_h_batch<uint32_t>, dim3(nwg), dim3(256), 0, st, tab, ng, step);
-  else hipLaunchKernelGGL(k_slab_h_batch<uint64_t>, dim3(nwg), dim3(256), 0, st, tab, ng, step);
-  return hipGetLastError();
-}
-
-hipError_t launch_slab_e_batch(hipStream_t st, bool idx, bool narrow, const BatchGrid *tab, int ng, int nwg, int step) {
-  if (nwg <= 0) return hipSuccess;
-  if (idx) {
-    if (narrow) hipLaunchKernelGGL((k_slab_e_batch<true, uint32_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
-    else hipLaunchKernelGGL((k_slab_e_batch<true, uint64_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
-  } else {
-    if (narrow) hipLaunchKernelGGL((k_slab_e_batch<false, uint32_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
-    else hipLaunchKernelGGL((k_slab_e_batch<false, uint64_t>), dim3(nwg), dim3(256), 0, st, tab, ng, step);
-  }
+  hipLaunchKernelGGL(k_sheets_batch, dim3(nwg), dim3(256), 0, st, tab, ng, step);
   return hipGetLastError();
 }
 
@@ -2093,7 +1871,8 @@ hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, 
 }
 
 // One time step: the TF/SF face sheets of step n (when a plane wave is active), then the row
-// kernel over every cell.  ev_rows (optional): two events recorded around the row kernel.
+// kernel over every cell: its interior part, then its slab part.  ev_rows (optional): two events
+// recorded around the interior part (the step's dominant launch).
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows) {
   SlabArgs sa = slab_args(f, n);
@@ -2101,17 +1880,21 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
   const unsigned nbx = sheets ? cdiv(nsheet, 256) : 0;
-  if (nbx) hipLaunchKernelGGL(k_slab_h<uint32_t>, dim3(nbx), dim3(256), 0, st, sa);
+  if (nbx) hipLaunchKernelGGL(k_sheets, dim3(nbx), dim3(256), 0, st, sa);
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[0], st);
     if (e != hipSuccess) return e;
   }
   if (nwg > 0) {
-    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, -1, n);
+    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, 0, n);
     if (e != hipSuccess) return e;
   }
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[1], st);
+    if (e != hipSuccess) return e;
+  }
+  if (nwg > 0) {
+    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, 1, n);
     if (e != hipSuccess) return e;
   }
   return hipGetLastError();

@@ -54,7 +54,7 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
 
 // Fused step: reads the cur state, writes the nxt state (packed buffers).
 struct StepTuning {
-  int tile_rows = 0;  // rows per workgroup of the row kernel (1, 2, 4; 0: automatic)
+  int tile_rows = 0;  // rows per workgroup of the row kernel (1, 2, 4, 8, 16; 0: automatic)
   int z_chunk = 0;    // planes per chunk (0: automatic)
   int cpl = 0;        // columns per lane (1, 2, 4; 0: automatic)
   bool no_split = false;  // 1 column per lane without the split edge segment
@@ -81,7 +81,7 @@ struct FusedDev {
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.
 size_t step_args_size();
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out);
-// ev_rows (optional): two events recorded on st around the row kernel
+// ev_rows (optional): two events recorded on st around the row kernel's interior part
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows = nullptr);
 bool fused_supported(const GridDev &g);
@@ -107,21 +107,22 @@ struct ProbeJob {
 struct RowsKey {
   bool idx = false, buf = false;
   int cpl = 1, threads = 256;
+  unsigned lds = 0;  // dynamic LDS bytes of the row exchange
   bool operator<(const RowsKey &o) const {
-    return idx != o.idx ? idx < o.idx : buf != o.buf ? buf < o.buf : cpl != o.cpl ? cpl < o.cpl : threads < o.threads;
+    return idx != o.idx ? idx < o.idx : buf != o.buf ? buf < o.buf : cpl != o.cpl ? cpl < o.cpl
+           : threads != o.threads ? threads < o.threads : lds < o.lds;
   }
 };
 RowsKey rows_key(const void *hargs);
-// Slab-kernel argument block of f (host copy; n is supplied per step) with its workgroup counts:
-// slab cells, and the face-sheet workgroups k_slab_h adds.
+// Face-sheet argument block of f (host copy; n is supplied per step) with its workgroup counts:
+// nb_slab (always 0: the row kernel covers every cell) and the face-sheet workgroups.
 size_t slab_args_size();
 void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow);
 hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
                              int step);
 // Workgroups of part 0 (interior rows and chunks) or part 1 (the slab rows and chunks) of the row kernel.
 int rows_part_nwg(const void *hargs, int part);
-hipError_t launch_slab_h_batch(hipStream_t st, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
-hipError_t launch_slab_e_batch(hipStream_t st, bool idx, bool narrow, const BatchGrid *tab, int ng, int nwg, int step);
+hipError_t launch_sheets_batch(hipStream_t st, const BatchGrid *tab, int ng, int nwg, int step);
 hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, int max_ant, int step);
 
 }  // namespace fdtd

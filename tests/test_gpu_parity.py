@@ -104,12 +104,14 @@ def test_fused_step_axis_modes(mode, cpml):
 # with 2 take 1 and 2 segments, 264 with 4 takes 2, 136 with 1 takes 3), several rows per
 # workgroup; with 1 column per lane and I a multiple of 64 the split edge segment (both row ends in
 # one wave: I = 64, 128, 192, 256 take 1 to 4 segments), with the x faces of the TF/SF box in the
-# edge wave or (inset 40) in plain segments; -1 columns per lane: the same grids unsplit.
+# edge wave or (inset 40) in plain segments; -1 columns per lane: the same grids unsplit; rows of a
+# workgroup exchange their neighbour rows through LDS (up to 16 rows, 1024 threads).
 @pytest.mark.parametrize("I,rows,z_chunk,cpl,inset", [
     (52, 0, 0, 0, 15), (52, 1, 5, 1, 15), (52, 2, 13, 4, 15), (52, 4, 3, 2, 15),
     (136, 0, 0, 1, 15), (200, 0, 7, 2, 15), (264, 0, 0, 4, 15), (264, 1, 4, 2, 15),
     (64, 0, 0, 1, 15), (64, 4, 6, 1, 13), (128, 0, 0, 1, 15), (128, 2, 5, 1, 40), (192, 1, 0, 1, 40),
-    (192, 0, 3, 1, 15), (256, 0, 0, 1, 15), (256, 0, 9, 1, 40), (128, 0, 0, -1, 15), (256, 0, 0, -1, 15)])
+    (192, 0, 3, 1, 15), (256, 0, 0, 1, 15), (256, 0, 9, 1, 40), (128, 0, 0, -1, 15), (256, 0, 0, -1, 15),
+    (64, 16, 0, 1, 15), (128, 8, 5, 1, 15), (52, 8, 4, 1, 15), (256, 4, 0, 1, 15), (200, 2, 0, 2, 15)])
 def test_fused_tilings_oblique(I, rows, z_chunk, cpl, inset):
     """Oblique plane wave through every row layout: halo rows/planes recomputed from old data."""
     from maxwell_solver_amd import scenarios as S
