@@ -405,10 +405,12 @@ struct fdtd_sim {
     for (int c = 0; c < 6; ++c) dF[which][c] = dFb[which] + c * cs;
     if (int rc = dalloc(&dPsiB[which], 4 * (sx + sy + sz))) return rc;
     float *px = dPsiB[which], *py = px + 4 * sx, *pz = py + 4 * sy;
+    // three groups of four psi arrays, the members of a group interleaved per element (element t
+    // of member m at 4 t + m): the fused step reads and writes a cell's four values at once
     float **q = dPsi[which];
-    q[P_EYX] = px; q[P_EZX] = px + sx; q[P_HYX] = px + 2 * sx; q[P_HZX] = px + 3 * sx;
-    q[P_EXY] = py; q[P_EZY] = py + sy; q[P_HXY] = py + 2 * sy; q[P_HZY] = py + 3 * sy;
-    q[P_EXZ] = pz; q[P_EYZ] = pz + sz; q[P_HXZ] = pz + 2 * sz; q[P_HYZ] = pz + 3 * sz;
+    q[P_EYX] = px; q[P_EZX] = px + 1; q[P_HYX] = px + 2; q[P_HZX] = px + 3;
+    q[P_EXY] = py; q[P_EZY] = py + 1; q[P_HXY] = py + 2; q[P_HZY] = py + 3;
+    q[P_EXZ] = pz; q[P_EYZ] = pz + 1; q[P_HXZ] = pz + 2; q[P_HYZ] = pz + 3;
     const size_t len[12] = {sy, sz, sx, sz, sx, sy, sy, sz, sx, sz, sx, sy};
     for (int k = 0; k < 12; ++k) psi_len[k] = len[k];
     psi_total = 4 * (sx + sy + sz);
@@ -1573,6 +1575,13 @@ int fdtd_read_array(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
   if (count) *count = n;
   if (!out) return 0;
   if (host) { std::memcpy(out, tmp.data(), n * sizeof(float)); return 0; }
+  if (what >= 20 && what < 32) {  // psi: every 4th float of its interleaved group (one linear copy)
+    std::vector<float> grp(4 * n);
+    HIPCHK(hipMemcpyAsync(grp.data(), src, (4 * n - 3) * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t t = 0; t < n; ++t) out[t] = grp[4 * t];
+    return 0;
+  }
   HIPCHK(hipMemcpyAsync(out, src, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;

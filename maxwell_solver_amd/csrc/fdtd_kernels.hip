@@ -192,10 +192,10 @@ __global__ void __launch_bounds__(256) k_h_pml_y(GridDev g, FieldSet F, PsiSet p
   const size_t id = (size_t)i + (size_t)jj * I + (size_t)k * IJ;
   const float dx = co.dx;
   float *pxy = ps.p[P_HXY], *pzy = ps.p[P_HZY];
-  pxy[t] = co.bH[s] * pxy[t] + co.cH[s] * (F.f[EZ][id + I] - F.f[EZ][id]) / dx;
-  F.f[HX][id] = F.f[HX][id] - co.Che * pxy[t] * dx;
-  pzy[t] = co.bH[s] * pzy[t] + co.cH[s] * (F.f[EX][id + I] - F.f[EX][id]) / dx;
-  F.f[HZ][id] = F.f[HZ][id] + co.Che * pzy[t] * dx;
+  pxy[4 * t] = co.bH[s] * pxy[4 * t] + co.cH[s] * (F.f[EZ][id + I] - F.f[EZ][id]) / dx;
+  F.f[HX][id] = F.f[HX][id] - co.Che * pxy[4 * t] * dx;
+  pzy[4 * t] = co.bH[s] * pzy[4 * t] + co.cH[s] * (F.f[EX][id + I] - F.f[EX][id]) / dx;
+  F.f[HZ][id] = F.f[HZ][id] + co.Che * pzy[4 * t] * dx;
 }
 
 // K4 GPU_updateHPmlX, src/gpu.h:604-642: s in [0, 2P-1), j < J-1, k < K-1
@@ -211,10 +211,10 @@ __global__ void __launch_bounds__(256) k_h_pml_x(GridDev g, FieldSet F, PsiSet p
   const size_t id = (size_t)ii + (size_t)j * I + (size_t)k * IJ;
   const float dx = co.dx;
   float *pyx = ps.p[P_HYX], *pzx = ps.p[P_HZX];
-  pyx[t] = co.bH[s] * pyx[t] + co.cH[s] * (F.f[EZ][id + 1] - F.f[EZ][id]) / dx;
-  F.f[HY][id] = F.f[HY][id] + co.Che * pyx[t] * dx;
-  pzx[t] = co.bH[s] * pzx[t] + co.cH[s] * (F.f[EY][id + 1] - F.f[EY][id]) / dx;
-  F.f[HZ][id] = F.f[HZ][id] - co.Che * pzx[t] * dx;
+  pyx[4 * t] = co.bH[s] * pyx[4 * t] + co.cH[s] * (F.f[EZ][id + 1] - F.f[EZ][id]) / dx;
+  F.f[HY][id] = F.f[HY][id] + co.Che * pyx[4 * t] * dx;
+  pzx[4 * t] = co.bH[s] * pzx[4 * t] + co.cH[s] * (F.f[EY][id + 1] - F.f[EY][id]) / dx;
+  F.f[HZ][id] = F.f[HZ][id] - co.Che * pzx[4 * t] * dx;
 }
 
 // K6 GPU_updateHPmlZ, src/gpu.h:687-725: i < I-1, j < J-1, s in [0, 2P-1)
@@ -228,10 +228,10 @@ __global__ void __launch_bounds__(256) k_h_pml_z(GridDev g, FieldSet F, PsiSet p
   const size_t id = (size_t)i + (size_t)j * I + (size_t)kk * IJ;
   const float dx = co.dx;
   float *pxz = ps.p[P_HXZ], *pyz = ps.p[P_HYZ];
-  pxz[t] = co.bH[s] * pxz[t] + co.cH[s] * (F.f[EY][id + IJ] - F.f[EY][id]) / dx;
-  F.f[HX][id] = F.f[HX][id] + co.Che * pxz[t] * dx;
-  pyz[t] = co.bH[s] * pyz[t] + co.cH[s] * (F.f[EX][id + IJ] - F.f[EX][id]) / dx;
-  F.f[HY][id] = F.f[HY][id] - co.Che * pyz[t] * dx;
+  pxz[4 * t] = co.bH[s] * pxz[4 * t] + co.cH[s] * (F.f[EY][id + IJ] - F.f[EY][id]) / dx;
+  F.f[HX][id] = F.f[HX][id] + co.Che * pxz[4 * t] * dx;
+  pyz[4 * t] = co.bH[s] * pyz[4 * t] + co.cH[s] * (F.f[EX][id + IJ] - F.f[EX][id]) / dx;
+  F.f[HY][id] = F.f[HY][id] - co.Che * pyz[4 * t] * dx;
 }
 
 // K8 GPU_updateEPmlY, src/gpu.h:790-827: 0 < i < I-1, s in [1, 2P), 1 <= k < K-1
@@ -248,10 +248,10 @@ __global__ void __launch_bounds__(256) k_e_pml_y(GridDev g, FieldSet F, PsiSet p
   cell_coef<IDX>(co, co.tab, id, i, I, ce, ch);
   const float dx = co.dx;
   float *pxy = ps.p[P_EXY], *pzy = ps.p[P_EZY];
-  pxy[t] = co.bE[s] * pxy[t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - I]) / dx;
-  F.f[EX][id] = F.f[EX][id] + ch[0] * pxy[t] * dx;
-  pzy[t] = co.bE[s] * pzy[t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - I]) / dx;
-  F.f[EZ][id] = F.f[EZ][id] - ch[2] * pzy[t] * dx;
+  pxy[4 * t] = co.bE[s] * pxy[4 * t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - I]) / dx;
+  F.f[EX][id] = F.f[EX][id] + ch[0] * pxy[4 * t] * dx;
+  pzy[4 * t] = co.bE[s] * pzy[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - I]) / dx;
+  F.f[EZ][id] = F.f[EZ][id] - ch[2] * pzy[4 * t] * dx;
 }
 
 // K9 GPU_updateEPmlZ, src/gpu.h:829-868: 0 < i < I-1, 0 < j < J-1, s in [1, 2P)
@@ -268,10 +268,10 @@ __global__ void __launch_bounds__(256) k_e_pml_z(GridDev g, FieldSet F, PsiSet p
   cell_coef<IDX>(co, co.tab, id, i, I, ce, ch);
   const float dx = co.dx;
   float *pxz = ps.p[P_EXZ], *pyz = ps.p[P_EYZ];
-  pxz[t] = co.bE[s] * pxz[t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - IJ]) / dx;
-  F.f[EX][id] = F.f[EX][id] - ch[0] * pxz[t] * dx;
-  pyz[t] = co.bE[s] * pyz[t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - IJ]) / dx;
-  F.f[EY][id] = F.f[EY][id] + ch[1] * pyz[t] * dx;
+  pxz[4 * t] = co.bE[s] * pxz[4 * t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - IJ]) / dx;
+  F.f[EX][id] = F.f[EX][id] - ch[0] * pxz[4 * t] * dx;
+  pyz[4 * t] = co.bE[s] * pyz[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - IJ]) / dx;
+  F.f[EY][id] = F.f[EY][id] + ch[1] * pyz[4 * t] * dx;
 }
 
 // K7 GPU_updateEPmlX, src/gpu.h:765-788: s in [1, 2P), 0 < j < J-1, 1 <= k < K-1
@@ -290,10 +290,10 @@ __global__ void __launch_bounds__(256) k_e_pml_x(GridDev g, FieldSet F, PsiSet p
   cell_coef<IDX>(co, co.tab, id, ii, I, ce, ch);
   const float dx = co.dx;
   float *pyx = ps.p[P_EYX], *pzx = ps.p[P_EZX];
-  pyx[t] = co.bE[s] * pyx[t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - 1]) / dx;
-  F.f[EY][id] = F.f[EY][id] - ch[1] * pyx[t] * dx;
-  pzx[t] = co.bE[s] * pzx[t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - 1]) / dx;
-  F.f[EZ][id] = F.f[EZ][id] + ch[2] * pzx[t] * dx;
+  pyx[4 * t] = co.bE[s] * pyx[4 * t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - 1]) / dx;
+  F.f[EY][id] = F.f[EY][id] - ch[1] * pyx[4 * t] * dx;
+  pzx[4 * t] = co.bE[s] * pzx[4 * t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - 1]) / dx;
+  F.f[EZ][id] = F.f[EZ][id] + ch[2] * pzx[4 * t] * dx;
 }
 
 // TF/SF corrections of face axis FA, both faces (one kernel per face axis, launched x, y, z).
@@ -872,20 +872,18 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     // and the rows the z faces of the box cover (j, and j-1 for the recomputed Hx)
     const int jfl = uni((j >= bj0 && j <= bj1 ? 1 : 0) | (j >= bj0 && j <= bj1 - 1 ? 2 : 0) |
                         (j - 1 >= bj0 && j - 1 <= bj1 - 1 ? 4 : 0) | (zrow ? 8 : 0) | (zrow_m ? 16 : 0));
-    // psi groups (bytes): x [Eyx Ezx Hyx Hzx] of 2P J K elements, element s + 2P j + 2P J k;
-    // y [Exy Ezy Hxy Hzy] of 2P I K, element i + I s + 2P I k; z [Exz Eyz Hxz Hyz] of 2P I J,
-    // element i + I j + I J s
+    // psi groups, the four members of a group interleaved per element (16 bytes): x [Eyx Ezx Hyx
+    // Hzx], 2P J K elements, element s + 2P j + 2P J k; y [Exy Ezy Hxy Hzy], 2P I K elements,
+    // i + I s + 2P I k; z [Exz Eyz Hxz Hyz], 2P I J elements, i + I j + I J s.  A lane reads and
+    // writes the four values of a column with one 16-byte access.
     const size_t sx = pa->sx, syy = pa->sy, szz = pa->sz;
     const Acc<BUF> Pc(pa->Pc, 16 * (sx + syy + szz));
     const Acc<BUF> Pn(pa->Pn, 16 * (sx + syy + szz));
-    const O gEz = (O)(sx * 4), gHy = 2 * gEz, gHz = 3 * gEz;
-    const O gExy = 4 * gEz, gEzy = gExy + (O)(syy * 4), gHxy = gExy + (O)(2 * syy * 4), gHzy = gExy + (O)(3 * syy * 4);
-    const O gExz = gExy + (O)(4 * syy * 4), gEyz = gExz + (O)(szz * 4), gHxz = gExz + (O)(2 * szz * 4),
-            gHyz = gExz + (O)(3 * szz * 4);
-    const O oc = (O)((on ? (size_t)ib : 0) * 4);  // lane offset of the y- and z-slab psi (column i)
+    const O gY = (O)(16 * sx), gZ = gY + (O)(16 * syy);  // byte offsets of the y and z groups
+    const O oc = (O)((on ? (size_t)ib : 0) * 16);  // lane offset of the y- and z-slab psi (column i)
     O so[CPL];
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 4);
+    for (int c = 0; c < CPL; ++c) so[c] = (O)((sl[c] < 0 ? 0 : sl[c]) * 16);
 
     // ---- carried state: E of row j at plane k, Ey of row j-1 at plane k, H(i, j, k-1) ----
     float ex[CPL], ey[CPL], ez[CPL], eym[CPL], hpx[CPL], hpy[CPL];
@@ -922,34 +920,44 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         if (IDX && own) Mt.ldm(o1 >> 2, 0, Lp.mt);
         // y-slab psi of rows j and j-1
         if (SLAB && ys) {
-          const O u = uni((O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 4));
-          Pc.ld(oc, gHxy + u, Lp.qHxy); Pc.ld(oc, gHzy + u, Lp.qHzy);
-          if (own) { Pc.ld(oc, gExy + u, Lp.qExy); Pc.ld(oc, gEzy + u, Lp.qEzy); }
+          const O u = uni(gY + (O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 16));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            float v[4];
+            Pc.template ld<4>(oc + (O)(16 * c), u, v);
+            Lp.qExy[c] = v[0]; Lp.qEzy[c] = v[1]; Lp.qHxy[c] = v[2]; Lp.qHzy[c] = v[3];
+          }
         }
         if (SLAB && own && ysm && firstr) {
-          const O u = uni((O)((size_t)I * ((size_t)sym + (size_t)P2 * k) * 4));
-          Pc.ld(oc, gHxy + u, Lp.qHxym); Pc.ld(oc, gHzy + u, Lp.qHzym);
+          const O u = uni(gY + (O)((size_t)I * ((size_t)sym + (size_t)P2 * k) * 16));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            float v[4];
+            Pc.template ld<4>(oc + (O)(16 * c), u, v);
+            Lp.qHxym[c] = v[2]; Lp.qHzym[c] = v[3];
+          }
         }
         // z-slab psi of plane k (also on the plane below an interior chunk, when that is a slab plane)
         if ((SLAB || !own) && (k < P || k >= K - P)) {
           const int sz = k < P ? k : k - (K - P2);
-          const O u = uni((O)(((size_t)I * j + IJ * (size_t)sz) * 4));
-          Pc.ld(oc, gHxz + u, Lp.qHxz); Pc.ld(oc, gHyz + u, Lp.qHyz);
-          if (own) {
-            Pc.ld(oc, gExz + u, Lp.qExz); Pc.ld(oc, gEyz + u, Lp.qEyz);
-            if (has_jm && firstr) Pc.ld(oc, gHxz + u - (O)(I * 4), Lp.qHxzm);
+          const O u = uni(gZ + (O)(((size_t)I * j + IJ * (size_t)sz) * 16));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) {
+            float v[4];
+            Pc.template ld<4>(oc + (O)(16 * c), u, v);
+            Lp.qExz[c] = v[0]; Lp.qEyz[c] = v[1]; Lp.qHxz[c] = v[2]; Lp.qHyz[c] = v[3];
+            if (own && has_jm && firstr) Lp.qHxzm[c] = Pc.ld1(oc + (O)(16 * c), u - (O)(I * 16) + 8);
           }
         }
         if (EDGE && bit(f, F_SLAB)) {
-          const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
+          const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 16));
 #pragma unroll
           for (int c = 0; c < CPL; ++c)
             if (slab_col(f, c)) {
-              Lp.pHy[c] = Pc.ld1(so[c], gHy + pq);
-              Lp.pHz[c] = Pc.ld1(so[c], gHz + pq);
-              if (has_jm && firstr) Lp.pHzm[c] = Pc.ld1(so[c], gHz + pq - (O)(P2 * 4));
-              Lp.pEy[c] = Pc.ld1(so[c], pq);
-              Lp.pEz[c] = Pc.ld1(so[c], gEz + pq);
+              float v[4];
+              Pc.template ld<4>(so[c], pq, v);
+              Lp.pEy[c] = v[0]; Lp.pEz[c] = v[1]; Lp.pHy[c] = v[2]; Lp.pHz[c] = v[3];
+              if (has_jm && firstr) Lp.pHzm[c] = Pc.ld1(so[c], pq - (O)(P2 * 16) + 12);
             }
         }
       }
@@ -1229,15 +1237,6 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           Fn.st(o1, c3, hx);
           Fn.st(o1, c4, hy);
           Fn.st(o1, c5, hz);
-          if (EDGE && bit(f, F_SLAB)) {
-            const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
-#pragma unroll
-            for (int c = 0; c < CPL; ++c)
-              if (slab_col(f, c)) {
-                Pn.st1(so[c], gHy + pq, Lp.pHy[c]);
-                Pn.st1(so[c], gHz + pq, Lp.pHz[c]);
-              }
-          }
         }
         // E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML Y-Z-X (src/gpu.h:765-868)
         float ce[3][CPL], ch[3][CPL];
@@ -1375,25 +1374,32 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           Fn.st(o1, 0, ux);
           Fn.st(o1, c1, uy);
           Fn.st(o1, c2, uz);
+          // psi: the four members of each slab column in one 16-byte store; every element is written
+          // each step (updated, or copied outside the masks)
           if (EDGE && bit(f, F_SLAB)) {
-            const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 4));
+            const O pq = uni((O)((size_t)P2 * ((size_t)j + (size_t)J * k) * 16));
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
               if (slab_col(f, c)) {
-                Pn.st1(so[c], pq, Lp.pEy[c]);
-                Pn.st1(so[c], gEz + pq, Lp.pEz[c]);
+                const float v[4] = {Lp.pEy[c], Lp.pEz[c], Lp.pHy[c], Lp.pHz[c]};
+                Pn.template st<4>(so[c], pq, v);
               }
           }
-          // y- and z-slab psi: every element written each step (updated, or copied outside the masks)
           if (SLAB && ys) {
-            const O u = uni((O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 4));
-            Pn.st(oc, gHxy + u, Lp.qHxy); Pn.st(oc, gHzy + u, Lp.qHzy);
-            Pn.st(oc, gExy + u, Lp.qExy); Pn.st(oc, gEzy + u, Lp.qEzy);
+            const O u = uni(gY + (O)((size_t)I * ((size_t)sy + (size_t)P2 * k) * 16));
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+              const float v[4] = {Lp.qExy[c], Lp.qEzy[c], Lp.qHxy[c], Lp.qHzy[c]};
+              Pn.template st<4>(oc + (O)(16 * c), u, v);
+            }
           }
           if (SLAB && zs) {
-            const O u = uni((O)(((size_t)I * j + IJ * (size_t)sz) * 4));
-            Pn.st(oc, gHxz + u, Lp.qHxz); Pn.st(oc, gHyz + u, Lp.qHyz);
-            Pn.st(oc, gExz + u, Lp.qExz); Pn.st(oc, gEyz + u, Lp.qEyz);
+            const O u = uni(gZ + (O)(((size_t)I * j + IJ * (size_t)sz) * 16));
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+              const float v[4] = {Lp.qExz[c], Lp.qEyz[c], Lp.qHxz[c], Lp.qHyz[c]};
+              Pn.template st<4>(oc + (O)(16 * c), u, v);
+            }
           }
         }
       }
@@ -1755,11 +1761,11 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
     s.cur.f[c] = const_cast<float *>(f.Fc) + c * f.cs;
     s.nxt.f[c] = f.Fn + c * f.cs;
   }
-  auto psi = [&](float *base, PsiSet &ps) {
+  auto psi = [&](float *base, PsiSet &ps) {  // interleaved groups: element t of member m at 4 t + m
     float *px = base, *py = px + 4 * sx, *pz = py + 4 * sy;
-    ps.p[P_EYX] = px; ps.p[P_EZX] = px + sx; ps.p[P_HYX] = px + 2 * sx; ps.p[P_HZX] = px + 3 * sx;
-    ps.p[P_EXY] = py; ps.p[P_EZY] = py + sy; ps.p[P_HXY] = py + 2 * sy; ps.p[P_HZY] = py + 3 * sy;
-    ps.p[P_EXZ] = pz; ps.p[P_EYZ] = pz + sz; ps.p[P_HXZ] = pz + 2 * sz; ps.p[P_HYZ] = pz + 3 * sz;
+    ps.p[P_EYX] = px; ps.p[P_EZX] = px + 1; ps.p[P_HYX] = px + 2; ps.p[P_HZX] = px + 3;
+    ps.p[P_EXY] = py; ps.p[P_EZY] = py + 1; ps.p[P_HXY] = py + 2; ps.p[P_HZY] = py + 3;
+    ps.p[P_EXZ] = pz; ps.p[P_EYZ] = pz + 1; ps.p[P_HXZ] = pz + 2; ps.p[P_HYZ] = pz + 3;
   };
   psi(const_cast<float *>(f.Pc), s.pcur);
   psi(f.Pn, s.pnxt);
