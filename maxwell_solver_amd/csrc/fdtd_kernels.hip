@@ -412,7 +412,8 @@ __global__ void k_probes(FieldSet F, const int *idx, int nAnt, const int *comps,
 struct StepArgs {
   int I, J, K, P;
   int i0, i1, j0, j1, k0, k1;
-  int S, R, CPL, nTy;  // row segments per row, rows per workgroup, columns per lane, row tiles
+  int S, R, CPL, nTy;  // row segments per row, rows per workgroup (slab part), columns per lane, row tiles
+  int R0, nTmid0;      // rows per workgroup and row tiles of the interior part
   int KC, ntab, act_h, act_e;
   int KCS, nz1, nz2;  // z chunks: nz1 of KCS planes over [0, P), nz2 of KC over [P, K-P), nz1 over [K-P, K)
   int Wrow;           // LDS row stride of the row exchange: 64 CPL S columns
@@ -698,7 +699,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   constexpr int W = 64 * CPL;
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
-  const int S = pa->S, R = pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
+  const int S = pa->S, R = PART == 0 ? pa->R0 : pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
   const int act_h = uni(pa->act_h), act_e = uni(pa->act_e);
   const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J;
   const float Chh = pa->Chh, Che = pa->Che, dx = pa->dx;
@@ -737,7 +738,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   {
     const int nTlo = pa->nTlo, nThi = pa->nThi, nTmid = pa->nTmid, n1 = pa->nz1, n2 = pa->nz2;
     if (PART == 0) {
-      row0 = P + 1 + R * (L % nTmid); row_end = J - P; tz = n1 + L / nTmid;
+      const int nT0 = pa->nTmid0;
+      row0 = P + 1 + R * (L % nT0); row_end = J - P; tz = n1 + L / nT0;
     } else {
       const int nb = nTlo + nThi, nA = nb * (2 * n1 + n2);
       if (L < nA) {
@@ -1657,7 +1659,10 @@ static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
 }
 
 // LDS bytes of the row exchange: four rows of Wrow floats per row of the tile (none for one row)
-static unsigned rows_lds(const StepArgs &a) { return a.R > 1 ? 16u * a.R * a.Wrow : 0u; }
+static unsigned rows_lds(const StepArgs &a, int part) {
+  const int R = part == 0 ? a.R0 : a.R;
+  return R > 1 ? 16u * R * a.Wrow : 0u;
+}
 
 template <bool IDX, bool BUF, int PART>
 static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, unsigned lds, const StepArgs *d, int n) {
@@ -1674,14 +1679,14 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
-  const dim3 g0(part == 1 ? 0 : a.nwg0), g1(part == 0 ? 0 : a.nwg1), blk(64 * a.S * a.R);
-  const unsigned L = rows_lds(a);
+  const dim3 g0(part == 1 ? 0 : a.nwg0), g1(part == 0 ? 0 : a.nwg1), b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
+  const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, blk, L, dargs, n); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, blk, L, dargs, n); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, blk, L, dargs, n); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, blk, L, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, blk, L, dargs, n); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
   }
   return hipGetLastError();
 }
@@ -1701,18 +1706,21 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   if ((f.act_h | f.act_e) && g.i1 - g.i0 < cpl) {
     if (tune.cpl || !row_layout(g.I, 1, cpl, S)) return -1;
   }
-  // rows per tile (workgroup of R S waves, at most 1024 threads): 2 for 3-4 segments, 4 for 1-2
+  // rows per tile (a workgroup of R S waves, at most the row kernel's bound).  Interior part: two
+  // rows (measured at 256^3: 180 us against 190 with one; the rows share their neighbour rows
+  // through LDS), four for one- and two-segment rows.  Slab part: one row where a row takes several
+  // segments (its heavier waves keep fewer workgroups resident; two rows cost it 35%), else two.
   const int max_waves = kRowsMaxThreads[cpl] / 64;
-  // One row per tile where a row takes several segments (a tile waits for its slowest wave, and
-  // the 4-wave tiles keep the most workgroups resident), two rows for one-segment rows.
-  int R = S == 1 ? 2 : 1;
+  int R0 = S >= 3 ? 2 : 4, R = S == 1 ? 2 : 1;
+  while (R0 > 1 && S * R0 > max_waves) R0 /= 2;
   while (R > 1 && S * R > max_waves) R /= 2;
-  if (tune.tile_rows > 0) R = tune.tile_rows;
-  if (S * R > max_waves) return -1;
+  if (tune.tile_rows > 0) R = R0 = tune.tile_rows;
+  if (S * R > max_waves || S * R0 > max_waves) return -1;
   // every row and plane of the grid; z chunks do not straddle the z-slab boundaries
   const int cJ = g.J, cK = g.K - 2 * g.P;
   const int nTy = (cJ + R - 1) / R;
   const int nTlo = (g.P + 1 + R - 1) / R, nThi = (g.P + R - 1) / R, nTmid = (g.J - 2 * g.P - 1 + R - 1) / R;
+  const int nTmid0 = (g.J - 2 * g.P - 1 + R0 - 1) / R0;
   int kc = tune.z_chunk;
   if (kc <= 0) {
     // 8 planes per chunk (one recomputed plane below each chunk: 1/8 extra reads), fewer when
@@ -1735,7 +1743,8 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
   a.Wrow = 64 * cpl * S;
   a.nTlo = nTlo; a.nThi = nThi; a.nTmid = nTmid;
-  a.nwg0 = nTmid * nz2;
+  a.R0 = R0; a.nTmid0 = nTmid0;
+  a.nwg0 = nTmid0 * nz2;
   a.nwg1 = (nTlo + nThi) * nKc + nTmid * 2 * nz1;
   a.xni = g.i1 - g.i0 + 1;
   a.xnj = g.j1 - g.j0 + 1;
@@ -1812,7 +1821,9 @@ RowsKey rows_key(const void *hargs) {
   k.buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
   k.cpl = a.CPL;
   k.threads = 64 * a.S * a.R;
-  k.lds = rows_lds(a);
+  k.threads0 = 64 * a.S * a.R0;
+  k.lds = rows_lds(a, 1);
+  k.lds0 = rows_lds(a, 0);
   return k;
 }
 
@@ -1842,18 +1853,18 @@ template <int PART>
 static void launch_rows_batch_p(hipStream_t st, const RowsKey &key, dim3 grid, dim3 blk, const BatchGrid *tab, int ng,
                                 int step) {
   if (key.idx) {
-    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
-    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
+    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
   } else {
-    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
-    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, key.lds, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
+    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
   }
 }
 
 hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
                              int step) {
   if (nwg <= 0) return hipSuccess;
-  const dim3 grid(nwg), blk(key.threads);
+  const dim3 grid(nwg), blk(part == 0 ? key.threads0 : key.threads);
   if (part == 0) launch_rows_batch_p<0>(st, key, grid, blk, tab, ng, step);
   else launch_rows_batch_p<1>(st, key, grid, blk, tab, ng, step);
   return hipGetLastError();

@@ -106,11 +106,16 @@ struct ProbeJob {
 // Row-kernel variant of a grid (grids of one batched row launch share it).
 struct RowsKey {
   bool idx = false, buf = false;
-  int cpl = 1, threads = 256;
-  unsigned lds = 0;  // dynamic LDS bytes of the row exchange
+  int cpl = 1, threads = 256, threads0 = 256;  // workgroup sizes of the slab part and the interior part
+  unsigned lds = 0, lds0 = 0;                    // their dynamic LDS bytes (row exchange)
   bool operator<(const RowsKey &o) const {
-    return idx != o.idx ? idx < o.idx : buf != o.buf ? buf < o.buf : cpl != o.cpl ? cpl < o.cpl
-           : threads != o.threads ? threads < o.threads : lds < o.lds;
+    if (idx != o.idx) return idx < o.idx;
+    if (buf != o.buf) return buf < o.buf;
+    if (cpl != o.cpl) return cpl < o.cpl;
+    if (threads != o.threads) return threads < o.threads;
+    if (threads0 != o.threads0) return threads0 < o.threads0;
+    if (lds != o.lds) return lds < o.lds;
+    return lds0 < o.lds0;
   }
 };
 RowsKey rows_key(const void *hargs);
