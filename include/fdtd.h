@@ -192,9 +192,11 @@ int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_
  * and z-chunk choice (reps steps each) and keep the fastest (*best_ms: its average step time).
  * The steps are run dry: the fields, psi, probe record and time step are unchanged. */
 int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms);
-/* Average device time of the fused step kernel over the last fdtd_run with timing on
- * (HIP events around each launch on the simulation's stream). */
-int fdtd_set_timing(fdtd_sim *s, int32_t on);
+/* Average device time of the fused step over the last fdtd_run with timing on: HIP events on the
+ * simulation's stream around every `every`-th step of the run (steps 0, every, 2 every, ...;
+ * every = 1: all steps, 0: off).  Each pair of events costs the stream a few microseconds, so a
+ * long run is timed on a sample of its steps. *launches = the steps timed. */
+int fdtd_set_timing(fdtd_sim *s, int32_t every);
 int fdtd_kernel_time(fdtd_sim *s, double *avg_ms, int64_t *launches);
 /* Average device time of the row kernel's interior part alone (the rows j in [P+1, J-P) of the
  * planes k in [P, K-P), the step's dominant launch) over the same run. */

@@ -110,7 +110,7 @@ struct fdtd_sim {
   float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
   float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
   float *dPwTab[6] = {};      // per plane-wave mode: E tables x,y,z (int, half), then H tables
-  float *dXs = nullptr;       // face sheets of the TF/SF box (fused step, refilled every step)
+  float *dXs[2] = {nullptr, nullptr};  // face sheets of the TF/SF box, one buffer per state set (fused step)
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
@@ -127,7 +127,7 @@ struct fdtd_sim {
   };
   ArgSlot argslots[2][6];
   int args_gen = 0;
-  bool timing = false;
+  int timing = 0;  // HIP events around every timing-th step of fdtd_run (0: none)
   std::vector<hipEvent_t> ev;
   double kernel_ms = 0, rows_ms = 0;  // summed step / row-kernel event times of the last timed run
   int64_t kernel_launches = 0;
@@ -168,7 +168,7 @@ struct fdtd_sim {
       b += 2 * (2 * (size_t)T * N + 4 * N + T) * sizeof(float);  // two lines: histories + uploads
       b += 6 * 4 * ((size_t)I + J + K) * sizeof(float);       // plane-wave tables
       const size_t ni = I, nj = J, nk = K;
-      b += 8 * (nj * nk + ni * nk + ni * nj) * sizeof(float);  // face sheets
+      b += 2 * 8 * (nj * nk + ni * nk + ni * nj) * sizeof(float);  // face sheets, two buffers
     }
     if (init_antenna) b += (probe_comps.size() * (size_t)T * nAnt + nAnt + probe_comps.size()) * sizeof(float);
     return b + ((size_t)1 << 20);  // argument blocks and small tables
@@ -566,10 +566,10 @@ struct fdtd_sim {
   int fused_slot(int which, int mode, const PWDev *pw, FusedDev &f, ArgSlot **out) {
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
-    if (pw && !dXs && g.i1 >= g.i0 && g.j1 >= g.j0 && g.k1 >= g.k0) {
+    if (pw && !dXs[0] && g.i1 >= g.i0 && g.j1 >= g.j0 && g.k1 >= g.k0) {
       const size_t ni = g.i1 - g.i0 + 1, nj = g.j1 - g.j0 + 1, nk = g.k1 - g.k0 + 1;
       const size_t nxs = 8 * (nj * nk + ni * nk + ni * nj);
-      if (dalloc(&dXs, nxs)) return FDTD_ERR_NOMEM;
+      if (dalloc(&dXs[0], nxs) || dalloc(&dXs[1], nxs)) return FDTD_ERR_NOMEM;
       args_gen++;
     }
     ArgSlot &slot = argslots[which][slot_mode];
@@ -583,7 +583,7 @@ struct fdtd_sim {
       f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
       f.prm = dPrm;
       if (pw) {
-        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1; f.xs = dXs;
+        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1; f.xs = dXs[which];
         f.act_h = pw->active_h; f.act_e = pw->active_e;
         for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
       }
@@ -604,9 +604,26 @@ struct fdtd_sim {
     return 0;
   }
 
+  // State carried from step to step by a run of consecutive steps (fdtd_run): which face sheets
+  // the previous step filled in advance, and the probe record row still to be gathered from the
+  // current state (the slab part's launch of the next step, or flush_probes, gathers it).
+  struct RunFold {
+    int sheets_n = -1, sheets_set = -1;
+    int probe_row = -1;
+  };
+
+  int flush_probes(RunFold &rf) {
+    if (rf.probe_row >= 0)
+      HIPCHK(launch_probes(stream, fields(cur), dProbeIdx, nAnt, dProbeComps, (int)probe_comps.size(), dPort, rf.probe_row, T));
+    rf.probe_row = -1;
+    return 0;
+  }
+
   // One fused time step.  dry: launch the kernels (they read the current set and write the next)
   // but keep the current set and the probe recorder: the state is unchanged (used for timing).
-  int step(int n, int mode, bool timed, int *ev_slot, bool dry = false) {
+  // rf (a run of steps): fold the next step's face sheets and this state's probe gather into the
+  // step's launches.
+  int step(int n, int mode, bool timed, int *ev_slot, bool dry = false, RunFold *rf = nullptr) {
     if (!init_ready) return fail(FDTD_ERR_STATE, "Step before ReadySimulation");
     const PWDev *pw = nullptr;
     if (int rc = check_pw(n, mode, &pw)) return rc;
@@ -616,6 +633,10 @@ struct fdtd_sim {
     ArgSlot &slot = *slotp;
     if (slot.nwg < 0) {
       if (dry) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
+      if (rf) {
+        if (int rc = flush_probes(*rf)) return rc;
+        rf->sheets_n = -1;
+      }
       // a grid outside the fused step's layouts (rows over 1024 columns, P over 64, ...): the
       // per-op kernels in the reference op order, in place on the current set
       if (timed) {
@@ -636,11 +657,29 @@ struct fdtd_sim {
       HIPCHK(hipEventRecord(e[0], stream));
       *ev_slot += 4;
     }
-    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr));
+    if (!rf || dry) {
+      HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr));
+      if (timed) HIPCHK(hipEventRecord(e[3], stream));
+      if (dry) return 0;
+      cur = 1 - cur;
+      return record_probes(cur);
+    }
+    StepFold fo;
+    fo.sheets_ready = rf->sheets_n == n && rf->sheets_set == cur;
+    fo.xs_next = n + 1 < T ? dXs[1 - cur] : nullptr;
+    if (rf->probe_row >= 0) {
+      fo.F = fields(cur);
+      fo.idx = dProbeIdx; fo.comps = dProbeComps; fo.port = dPort;
+      fo.nAnt = nAnt; fo.ncomp = (int)probe_comps.size(); fo.row = rf->probe_row; fo.T = T;
+    }
+    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr, &fo));
     if (timed) HIPCHK(hipEventRecord(e[3], stream));
-    if (dry) return 0;
+    rf->sheets_n = fo.next_filled ? n + 1 : -1;
+    rf->sheets_set = 1 - cur;
     cur = 1 - cur;
-    return record_probes(cur);
+    rf->probe_row = dPort && rec_step < T ? rec_step : -1;
+    rec_step++;
+    return 0;
   }
 };
 
@@ -1462,26 +1501,33 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
       s->dSnaps.push_back((float *)p);
     }
   }
-  if (s->timing) {
-    while ((int)s->ev.size() < 4 * count) {
+  const int every = s->timing;  // time steps q with q % every == 0 (0: none)
+  const int ntimed = every > 0 ? (count + every - 1) / every : 0;
+  if (every > 0) {
+    while ((int)s->ev.size() < 4 * ntimed) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
       s->ev.push_back(e);
     }
   }
   int slot = 0;
+  fdtd_sim::RunFold rf;
   for (int q = 0; q < count; ++q) {
-    if (int rc = s->step(n0 + q, mode, s->timing, &slot)) return rc;
+    if (int rc = s->step(n0 + q, mode, every > 0 && q % every == 0, &slot, false, &rf)) {
+      s->flush_probes(rf);
+      return rc;
+    }
     if (snap_every > 0 && s->rec_step % snap_every == 0) {
       HIPCHK(hipMemcpyAsync(s->dSnaps[s->nsnap], s->dF[s->cur][snap_comp - 1], s->IJK * sizeof(float),
                             hipMemcpyDeviceToDevice, s->stream));
       s->nsnap++;
     }
   }
-  if (s->timing && count > 0) {
+  if (int rc = s->flush_probes(rf)) return rc;
+  if (ntimed > 0) {
     HIPCHK(hipStreamSynchronize(s->stream));
     double tot = 0, rows = 0;
-    for (int q = 0; q < count; ++q) {
+    for (int q = 0; q < ntimed; ++q) {
       float ms = 0, rms = 0;
       HIPCHK(hipEventElapsedTime(&ms, s->ev[4 * q], s->ev[4 * q + 3]));
       HIPCHK(hipEventElapsedTime(&rms, s->ev[4 * q + 1], s->ev[4 * q + 2]));
@@ -1490,7 +1536,7 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
     }
     s->kernel_ms = tot;
     s->rows_ms = rows;
-    s->kernel_launches = count;
+    s->kernel_launches = ntimed;
   }
   return 0;
 }
@@ -1644,9 +1690,10 @@ int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_
   return 0;
 }
 
-int fdtd_set_timing(fdtd_sim *s, int32_t on) {
+int fdtd_set_timing(fdtd_sim *s, int32_t every) {
   NEED(s);
-  s->timing = on != 0;
+  if (every < 0) return fail(FDTD_ERR_INVALID, "negative timing interval");
+  s->timing = every;
   return 0;
 }
 

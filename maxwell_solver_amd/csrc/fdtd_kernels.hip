@@ -8,11 +8,14 @@
 //    one pass: one wave per aligned row segment marching up a chunk of planes, all CPML slabs and
 //    TF/SF corrections inline, in two launches (the interior rows and planes; the slab rows and
 //    planes).  Each field is read once and written once per step outside the recomputed halo.
+//    In a run of steps the slab launch also fills the next step's sheets and gathers the probes,
+//    so a step is two launches.
 //
 // All arithmetic is plain IEEE float with no contraction (-ffp-contract=off); expressions keep the
 // reference's association order so results equal the CPU oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <type_traits>
 
@@ -1450,9 +1453,50 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   }
 }
 
+// Work the slab part's launch carries after its rows (fdtd_run): the TF/SF face sheets of the next
+// time step, into the sheet buffer of the other state set, and the probe gather of the state the
+// step reads (the result of the previous step).  Neither touches what the rows read or write, so
+// these workgroups fill the slots the slab rows leave free instead of taking launches of their own.
+struct TailArgs {
+  GridDev g;
+  PWDev pw;
+  float *xs;                 // sheets of time index nsh (nb_sheets workgroups; 0: none)
+  int xni, xnj, xnk, nsh;
+  unsigned nb_sheets, nb_probes;
+  FieldSet F;                // probes (nb_probes workgroups): record row prow of port
+  const int *pidx, *pcomps;
+  float *port;
+  int nAnt, ncomp, prow, T;
+};
+
+__device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
+  const unsigned nthr = blockDim.x;
+  if (blk < ta.nb_sheets) {
+    const uint32_t ns = 8u * ((uint32_t)ta.xnj * ta.xnk + (uint32_t)ta.xni * ta.xnk + (uint32_t)ta.xni * ta.xnj);
+    const float *rowE = ta.pw.active_h ? ta.pw.Eh + (size_t)ta.nsh * ta.pw.N1 : nullptr;
+    const float *rowH = ta.pw.active_e ? ta.pw.Hh + (size_t)ta.nsh * ta.pw.N1 : nullptr;
+    for (uint32_t t = blk * nthr + threadIdx.x; t < ns; t += ta.nb_sheets * nthr)
+      tfsf_sheet(ta.g, ta.pw, rowE, rowH, ta.xs, ta.xni, ta.xnj, ta.xnk, t);
+    return;
+  }
+  blk -= ta.nb_sheets;
+  const uint32_t np = (uint32_t)ta.nAnt * ta.ncomp;
+  for (uint32_t t = blk * nthr + threadIdx.x; t < np; t += ta.nb_probes * nthr) {
+    const int q = t / ta.nAnt, a = t % ta.nAnt;
+    ta.port[((size_t)q * ta.T + ta.prow) * ta.nAnt + a] = ta.F.f[ta.pcomps[q]][ta.pidx[a]];
+  }
+}
+
+// The row kernel's workgroups [0, nwg); the slab part's launch may carry tail workgroups after them.
 template <bool IDX, bool BUF, int CPL, int PART>
-__global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n) {
-  rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(blockIdx.x, gridDim.x));
+__global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n, int nwg,
+                                                               const TailArgs ta) {
+  const int b = blockIdx.x;
+  if (PART == 1 && b >= nwg) {
+    tail_body(ta, b - nwg);
+    return;
+  }
+  rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(b, nwg));
 }
 // The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
 // (measured 4% faster at 256^3 than the 68 the compiler picks freely).
@@ -1665,28 +1709,36 @@ static unsigned rows_lds(const StepArgs &a, int part) {
 }
 
 template <bool IDX, bool BUF, int PART>
-static void launch_rows_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, unsigned lds, const StepArgs *d, int n) {
+static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
+                          int n, const TailArgs &ta) {
+  const dim3 grid(nwg + ntail);
   if (grid.x == 0) return;
-  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, lds, st, d, n);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, lds, st, d, n);
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, lds, st, d, n, nwg, ta);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, lds, st, d, n, nwg, ta);
   else if (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF>), grid, blk, lds, st, d, n);
-  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, lds, st, d, n);
+  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, lds, st, d, n, nwg, ta);
 }
 
-// part: 0 the interior part, 1 the slab part, -1 both (one after the other)
-static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int part, int n) {
+// part: 0 the interior part, 1 the slab part (with the tail workgroups of ta, if any), -1 both
+// (one after the other)
+static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs *dargs, int part, int n,
+                              const TailArgs *tail = nullptr) {
   // buffer addressing when every array the kernel touches (a state set, a psi set, the material
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
-  const dim3 g0(part == 1 ? 0 : a.nwg0), g1(part == 0 ? 0 : a.nwg1), b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
+  const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
+  TailArgs ta{};
+  if (tail && part != 0) ta = *tail;
+  const unsigned nt = ta.nb_sheets + ta.nb_probes;
+  const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<true, true, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<true, false, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<false, true, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, g0, b0, L0, dargs, n); launch_rows_w<false, false, 1>(st, a.CPL, g1, b1, L1, dargs, n); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
   }
   return hipGetLastError();
 }
@@ -1889,29 +1941,54 @@ hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, 
 
 // One time step: the TF/SF face sheets of step n (when a plane wave is active), then the row
 // kernel over every cell: its interior part, then its slab part.  ev_rows (optional): two events
-// recorded around the interior part (the step's dominant launch).
+// recorded around the interior part (the step's dominant launch).  fold (optional, fdtd_run): the
+// sheets of step n may be filled already, and the slab part's launch carries the sheets of step
+// n + 1 (into fold->xs_next) and the probe gather of the state the step reads (record row
+// fold->row) as tail workgroups.
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
-                             hipEvent_t *ev_rows) {
+                             hipEvent_t *ev_rows, StepFold *fold) {
   SlabArgs sa = slab_args(f, n);
   sa.nbs = 0;
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
   const unsigned nbx = sheets ? cdiv(nsheet, 256) : 0;
-  if (nbx) hipLaunchKernelGGL(k_sheets, dim3(nbx), dim3(256), 0, st, sa);
+  if (nbx && !(fold && fold->sheets_ready)) hipLaunchKernelGGL(k_sheets, dim3(nbx), dim3(256), 0, st, sa);
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[0], st);
     if (e != hipSuccess) return e;
   }
+  const StepArgs &a = *(const StepArgs *)hargs;
   if (nwg > 0) {
-    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, 0, n);
+    const hipError_t e = launch_rows(st, a, (const StepArgs *)dargs, 0, n);
     if (e != hipSuccess) return e;
   }
   if (ev_rows) {
     const hipError_t e = hipEventRecord(ev_rows[1], st);
     if (e != hipSuccess) return e;
   }
+  TailArgs ta{};
+  if (fold) {
+    fold->next_filled = false;
+    const unsigned nthr = 64u * a.S * a.R;
+    if (nwg <= 0) {  // no row launch to carry the tail
+      if (fold->row >= 0)
+        return launch_probes(st, fold->F, fold->idx, fold->nAnt, fold->comps, fold->ncomp, fold->port, fold->row, fold->T);
+      return hipGetLastError();
+    }
+    if (sheets && fold->xs_next) {
+      ta.g = sa.g; ta.pw = sa.pw;
+      ta.xs = fold->xs_next; ta.xni = sa.xni; ta.xnj = sa.xnj; ta.xnk = sa.xnk; ta.nsh = n + 1;
+      ta.nb_sheets = std::min(cdiv(nsheet, nthr), 1024u);
+      fold->next_filled = true;
+    }
+    if (fold->row >= 0 && fold->row < fold->T && fold->nAnt > 0 && fold->ncomp > 0) {
+      ta.F = fold->F; ta.pidx = fold->idx; ta.pcomps = fold->comps; ta.port = fold->port;
+      ta.nAnt = fold->nAnt; ta.ncomp = fold->ncomp; ta.prow = fold->row; ta.T = fold->T;
+      ta.nb_probes = std::min(cdiv((long)fold->nAnt * fold->ncomp, nthr), 64u);
+    }
+  }
   if (nwg > 0) {
-    const hipError_t e = launch_rows(st, *(const StepArgs *)hargs, (const StepArgs *)dargs, 1, n);
+    const hipError_t e = launch_rows(st, a, (const StepArgs *)dargs, 1, n, &ta);
     if (e != hipSuccess) return e;
   }
   return hipGetLastError();

@@ -81,9 +81,22 @@ struct FusedDev {
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.
 size_t step_args_size();
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out);
+// Work folded into the launches of a fused step by a run of consecutive steps: the face sheets of
+// step n filled by the previous step (sheets_ready), the sheets of step n + 1 filled into xs_next
+// by the slab part's launch (next_filled reports whether it did), and the probe gather of the
+// state the step reads into record row `row` (-1: none) by the same launch.
+struct StepFold {
+  bool sheets_ready = false;
+  float *xs_next = nullptr;
+  bool next_filled = false;
+  FieldSet F{};
+  const int *idx = nullptr, *comps = nullptr;
+  float *port = nullptr;
+  int nAnt = 0, ncomp = 0, row = -1, T = 0;
+};
 // ev_rows (optional): two events recorded on st around the row kernel's interior part
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
-                             hipEvent_t *ev_rows = nullptr);
+                             hipEvent_t *ev_rows = nullptr, StepFold *fold = nullptr);
 bool fused_supported(const GridDev &g);
 
 // ---- batched fused step: one launch per kernel covers many grids ----

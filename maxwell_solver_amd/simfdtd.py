@@ -240,8 +240,9 @@ class Simulation:
         self._chk(lib().fdtd_autotune(self._h, int(n), int(mode), int(reps), C.byref(ms)))
         return ms.value
 
-    def set_timing(self, on=True):
-        self._chk(lib().fdtd_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, on=True, every=1):
+        """HIP events around every `every`-th step of the following runs (off: none)."""
+        self._chk(lib().fdtd_set_timing(self._h, int(every) if on else 0))
 
     def kernel_time(self):
         ms, n = C.c_double(), C.c_int64()

@@ -127,6 +127,51 @@ def test_fused_tilings_oblique(I, rows, z_chunk, cpl, inset):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+def test_runs_and_steps_interleaved():
+    """A run fills the next step's face sheets and gathers the previous step's probes inside the
+    row kernel's slab-part launch: runs of one and two steps, single steps between runs, and a run
+    that ends at the last time index keep every record and field equal to the oracle."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    n = 0
+    for kind, count in (("run", 1), ("step", 1), ("run", 2), ("step", 1), ("step", 1), ("run", 7),
+                        ("run", 1), ("step", 1)):
+        if kind == "run":
+            g.run(n, count, 5)
+        else:
+            g.step(n, 5)
+        n += count
+    g.run(n, sc.T - n, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def test_sampled_timing_keeps_parity():
+    """Timing events on every third step of a run: the run still equals the oracle, the timed step
+    count is ceil(count / 3), and the interior part's time lies inside the step's."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_timing(True, every=3)
+    g.run(0, 10, 5)
+    step_ms, launches = g.kernel_time()
+    rows_ms = g.row_kernel_time()
+    assert launches == 4
+    assert 0 < rows_ms < step_ms
+    g.set_timing(False)
+    g.run(10, sc.T - 10, 5)
+    assert g.kernel_time()[1] == 4  # an untimed run leaves the last figures
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def test_step_outside_fused_layouts():
     """Rows longer than 1024 columns are outside the row kernel's layouts: Step and Run take the
     per-op kernels in the reference op order instead, with the same result."""
