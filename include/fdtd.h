@@ -73,7 +73,10 @@ int fdtd_update_e_all(fdtd_sim *s);
 int fdtd_update_h_pml(fdtd_sim *s);
 int fdtd_update_e_pml(fdtd_sim *s);
 /* 'UpdateH_PlaneWave' / 'UpdateE_PlaneWave': time = [n, mode]; modes 1..4 as the reference
- * (1 ZY, 2 ZX, 3 XZ, 4 XY), 5 = oblique; src/simulation.h:330-369, :421-570 */
+ * (1 ZY, 2 ZX, 3 XZ, 4 XY), 5 = oblique; src/simulation.h:330-369, :421-570.  Modes 6 (YX) and
+ * 7 (YZ): the y-propagating modes the reference defines but never dispatches
+ * (src/simulation.h:446-467, 519-544; kernels src/gpu.h:294-422), on the forward 1D line with
+ * node offset N1 - J (the z and x modes use N1 - K and N1 - I). */
 int fdtd_update_h_planewave(fdtd_sim *s, const int32_t time[2]);
 int fdtd_update_e_planewave(fdtd_sim *s, const int32_t time[2]);
 /* 'AddCurrents': time = [n, pol], box = [i0,i1,j0,j1,k0,k1]; src/simulation.h:256-286 (gpu.h:55-82) */
@@ -93,7 +96,7 @@ int fdtd_output_vis(fdtd_sim *s, float *out, uint32_t size, int32_t flag);
 int fdtd_set_cpml(fdtd_sim *s, const double params[6]);
 /* TF/SF box inset from the grid faces (reference 15, src/simulation.h:102); before ReadySimulation */
 int fdtd_set_tfsf(fdtd_sim *s, int32_t inset);
-/* Plane-wave set-up used by Init1D.  mode 1..4: params[0] != 0 makes the 1D line carry the
+/* Plane-wave set-up used by Init1D.  mode 1..4, 6, 7: params[0] != 0 makes the 1D line carry the
  * grid's centre-column material profile (layered media).  mode 5: oblique incidence,
  * params = [theta, phi, psi] (radians), dispersion-matched 1D line. */
 int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);

@@ -109,14 +109,14 @@ struct fdtd_sim {
   float *dE1h = nullptr, *dH1h = nullptr, *dEinv = nullptr, *dHinv = nullptr;
   float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
   float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
-  float *dPwTab[6] = {};      // per plane-wave mode: E tables x,y,z (int, half), then H tables
+  float *dPwTab[8] = {};      // per plane-wave mode (1..7): E tables x,y,z (int, half), then H tables
   float *dXs[2] = {nullptr, nullptr};  // face sheets of the TF/SF box, one buffer per state set (fused step)
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
   std::vector<float *> dSnaps;
   int nsnap = 0;
-  PWDev pwdev[6];
+  PWDev pwdev[8];
   StepTuning tune;
   // device argument blocks of the fused kernel per (state parity, plane-wave mode); rebuilt when
   // args_gen moves (any buffer or tuning change)
@@ -125,7 +125,7 @@ struct fdtd_sim {
     void *dptr = nullptr;
     std::vector<char> host;
   };
-  ArgSlot argslots[2][6];
+  ArgSlot argslots[2][8];
   int args_gen = 0;
   int timing = 0;  // HIP events around every timing-th step of fdtd_run (0: none)
   std::vector<hipEvent_t> ev;
@@ -454,15 +454,16 @@ struct fdtd_sim {
       N1 = obl.N1;
       line_src = obl.src;
       if (int rc = run_line(N1, obl.ds, obl.Ch1, std::vector<float>(N1, 1.0f), std::vector<float>(N1, obl.Cb1), obl.src, pw, &dE1h, &dH1h)) return rc;
-    } else if (pw_layered && pw_mode >= 1 && pw_mode <= 4) {
+    } else if (pw_layered && pw_mode >= 1 && pw_mode != 5 && pw_mode <= 7) {
       // exact restriction of the centre column along the propagation axis
-      const int ax = pw_mode <= 2 ? 2 : 0;
-      const int pol = pw_mode == 1 ? EY : pw_mode == 2 ? EX : pw_mode == 3 ? EZ : EY;
-      const int Q = ax == 2 ? K : I;
+      const int ax = pw_mode <= 2 ? 2 : pw_mode <= 4 ? 0 : 1;
+      const int pol = pw_mode == 1 ? EY : pw_mode == 2 ? EX : pw_mode == 3 ? EZ : pw_mode == 4 ? EY : pw_mode == 6 ? EX : EZ;
+      const int Q = ax == 2 ? K : ax == 0 ? I : J;
       const size_t ic = I / 2, jc = J / 2, kc = K / 2;
       std::vector<float> Ce(Q), Cb(Q);
       for (int q = 0; q < Q; ++q) {
-        const size_t id = ax == 2 ? (ic + jc * I + (size_t)q * IJ) : ((size_t)q + jc * I + kc * IJ);
+        const size_t id = ax == 2 ? (ic + jc * I + (size_t)q * IJ)
+                          : ax == 0 ? ((size_t)q + jc * I + kc * IJ) : (ic + (size_t)q * I + kc * IJ);
         cell_coeff(id, pol, &Ce[q], &Cb[q]);
       }
       N1 = Q;
@@ -499,13 +500,15 @@ struct fdtd_sim {
   }
 
   // Per-mode incident-field tables, one buffer each: E tables (x int, x half, y int, y half, z int,
-  // z half) then the same for H.  Axis modes 1..4 map the propagation coordinate q to line node
-  // q + N1 - Q (src/simulation.h:330-369) and hold zeros on the other axes.
+  // z half) then the same for H.  Axis modes map the propagation coordinate q to line node
+  // q + N1 - Q (src/simulation.h:330-369) and hold zeros on the other axes: 1, 2 along z, 3, 4
+  // along x, 6 (YX) and 7 (YZ) along y, the y-propagating modes the reference defines but never
+  // dispatches (src/simulation.h:446-467, 519-544), with the y extent J in place of K or I.
   int build_pw() {
     args_gen++;
     const size_t tl = 2 * ((size_t)I + J + K);
-    for (int m = 0; m < 6; ++m) pwdev[m] = PWDev();
-    for (int m = 1; m <= 5; ++m) {
+    for (int m = 0; m < 8; ++m) pwdev[m] = PWDev();
+    for (int m = 1; m <= 7; ++m) {
       if (m == 5 && pw_mode != 5) continue;
       std::vector<float> t(2 * tl, 0.0f);
       const int Q[3] = {I, J, K};
@@ -516,7 +519,8 @@ struct fdtd_sim {
             for (int q = 0; q < Q[u]; ++q) {
               float v = 0.0f;
               if (m == 5) v = eh == 0 ? obl.tabE[u][h][q] : obl.tabH[u][h][q];
-              else if ((m <= 2 && u == 2) || (m >= 3 && u == 0)) v = (float)(q + N1 - Q[u]);
+              else if ((m <= 2 && u == 2) || ((m == 3 || m == 4) && u == 0) || (m >= 6 && u == 1))
+                v = (float)(q + N1 - Q[u]);
               t[eh * tl + off[u] + (size_t)h * Q[u] + q] = v;
             }
       if (int rc = upload(&dPwTab[m], t)) return rc;
@@ -535,6 +539,12 @@ struct fdtd_sim {
       if (m == 2) { p.polE[0] = 1; p.polH[1] = -1; p.active_h = (1 << 7) | (1 << 5); p.active_e = (1 << 6) | (1 << 2); }
       if (m == 3) { p.polE[2] = 1; p.polH[1] = 1; p.active_h = (1 << 3) | (1 << 1); p.active_e = (1 << 6) | (1 << 2); }
       if (m == 4) { p.polE[1] = 1; p.polH[2] = -1; p.active_h = (1 << 6) | (1 << 2); p.active_e = (1 << 3) | (1 << 1); }
+      // YX (src/gpu.h:294-356): Hy on the z faces, Hz on the y faces; Ex on the y faces, Ey on
+      // the x faces; the E kernels get +c
+      if (m == 6) { p.polE[0] = 1; p.polH[2] = 1; p.active_h = (1 << 7) | (1 << 5); p.active_e = (1 << 3) | (1 << 1); }
+      // YZ (src/gpu.h:359-422): Hx on the y faces, Hy on the x faces; Ey on the z faces, Ez on
+      // the y faces; the E kernels get -c (updateE_PW_YZ, src/simulation.h:532-544)
+      if (m == 7) { p.polE[2] = 1; p.polH[0] = -1; p.active_h = (1 << 3) | (1 << 1); p.active_e = (1 << 7) | (1 << 5); }
       if (m == 5) {
         for (int c = 0; c < 3; ++c) { p.polE[c] = obl.polE[c]; p.polH[c] = obl.polH[c]; }
         p.active_h = all;
@@ -546,7 +556,7 @@ struct fdtd_sim {
 
   int check_pw(int n, int mode, const PWDev **out) {
     *out = nullptr;
-    if (mode < 1 || mode > 5) return 0;  // the reference ignores other modes
+    if (mode < 1 || mode > 7) return 0;  // the reference ignores other modes
     if (!have_1d) return fail(FDTD_ERR_STATE, "plane wave before Init1D");
     if (n < 0 || n >= T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
     if (!pwdev[mode].active_h) return fail(FDTD_ERR_STATE, "the 1D line for this plane-wave mode was not initialised");
@@ -1188,7 +1198,7 @@ int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *tabl
 int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]) {
   NEED(s);
   if (!s->init_const) return fail(FDTD_ERR_STATE, "SetPlaneWave before SetConst");
-  if (mode < 1 || mode > 5 || !params) return fail(FDTD_ERR_INVALID, "plane-wave mode must be 1..5");
+  if (mode < 1 || mode > 7 || !params) return fail(FDTD_ERR_INVALID, "plane-wave mode must be 1..7");
   s->pw_mode = mode;
   if (mode == 5) {
     s->pw_prm[0] = params[0]; s->pw_prm[1] = params[1]; s->pw_prm[2] = params[2];

@@ -91,7 +91,7 @@ class Scenario:
     fctr: float
     cpml: tuple            # [P, m, kappa_max, alpha_max, alpha_m, sigma_factor]
     inset: int
-    mode: int              # plane-wave mode 1..5
+    mode: int              # plane-wave mode 1..7 (0: none)
     pw_params: tuple       # SetPlaneWave parameters
     source: np.ndarray     # PW source samples [T]
     mat_index: np.ndarray | None = None  # uint8 [K, J, I] (x fastest in memory)
@@ -392,7 +392,7 @@ def setup_host(backend, sc: Scenario, use_arrays: bool = False):
                 backend.update_material(f + 1, arrs[f])
         else:
             backend.set_material_index(sc.mat_index, sc.mat_table)
-    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4) and sc.pw_params and sc.pw_params[0]):
+    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4, 6, 7) and sc.pw_params and sc.pw_params[0]):
         backend.set_plane_wave(sc.mode, sc.pw_params)
 
 
@@ -414,7 +414,7 @@ def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = Tru
                 backend.update_material(f + 1, arrs[f])
         else:
             backend.set_material_index(sc.mat_index, sc.mat_table)
-    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4) and sc.pw_params and sc.pw_params[0]):
+    if sc.mode == 5 or (sc.mode in (1, 2, 3, 4, 6, 7) and sc.pw_params and sc.pw_params[0]):
         backend.set_plane_wave(sc.mode, sc.pw_params)
     backend.ready()
     if init_line and sc.mode:

@@ -375,7 +375,7 @@ struct Sim : Base {
 
   int set_plane_wave(int mode, const double *prm) {
     if (!init_const) { g_err = "simFDTD:InvalidInput|SetPlaneWave before SetConst"; return 2; }
-    if (mode < 1 || mode > 5) { g_err = "simFDTD:InvalidInput|plane-wave mode must be 1..5"; return 1; }
+    if (mode < 1 || mode > 7) { g_err = "simFDTD:InvalidInput|plane-wave mode must be 1..7"; return 1; }
     pw_mode = mode;
     if (mode == 5) {
       pw_theta = prm[0]; pw_phi = prm[1]; pw_psi = prm[2]; pw_layered = 0;
@@ -432,16 +432,16 @@ struct Sim : Base {
   // propagation axis (length Q, so index q maps to q), polarised-component coefficients per node.
   int init_layered(const std::vector<R> &pw) {
     int m = pw_mode;
-    int ax = (m <= 2) ? 2 : 0;                    // modes 1,2 along z; 3,4 along x
-    int pol = (m == 1) ? EY : (m == 2) ? EX : (m == 3) ? EZ : EY;
-    size_t Q = ax == 2 ? K : I;
+    int ax = (m <= 2) ? 2 : (m <= 4) ? 0 : 1;     // modes 1,2 along z; 3,4 along x; 6,7 along y
+    int pol = (m == 1) ? EY : (m == 2) ? EX : (m == 3) ? EZ : (m == 4) ? EY : (m == 6) ? EX : EZ;
+    size_t Q = ax == 2 ? K : ax == 0 ? I : J;
     size_t ic = I / 2, jc = J / 2, kc = K / 2;
     N1 = Q;
     Line L;
     L.N = Q; L.ds = delX; L.Ch = Che;
     L.Ce.assign(Q, 1); L.Cb.assign(Q, 0);
     for (size_t q = 0; q < Q; ++q) {
-      size_t id = ax == 2 ? (ic + jc * I + q * IJ) : (q + jc * I + kc * IJ);
+      size_t id = ax == 2 ? (ic + jc * I + q * IJ) : ax == 0 ? (q + jc * I + kc * IJ) : (ic + q * I + kc * IJ);
       L.Ce[q] = Ce[pol][id];
       L.Cb[q] = Ch[pol][id];
     }
@@ -673,7 +673,7 @@ struct Sim : Base {
   size_t IDX(size_t i, size_t j, size_t k) const { return i + j * I + k * IJ; }
 
   int update_h_pw(int n, int mode) {
-    if (mode < 1 || mode > 5) return 0;
+    if (mode < 1 || mode > 7) return 0;
     if (!have_1d) { g_err = "simFDTD:InvalidInput|plane wave before Init1D"; return 2; }
     if (n < 0 || (size_t)n >= T) { g_err = "simFDTD:InvalidInput|time index outside [0,T)"; return 1; }
     if (mode == 5) return oblique_h((size_t)n);
@@ -725,7 +725,7 @@ struct Sim : Base {
           Hy[a] = Hy[a] - Chye * E[i0] * c;
           Hy[b] = Hy[b] + Chye * E[i1] * c;
         }
-    } else {  // mode 4, updateH_PW_XY: K22 gpu.h:217-231, K23 gpu.h:233-247
+    } else if (mode == 4) {  // updateH_PW_XY: K22 gpu.h:217-231, K23 gpu.h:233-247
       const R *E = Eh.data() + nstep + N1 - I;
       for (size_t y = j0; y <= j1 - 1; ++y)
         for (size_t x = i0; x <= i1; ++x) {
@@ -739,12 +739,40 @@ struct Sim : Base {
           Hz[a] = Hz[a] + Chze * E[i0] * c;
           Hz[b] = Hz[b] - Chze * E[i1] * c;
         }
+    } else if (mode == 6) {  // updateH_PW_YX (src/simulation.h:446-456): gpu.h:294-324
+      const R *E = Eh.data() + nstep + N1 - J;  // the y modes' offset by analogy with :330-369
+      for (size_t y = j0; y <= j1; ++y)
+        for (size_t x = i0; x <= i1 - 1; ++x) {
+          size_t a = IDX(x, y, k0 - 1), b = IDX(x, y, k1);
+          Hy[a] = Hy[a] + Chye * E[y] * c;
+          Hy[b] = Hy[b] - Chye * E[y] * c;
+        }
+      for (size_t z = k0; z <= k1; ++z)
+        for (size_t x = i0; x <= i1 - 1; ++x) {
+          size_t a = IDX(x, j0 - 1, z), b = IDX(x, j1, z);
+          Hz[a] = Hz[a] - Chze * E[j0] * c;
+          Hz[b] = Hz[b] + Chze * E[j1] * c;
+        }
+    } else {  // mode 7, updateH_PW_YZ (src/simulation.h:457-467): gpu.h:359-390
+      const R *E = Eh.data() + nstep + N1 - J;
+      for (size_t z = k0; z <= k1 - 1; ++z)
+        for (size_t x = i0; x <= i1; ++x) {
+          size_t a = IDX(x, j0 - 1, z), b = IDX(x, j1, z);
+          Hx[a] = Hx[a] + Chxe * E[j0] * c;
+          Hx[b] = Hx[b] - Chxe * E[j1] * c;
+        }
+      for (size_t z = k0; z <= k1 - 1; ++z)
+        for (size_t y = j0; y <= j1; ++y) {
+          size_t a = IDX(i0 - 1, y, z), b = IDX(i1, y, z);
+          Hy[a] = Hy[a] - Chye * E[y] * c;
+          Hy[b] = Hy[b] + Chye * E[y] * c;
+        }
     }
     return 0;
   }
 
   int update_e_pw(int n, int mode) {
-    if (mode < 1 || mode > 5) return 0;
+    if (mode < 1 || mode > 7) return 0;
     if (!have_1d) { g_err = "simFDTD:InvalidInput|plane wave before Init1D"; return 2; }
     if (n < 0 || (size_t)n >= T) { g_err = "simFDTD:InvalidInput|time index outside [0,T)"; return 1; }
     if (mode == 5) return oblique_e((size_t)n);
@@ -799,7 +827,7 @@ struct Sim : Base {
           Ez[a] = Ez[a] - Cezh[a] * H[i0 - 1] * c;
           Ez[b] = Ez[b] + Cezh[b] * H[i1] * c;
         }
-    } else {  // mode 4, updateE_PW_XY (kernels get -c): K24 gpu.h:249-275, K25 gpu.h:277-291
+    } else if (mode == 4) {  // updateE_PW_XY (kernels get -c): K24 gpu.h:249-275, K25 gpu.h:277-291
       const R c = -1;
       const R *H = Hh.data() + nstep + N1 - I;
       for (size_t z = k0; z <= k1; ++z)
@@ -813,6 +841,36 @@ struct Sim : Base {
           size_t a = IDX(i0, y, z), b = IDX(i1, y, z);
           Ey[a] = Ey[a] + Ceyh[a] * H[i0 - 1] * c;
           Ey[b] = Ey[b] - Ceyh[b] * H[i1] * c;
+        }
+    } else if (mode == 6) {  // updateE_PW_YX (c = +1, src/simulation.h:519-531): gpu.h:326-356
+      const R c = 1;
+      const R *H = Hh.data() + nstep + N1 - J;
+      for (size_t z = k0; z <= k1; ++z)
+        for (size_t x = i0; x <= i1 - 1; ++x) {
+          size_t a = IDX(x, j0, z), b = IDX(x, j1, z);
+          Ex[a] = Ex[a] - Cexh[a] * H[j0 - 1] * c;
+          Ex[b] = Ex[b] + Cexh[b] * H[j1] * c;
+        }
+      for (size_t z = k0; z <= k1; ++z)
+        for (size_t y = j0; y <= j1 - 1; ++y) {
+          size_t a = IDX(i0, y, z), b = IDX(i1, y, z);
+          Ey[a] = Ey[a] + Ceyh[a] * H[y] * c;
+          Ey[b] = Ey[b] - Ceyh[b] * H[y] * c;
+        }
+    } else {  // mode 7, updateE_PW_YZ (kernels get -c, src/simulation.h:532-544): gpu.h:391-422
+      const R c = -1;
+      const R *H = Hh.data() + nstep + N1 - J;
+      for (size_t y = j0; y <= j1 - 1; ++y)
+        for (size_t x = i0; x <= i1; ++x) {
+          size_t a = IDX(x, y, k0), b = IDX(x, y, k1);
+          Ey[a] = Ey[a] - Ceyh[a] * H[y] * c;
+          Ey[b] = Ey[b] + Ceyh[b] * H[y] * c;
+        }
+      for (size_t z = k0; z <= k1 - 1; ++z)
+        for (size_t x = i0; x <= i1; ++x) {
+          size_t a = IDX(x, j0, z), b = IDX(x, j1, z);
+          Ez[a] = Ez[a] + Cezh[a] * H[j0 - 1] * c;
+          Ez[b] = Ez[b] - Cezh[b] * H[j1] * c;
         }
     }
     return 0;
@@ -958,11 +1016,11 @@ struct Sim : Base {
   int step(int n, int mode) {
     update_h_all();
     int rc = 0;
-    if (mode >= 1 && mode <= 5) rc = update_h_pw(n, mode);
+    if (mode >= 1 && mode <= 7) rc = update_h_pw(n, mode);
     if (rc) return rc;
     update_h_pml();
     update_e_all();
-    if (mode >= 1 && mode <= 5) rc = update_e_pw(n, mode);
+    if (mode >= 1 && mode <= 7) rc = update_e_pw(n, mode);
     if (rc) return rc;
     update_e_pml();
     return 0;

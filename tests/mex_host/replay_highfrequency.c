@@ -120,8 +120,8 @@ int main(int argc, char **argv) {
   {
     mxArray *sz = vec(mxUINT32_CLASS, 1);
     *(unsigned *)mxGetData(sz) = (unsigned)IJK;
-    /* the polarised component: mode 1/4 Ey, mode 2 Ex, mode 3 Ez */
-    const int flag = (mode == 2) ? 1 : (mode == 3) ? 3 : 2;
+    /* the polarised component: mode 1/4 Ey, mode 2/6 Ex, mode 3/7 Ez */
+    const int flag = (mode == 2 || mode == 6) ? 1 : (mode == 3 || mode == 7) ? 3 : 2;
     mxArray *a[2] = {sz, i32s(1, &flag)};
     OK(op("OutputVis", h, 2, a, &out));
   }

@@ -56,9 +56,9 @@ def _per_op_steps(b, T, mode, currents=None):
             b.add_currents(n, currents[0], currents[1])
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 6, 7])
 def test_per_op_reference_sequence(mode):
-    """highfrequency.m:85-91 op sequence, reference CPML preset, modes 1..4."""
+    """highfrequency.m:85-91 op sequence, reference CPML preset, modes 1..4 and the y modes 6, 7."""
     from maxwell_solver_amd import scenarios as S
     sc = _scenario(mode=mode)
     g, o = _backends()
@@ -83,7 +83,7 @@ def test_per_op_gic_currents():
     assert_close(compare_state(g, o), exact=True)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 6, 7])
 @pytest.mark.parametrize("cpml", [REF_CPML, CFG_CPML], ids=["refcpml", "cfgcpml"])
 def test_fused_step_axis_modes(mode, cpml):
     """One fused kernel per step equals the per-op sequence of the oracle."""

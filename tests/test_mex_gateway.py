@@ -24,7 +24,7 @@ def test_gateway_error_identifiers():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [3, 1, 2, 4])
+@pytest.mark.parametrize("mode", [3, 1, 2, 4, 6, 7])
 def test_gateway_replays_highfrequency(tmp_path, mode):
     from maxwell_solver_amd import scenarios as S
     from oracle.oracle import OracleSim
@@ -44,6 +44,6 @@ def test_gateway_replays_highfrequency(tmp_path, mode):
     o.update_coeffs()
     o.reset()
     o.run(0, T, mode)
-    want = o.field({1: "Ey", 2: "Ex", 3: "Ez", 4: "Ey"}[mode]).reshape(-1)
+    want = o.field({1: "Ey", 2: "Ex", 3: "Ez", 4: "Ey", 6: "Ex", 7: "Ez"}[mode]).reshape(-1)
     assert np.abs(want).max() > 1e-3
     np.testing.assert_array_equal(ez, want)

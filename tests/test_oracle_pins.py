@@ -173,9 +173,12 @@ def _axis_run(mode, cpml, T=110, N=48, dtype=np.float64):
     return s
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 6, 7])
 @pytest.mark.parametrize("cpml", [(10, 3, 1, 0.08, 1, 0.8), (10, 3, 5, 0.05, 1, 0.8)], ids=["ref", "cfg"])
 def test_tfsf_axis_modes_are_exact(mode, cpml):
+    """In vacuum the scattered-field region stays zero: every face correction of the mode cancels
+    the incident wave exactly (a wrong sign, node or face would leak).  Modes 6 (YX) and 7 (YZ)
+    are the reference's undispatched y-propagating kernels."""
     s = _axis_run(mode, list(cpml))
     s.run(0, 110, mode)
     sc = s.scalars()
@@ -207,6 +210,27 @@ def test_total_field_equals_line_mode3():
     Hy = s.field("Hy")
     np.testing.assert_allclose(Hy[k0 + 3, j0 + 4, i0:i1], s.line(1)[n - 1][i0 + N - I:i1 + N - I],
                                atol=1e-12 * np.abs(s.line(1)[n - 1]).max())
+
+
+@pytest.mark.parametrize("mode,comp,hcomp,sign", [(6, "Ex", "Hz", 1.0), (7, "Ez", "Hx", -1.0)])
+def test_total_field_equals_line_y_modes(mode, comp, hcomp, sign):
+    """y-propagating modes: inside the TF box the polarised E component equals the 1D E history at
+    node j + N - J, and the H component (at j + 1/2) equals sign x the H history one half-step
+    earlier (YX: H along +z, YZ: H along -x, both waves travelling towards -y)."""
+    s = _axis_run(mode, [10, 3, 5, 0.05, 1, 0.8])
+    n = 95
+    s.run(0, n, mode)
+    sc = s.scalars()
+    N, J = int(sc["N1"]), s.dims[1]
+    i0, j0, j1, k0 = int(sc["i0"]), int(sc["j0"]), int(sc["j1"]), int(sc["k0"])
+    line = s.line(0)[n]
+    got = s.field(comp)[k0 + 3, j0:j1 + 1, i0 + 4]
+    want = line[j0 + N - J:j1 + 1 + N - J]
+    assert np.abs(want).max() > 0.05
+    np.testing.assert_allclose(got, want, atol=1e-12 * np.abs(want).max())
+    hl = s.line(1)[n - 1]
+    np.testing.assert_allclose(s.field(hcomp)[k0 + 3, j0:j1, i0 + 4], sign * hl[j0 + N - J:j1 + N - J],
+                               atol=1e-12 * np.abs(hl).max())
 
 
 def test_line_obeys_textbook_recurrence():
