@@ -100,6 +100,14 @@ int fdtd_set_tfsf(fdtd_sim *s, int32_t inset);
  * grid's centre-column material profile (layered media).  mode 5: oblique incidence,
  * params = [theta, phi, psi] (radians), dispersion-matched 1D line. */
 int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);
+/* 'CreateMu' src/simulation.h:28-68 (gateway src/simFDTD.cu:133-141): per-cell relative
+ * permeabilities (three arrays of I*J*K floats, each > 0).  The H update then uses the per-cell
+ * coefficients of K2 GPU_updateH_Full (src/gpu.h:4-30) instead of K1's scalar ones; the fused step
+ * runs as the per-op sequence.  The reference's initialisation order is fixed: it sets init_mu
+ * only before ReadySimulation but needs den_hx, which ReadySimulation fills (:30-32, :45), so it
+ * never takes effect there; here CreateMu may come any time after SetConst and takes effect at
+ * ReadySimulation, or at once when that has run. */
+int fdtd_create_mu(fdtd_sim *s, const float *mu_x, const float *mu_y, const float *mu_z);
 /* Per-cell uint8 material index into a table of n rows [sigma_x, sigma_y, sigma_z, eps_x, eps_y,
  * eps_z] (n <= 256); replaces the six UpdateMaterial arrays. */
 int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *table, int32_t n);

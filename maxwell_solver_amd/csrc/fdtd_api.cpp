@@ -110,7 +110,9 @@ struct fdtd_sim {
   float *dPrm = nullptr;      // bE cE bH cH, 1/kappa E x,y,z, 1/kappa H x,y,z
   float *dCoefFull = nullptr; // full mode: Ce x,y,z, Ch x,y,z per cell
   float *dPwTab[8] = {};      // per plane-wave mode (1..7): E tables x,y,z (int, half), then H tables
-  float *dXs[2] = {nullptr, nullptr};  // face sheets of the TF/SF box, one buffer per state set (fused step)
+  float *dXs[2] = {nullptr, nullptr};
+  std::vector<float> mu_in[3];               // CreateMu: relative permeabilities as given
+  float *dChmu[3] = {nullptr, nullptr, nullptr};  // their per-cell H coefficients (K2)  // face sheets of the TF/SF box, one buffer per state set (fused step)
   int *dProbeIdx = nullptr, *dProbeComps = nullptr;
   float *dPort = nullptr;
   size_t port_len = 0;
@@ -171,6 +173,7 @@ struct fdtd_sim {
       b += 2 * 8 * (nj * nk + ni * nk + ni * nj) * sizeof(float);  // face sheets, two buffers
     }
     if (init_antenna) b += (probe_comps.size() * (size_t)T * nAnt + nAnt + probe_comps.size()) * sizeof(float);
+    if (!mu_in[0].empty()) b += 3 * IJK * sizeof(float);  // CreateMu coefficients
     return b + ((size_t)1 << 20);  // argument blocks and small tables
   }
   template <typename Tp>
@@ -203,7 +206,32 @@ struct fdtd_sim {
     }
     co.bE = dbE; co.cE = dcE; co.bH = dbH; co.cH = dcH;
     co.Chh = Chh; co.Che = Che; co.dx = delX;
+    for (int c = 0; c < 3; ++c) co.Chmu[c] = dChmu[c];
     return co;
+  }
+
+  // CreateMu coefficients, src/simulation.h:43-61: C = delT / mu0 / mu / (1 + sigma_m delT / 2 /
+  // mu0 / mu) * den_hx with the reference's operand types (float delT, mu, sigma_m, den_hx; double
+  // mu0), stored as float.  den_hx (:861-885) is 1/dx on i < I-1 and 0 on the last x plane; its
+  // kappa factor is applied per axis through the 1/kappa tables, as for den_ex.
+  int upload_mu() {
+    const float sigma_m = 0;
+    std::vector<float> C(IJK);
+    if (!dChmu[0]) {
+      float *blk = nullptr;
+      if (int rc = dalloc(&blk, 3 * IJK)) return rc;
+      for (int c = 0; c < 3; ++c) dChmu[c] = blk + c * IJK;
+      args_gen++;
+    }
+    for (int c = 0; c < 3; ++c) {
+      for (size_t id = 0; id < IJK; ++id) {
+        const float mu = mu_in[c][id];
+        const float dh = ((int)(id % (size_t)I) == I - 1) ? 0.0f : den;
+        C[id] = (float)(delT / kMu0 / mu / (1 + sigma_m * delT / 2 / kMu0 / mu) * dh);
+      }
+      HIPCHK(hipMemcpy(dChmu[c], C.data(), IJK * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return 0;
   }
 
   void ensure_arrays() {
@@ -378,6 +406,8 @@ struct fdtd_sim {
     dIkE[0] = dPrm + 4 * P2; dIkE[1] = dIkE[0] + I; dIkE[2] = dIkE[1] + J;
     dIkH[0] = dIkE[2] + K; dIkH[1] = dIkH[0] + I; dIkH[2] = dIkH[1] + J;
     init_ready = true;
+    if (!mu_in[0].empty())
+      if (int rc = upload_mu()) return rc;
     if (int rc = alloc_probes()) return rc;
     return zero_state();
   }
@@ -598,6 +628,7 @@ struct fdtd_sim {
         for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
       }
       f.Chh = Chh; f.Che = Che; f.dx = delX;
+      f.mu = dChmu[0] != nullptr;
     }
     if (slot.gen != args_gen) {
       slot.host.assign(step_args_size(), 0);
@@ -1177,6 +1208,20 @@ int fdtd_update_material(fdtd_sim *s, int32_t flag, const float *values) {
   s->ensure_arrays();
   std::vector<float> &dst = flag <= 3 ? s->sig[flag - 1] : s->eps[flag - 4];
   std::memcpy(dst.data(), values, s->IJK * sizeof(float));
+  return 0;
+}
+
+int fdtd_create_mu(fdtd_sim *s, const float *mu_x, const float *mu_y, const float *mu_z) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "CreateMu before SetConst");
+  if (!mu_x || !mu_y || !mu_z) return fail(FDTD_ERR_INVALID, "mu_x, mu_y, mu_z required.");
+  const float *m[3] = {mu_x, mu_y, mu_z};
+  for (int c = 0; c < 3; ++c) {
+    for (size_t id = 0; id < s->IJK; ++id)
+      if (!(m[c][id] > 0.0f) || !std::isfinite(m[c][id])) return fail(FDTD_ERR_INVALID, "CreateMu: mu must be positive and finite");
+  }
+  for (int c = 0; c < 3; ++c) s->mu_in[c].assign(m[c], m[c] + s->IJK);
+  if (s->init_ready) return s->upload_mu();
   return 0;
 }
 

@@ -151,7 +151,8 @@ __device__ __forceinline__ void cell_coef(const CoefDev &co, const float *tab, s
 // per-op kernels (in place)
 // ------------------------------------------------------------------------------------------
 
-// K1 GPU_updateH, src/gpu.h:575-602
+// K1 GPU_updateH, src/gpu.h:575-602; with per-cell mu (co.Chmu set) K2 GPU_updateH_Full,
+// src/gpu.h:4-30: per-cell coefficients and no Chxh factor
 __global__ void __launch_bounds__(256) k_h_all(GridDev g, FieldSet F, CoefDev co) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= g.I - 1 || j >= g.J - 1 || k >= g.K - 1) return;
@@ -161,6 +162,12 @@ __global__ void __launch_bounds__(256) k_h_all(GridDev g, FieldSet F, CoefDev co
   float *Hx = F.f[HX], *Hy = F.f[HY], *Hz = F.f[HZ];
   const float ez = Ez[id], ey = Ey[id], ex = Ex[id];
   const float kx = co.ikH[0][i], ky = co.ikH[1][j], kz = co.ikH[2][k];
+  if (co.Chmu[0]) {
+    Hx[id] = Hx[id] + co.Chmu[0][id] * (-(Ez[id + I] - ez) * ky + (Ey[id + IJ] - ey) * kz);
+    Hy[id] = Hy[id] + co.Chmu[1][id] * (-(Ex[id + IJ] - ex) * kz + (Ez[id + 1] - ez) * kx);
+    Hz[id] = Hz[id] + co.Chmu[2][id] * (-(Ey[id + 1] - ey) * kx + (Ex[id + I] - ex) * ky);
+    return;
+  }
   Hx[id] = co.Chh * Hx[id] + co.Che * (-(Ez[id + I] - ez) * ky + (Ey[id + IJ] - ey) * kz);
   Hy[id] = co.Chh * Hy[id] + co.Che * (-(Ex[id + IJ] - ex) * kz + (Ez[id + 1] - ez) * kx);
   Hz[id] = co.Chh * Hz[id] + co.Che * (-(Ey[id + 1] - ey) * kx + (Ex[id + I] - ex) * ky);
@@ -1750,7 +1757,7 @@ size_t step_args_size() { return sizeof(StepArgs); }
 // not supported by the fused step.
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const GridDev &g = f.g;
-  if (!fused_supported(g)) return -1;
+  if (!fused_supported(g) || f.mu) return -1;
   if (f.mat && f.ntab > 256) return -1;
   int cpl = 1, S = 1;
   if (!row_layout(g.I, tune.cpl, cpl, S)) return -1;

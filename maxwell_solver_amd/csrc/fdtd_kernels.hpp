@@ -20,6 +20,7 @@ struct CoefDev {
   const float *Ch[3] = {nullptr, nullptr, nullptr};
   const float *ikE[3] = {nullptr, nullptr, nullptr}, *ikH[3] = {nullptr, nullptr, nullptr};
   const float *bE = nullptr, *cE = nullptr, *bH = nullptr, *cH = nullptr;
+  const float *Chmu[3] = {nullptr, nullptr, nullptr};  // per-cell mu (CreateMu): K2 H coefficients
   float Chh = 1, Che = 0, dx = 0;
 };
 
@@ -76,6 +77,7 @@ struct FusedDev {
   int N1 = 0, act_h = 0, act_e = 0;
   float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
   float Chh = 1, Che = 0, dx = 0;
+  bool mu = false;  // per-cell mu active: outside the fused step (the per-op kernels run instead)
 };
 // Argument block of the fused kernel: built on the host (make_step_args returns the workgroup
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.

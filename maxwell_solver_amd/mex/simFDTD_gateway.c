@@ -81,6 +81,10 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
   } else if (strcmp(op, "setantennalocations") == 0) {
     need(nrhs, 3, "Locations required.");
     check(fdtd_set_antenna_locations(s, (const int32_t *)mxGetData(prhs[2])));
+  } else if (strcmp(op, "createmu") == 0) {
+    need(nrhs, 5, "mu_x, mu_y, mu_z required.");
+    check(fdtd_create_mu(s, (const float *)mxGetData(prhs[2]), (const float *)mxGetData(prhs[3]),
+                         (const float *)mxGetData(prhs[4])));
   } else if (strcmp(op, "init1d") == 0) {
     need(nrhs, 4, "rate and PW_source required.");
     check(fdtd_init_1d(s, *(const float *)mxGetData(prhs[2]), (const float *)mxGetData(prhs[3])));

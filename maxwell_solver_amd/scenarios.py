@@ -396,9 +396,10 @@ def setup_host(backend, sc: Scenario, use_arrays: bool = False):
         backend.set_plane_wave(sc.mode, sc.pw_params)
 
 
-def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = True):
+def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = True, mu=None):
     """Apply the op sequence of highfrequency.m:63-84 (plus the extensions a workload needs) to a
-    backend exposing the op methods (HIP Simulation or CPU oracle)."""
+    backend exposing the op methods (HIP Simulation or CPU oracle).  mu: three per-cell relative
+    permeability arrays given by CreateMu before ReadySimulation."""
     nAnt = 0 if sc.probes is None else len(sc.probes)
     backend.set_const([sc.fctr, sc.dx, sc.dt], [sc.T, sc.I, sc.J, sc.K, nAnt])
     backend.set_cpml(list(sc.cpml))
@@ -416,6 +417,8 @@ def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = Tru
             backend.set_material_index(sc.mat_index, sc.mat_table)
     if sc.mode == 5 or (sc.mode in (1, 2, 3, 4, 6, 7) and sc.pw_params and sc.pw_params[0]):
         backend.set_plane_wave(sc.mode, sc.pw_params)
+    if mu is not None:
+        backend.create_mu(*mu)
     backend.ready()
     if init_line and sc.mode:
         backend.init_1d(1.0, sc.source)

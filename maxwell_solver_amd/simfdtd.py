@@ -19,7 +19,7 @@ _lib = None
 # Every symbol the header declares (include/fdtd.h).
 EXPORTS = [
     "fdtd_last_error", "fdtd_last_error_id", "fdtd_version", "fdtd_create", "fdtd_destroy",
-    "fdtd_set_const", "fdtd_set_source", "fdtd_set_antenna_locations", "fdtd_update_material",
+    "fdtd_set_const", "fdtd_set_source", "fdtd_set_antenna_locations", "fdtd_update_material", "fdtd_create_mu",
     "fdtd_ready", "fdtd_init_1d", "fdtd_update_coeffs", "fdtd_reset_fields", "fdtd_update_h_all",
     "fdtd_update_e_all", "fdtd_update_h_pml", "fdtd_update_e_pml", "fdtd_update_h_planewave",
     "fdtd_update_e_planewave", "fdtd_add_currents", "fdtd_output_ant", "fdtd_output_vis",
@@ -53,7 +53,7 @@ def lib():
         sig = {
             "fdtd_create": [C.POINTER(vp)], "fdtd_destroy": [vp],
             "fdtd_set_const": [vp, fp, ip], "fdtd_set_source": [vp, fp, i32],
-            "fdtd_set_antenna_locations": [vp, ip], "fdtd_update_material": [vp, i32, fp],
+            "fdtd_set_antenna_locations": [vp, ip], "fdtd_update_material": [vp, i32, fp], "fdtd_create_mu": [vp, fp, fp, fp],
             "fdtd_ready": [vp], "fdtd_init_1d": [vp, C.c_float, fp], "fdtd_update_coeffs": [vp],
             "fdtd_reset_fields": [vp], "fdtd_update_h_all": [vp], "fdtd_update_e_all": [vp],
             "fdtd_update_h_pml": [vp], "fdtd_update_e_pml": [vp],
@@ -170,6 +170,11 @@ class Simulation:
     def update_material(self, flag, values):
         v = _f32(values)
         self._chk(lib().fdtd_update_material(self._h, int(flag), _p(v, C.c_float)))
+
+    def create_mu(self, mu_x, mu_y, mu_z):
+        """CreateMu: per-cell relative permeabilities (three arrays of I*J*K floats)."""
+        m = [_f32(v) for v in (mu_x, mu_y, mu_z)]
+        self._chk(lib().fdtd_create_mu(self._h, *(_p(v, C.c_float) for v in m)))
 
     def set_material_index(self, index, table):
         idx = np.ascontiguousarray(np.asarray(index, dtype=np.uint8).reshape(-1))
@@ -477,6 +482,9 @@ def simFDTD(op: str, *args):
     elif o == "setantennalocations":
         need(1, "Locations required.")
         s.set_antenna_locations(a[0])
+    elif o == "createmu":
+        need(3, "mu_x, mu_y, mu_z required.")
+        s.create_mu(a[0], a[1], a[2])
     elif o == "init1d":
         need(2, "rate and PW_source required.")
         s.init_1d(float(np.asarray(a[0]).reshape(-1)[0]), a[1])

@@ -77,6 +77,10 @@ struct Sim : Base {
   std::vector<R> Ce[3], Ch[3];            // Cexe/Ceye/Ceze, Cexh/Ceyh/Cezh per cell
   std::vector<R> den_ex;                  // [I] (den_ex depends on i only)
   R Chh = 1, Che = 0;
+  // per-cell mu (createMu, src/simulation.h:28-68): relative permeabilities as given, and the
+  // per-cell H coefficients Chxe/Chye/Chze of K2 (empty until both mu and den_hx exist)
+  std::vector<float> mu_in[3];
+  std::vector<R> Chmu[3];
   std::vector<R> F[6];                    // Ex Ey Ez Hx Hy Hz
   std::vector<R> psi[12];
 
@@ -156,6 +160,34 @@ struct Sim : Base {
     else return 0;  // the reference silently ignores other flags
     for (size_t i = 0; i < IJK; ++i) (*dst)[i] = v[i];
     return 0;
+  }
+
+  // CreateMu, src/simulation.h:28-68, with its initialisation order fixed: the reference sets
+  // init_mu only when ReadySimulation has not run (:30-32) but reads den_hx, which ReadySimulation
+  // fills (:861-885), so neither order works there.  Here mu may be given any time after SetConst;
+  // the coefficients are computed once den_hx exists (now, or at ReadySimulation).
+  int create_mu(const float *mx, const float *my, const float *mz) {
+    if (!init_const) { g_err = "simFDTD:InvalidInput|CreateMu before SetConst"; return 2; }
+    const float *m[3] = {mx, my, mz};
+    for (int c = 0; c < 3; ++c) mu_in[c].assign(m[c], m[c] + IJK);
+    if (init_ready) mu_coefficients();
+    return 0;
+  }
+
+  // src/simulation.h:43-61: C = delT / mu0 / mu / (1 + sigma_m delT / 2 / mu0 / mu) * den_hx with
+  // the reference's operand types (delT, mu, sigma_m, den_hx float; mu0 double), stored as float.
+  // den_hx (:861-885) is 1/dx on i < I-1 and 0 on the last x plane; its kappa factor is applied
+  // per axis through ikH as for den_ex (equal at kappa_max = 1).
+  void mu_coefficients() {
+    const R sigma_m = 0;  // :914
+    for (int c = 0; c < 3; ++c) {
+      Chmu[c].assign(IJK, 0);
+      for (size_t id = 0; id < IJK; ++id) {
+        const R mu = (R)mu_in[c][id];
+        const R den = den_ex[id % I];
+        Chmu[c][id] = (R)(delT / kMu0 / mu / (1 + sigma_m * delT / 2 / kMu0 / mu) * den);
+      }
+    }
   }
 
   // src/simulation.h:1068-1095
@@ -299,6 +331,7 @@ struct Sim : Base {
     Chh = (1 - sigma_m * delT / 2 / kMu0) / (1 + sigma_m * delT / 2 / kMu0);
     Che = delT / kMu0 / (1 + sigma_m * delT / 2 / kMu0) / delX;
     coefficients();
+    if (!mu_in[0].empty()) mu_coefficients();
     // initField, src/simulation.h:980-1045
     for (int c = 0; c < 6; ++c) F[c].assign(IJK, 0);
     size_t sx = 2 * P * J * K, sy = 2 * I * P * K, sz = 2 * I * J * P;
@@ -503,6 +536,23 @@ struct Sim : Base {
     R *Hx = F[HX].data(), *Hy = F[HY].data(), *Hz = F[HZ].data();
     const R Chxh = Chh, Chxe = Che;
     const R *ikx = ikH[0].data(), *iky = ikH[1].data(), *ikz = ikH[2].data();
+    if (!Chmu[0].empty()) {
+      // K2 GPU_updateH_Full, src/gpu.h:4-30: per-cell Chxe/Chye/Chze, no Chxh factor
+      const R *Cx = Chmu[0].data(), *Cy = Chmu[1].data(), *Cz = Chmu[2].data();
+#pragma omp parallel for schedule(static)
+      for (long kk = 0; kk < (long)K - 1; ++kk) {
+        size_t k = (size_t)kk;
+        for (size_t j = 0; j + 1 < J; ++j)
+          for (size_t i = 0; i + 1 < I; ++i) {
+            size_t id = i + j * I + k * IJ;
+            R ez = Ez[id], ey = Ey[id], ex = Ex[id];
+            Hx[id] = Hx[id] + Cx[id] * (-(Ez[id + I] - ez) * iky[j] + (Ey[id + IJ] - ey) * ikz[k]);
+            Hy[id] = Hy[id] + Cy[id] * (-(Ex[id + IJ] - ex) * ikz[k] + (Ez[id + 1] - ez) * ikx[i]);
+            Hz[id] = Hz[id] + Cz[id] * (-(Ey[id + 1] - ey) * ikx[i] + (Ex[id + I] - ex) * iky[j]);
+          }
+      }
+      return;
+    }
 #pragma omp parallel for schedule(static)
     for (long kk = 0; kk < (long)K - 1; ++kk) {
       size_t k = (size_t)kk;
@@ -1078,6 +1128,7 @@ int orc_set_source(void *h, const float *src, int update) { DISPATCH(h, s->set_s
 int orc_set_antenna(void *h, const int32_t *loc) { DISPATCH(h, s->set_antenna(loc)); }
 int orc_set_probe_comps(void *h, const int32_t *c, int n) { DISPATCH(h, s->set_probe_comps(c, n)); }
 int orc_update_material(void *h, int32_t flag, const float *v) { DISPATCH(h, s->update_material(flag, v)); }
+int orc_create_mu(void *h, const float *mx, const float *my, const float *mz) { DISPATCH(h, s->create_mu(mx, my, mz)); }
 int orc_ready(void *h) { DISPATCH(h, s->ready()); }
 int orc_set_plane_wave(void *h, int32_t mode, const double *prm) { DISPATCH(h, s->set_plane_wave(mode, prm)); }
 int orc_init_1d(void *h, float rate, const float *pw) { DISPATCH(h, s->init_1d(rate, pw)); }

@@ -51,6 +51,7 @@ def lib():
             "orc_set_antenna": [vp, i32p],
             "orc_set_probe_comps": [vp, i32p, C.c_int],
             "orc_update_material": [vp, i32, f32p],
+            "orc_create_mu": [vp, f32p, f32p, f32p],
             "orc_ready": [vp],
             "orc_set_plane_wave": [vp, i32, f64p],
             "orc_init_1d": [vp, C.c_float, f32p],
@@ -158,6 +159,10 @@ class OracleSim:
     def update_material(self, flag, values):
         v = _f32(values)
         self._chk(lib().orc_update_material(self._h, int(flag), _p(v, C.c_float)))
+
+    def create_mu(self, mu_x, mu_y, mu_z):
+        m = [_f32(v) for v in (mu_x, mu_y, mu_z)]
+        self._chk(lib().orc_create_mu(self._h, *(_p(v, C.c_float) for v in m)))
 
     def ready(self):
         self._chk(lib().orc_ready(self._h))

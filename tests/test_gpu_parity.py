@@ -70,6 +70,54 @@ def test_per_op_reference_sequence(mode):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+def _mu(sc, seed):
+    rng = np.random.default_rng(seed)
+    n = sc.I * sc.J * sc.K
+    return [rng.uniform(1.0, 4.0, n).astype(np.float32) for _ in range(3)]
+
+
+@pytest.mark.parametrize("mode", [3, 6])
+def test_per_op_with_mu(mode):
+    """CreateMu (per-cell mu, K2 GPU_updateH_Full) in the reference op sequence, with spheres and a
+    plane wave; CreateMu given before ReadySimulation on the GPU and after it on the oracle (the
+    fixed initialisation order makes both take effect)."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=mode, cpml=CFG_CPML)
+    mu = _mu(sc, 11)
+    g, o = _backends()
+    S.setup(g, sc, mu=mu)  # before ReadySimulation
+    S.setup(o, sc)
+    o.create_mu(*mu)  # after ReadySimulation
+    _per_op_steps(g, sc.T, mode)
+    _per_op_steps(o, sc.T, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def test_fused_run_with_mu_takes_per_op_path():
+    """With per-cell mu the fused entry points (Run, Step, and a batch) run the per-op kernels in
+    the reference order: same bits as the oracle; CreateMu after ReadySimulation on the GPU."""
+    from maxwell_solver_amd import Batch
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=48, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5))
+    mu = _mu(sc, 12)
+    g, o = _backends()
+    S.setup(g, sc)
+    g.create_mu(*mu)
+    S.setup(o, sc, mu=mu)
+    g.run(0, 20, 5)
+    g.step(20, 5)
+    o.run(0, 21, 5)
+    assert_close(compare_state(g, o), exact=True)
+    b = Batch()
+    b.add(g, 21, sc.T - 21, 5)
+    b.run()
+    g.synchronize()
+    o.run(21, sc.T - 21, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def test_per_op_gic_currents():
     """GICsimulation.m:119-124 order (no plane wave, AddCurrents after UpdateE_PML)."""
     from maxwell_solver_amd import scenarios as S

@@ -138,6 +138,97 @@ def test_e_update_is_discretely_divergence_free():
     assert np.abs(div[5:-5, 5:-5, 5:-5]).max() <= 1e-13 * scale
 
 
+def _random_mu(I, J, K, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.uniform(1.0, 5.0, I * J * K).astype(np.float32) for _ in range(3)]
+
+
+def test_h_update_with_mu_keeps_div_mu_h_zero():
+    """K2 (per-cell mu, CreateMu): mu0 mu_c H_c = -(dt/dx) curl_c E per component and cell, so the
+    discrete divergence of (mu H) stays zero.  Three different random mu arrays: a mu component
+    applied to the wrong H component or cell breaks it."""
+    I, J, K = 23, 19, 29
+    s = _sim(I, J, K, cpml=[4, 3, 5, 0.05, 1, 0.8], inset=5)
+    mu = _random_mu(I, J, K, 3)
+    s.create_mu(*mu)
+    s.ready()
+    _interior_random(s, ["Ex", "Ey", "Ez"], 6, 1)
+    s.update_h_all()
+    s.update_h_pml()
+    mx, my, mz = (m.reshape(K, J, I).astype(np.float64) for m in mu)
+    Bx, By, Bz = mx * s.field("Hx"), my * s.field("Hy"), mz * s.field("Hz")
+    div = (Bx[:-1, :-1, 1:] - Bx[:-1, :-1, :-1]) + (By[:-1, 1:, :-1] - By[:-1, :-1, :-1]) + (Bz[1:, :-1, :-1] - Bz[:-1, :-1, :-1])
+    scale = max(np.abs(Bx).max(), np.abs(By).max(), np.abs(Bz).max())
+    assert scale > 0
+    assert np.abs(div[5:-5, 5:-5, 5:-5]).max() <= 1e-13 * scale
+    # without mu the same E gives a different H (the per-cell path is in use)
+    s2 = _sim(I, J, K, cpml=[4, 3, 5, 0.05, 1, 0.8], inset=5)
+    s2.ready()
+    _interior_random(s2, ["Ex", "Ey", "Ez"], 6, 1)
+    s2.update_h_all()
+    assert np.abs(s2.field("Hx") - s.field("Hx")).max() > 0.1 * np.abs(s2.field("Hx")).max()
+
+
+def test_yee_energy_conserved_with_mu():
+    """Per-cell mu: the discrete energy eps0 |E^n|^2 + mu0 sum mu_c H_c^{n-1/2} H_c^{n+1/2} stays
+    constant (the mu-weighted H inner product is the one the K2 update conserves)."""
+    N = 48
+    s = _sim(N, N, N, cpml=[4, 3, 5, 0.05, 1, 0.8])
+    mu = _random_mu(N, N, N, 5)
+    s.create_mu(*mu)
+    s.ready()
+    mus = [m.reshape(N, N, N).astype(np.float64) for m in mu]
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    blob = np.exp(-((x - 24) ** 2 + (y - 23) ** 2 + (z - 25) ** 2) / 4.0)
+    s.set_field("Ex", blob)
+    s.set_field("Ez", 0.5 * np.roll(blob, 1, axis=2))
+    energies = []
+    for n in range(20):
+        E2 = sum(np.sum(s.field(c) ** 2) for c in ("Ex", "Ey", "Ez"))
+        Hm = [s.field(c).copy() for c in ("Hx", "Hy", "Hz")]
+        s.update_h_all()
+        s.update_h_pml()
+        Hp = [s.field(c) for c in ("Hx", "Hy", "Hz")]
+        energies.append(EPS0 * E2 + MU0 * sum(np.sum(m * a * b) for m, a, b in zip(mus, Hm, Hp)))
+        s.update_e_all()
+        s.update_e_pml()
+    energies = np.array(energies)
+    assert np.abs(energies / energies[0] - 1).max() < 1e-12
+
+
+def test_create_mu_before_or_after_ready():
+    """The fixed initialisation order: CreateMu before or after ReadySimulation gives the same
+    coefficients; mu = 1 reproduces the scalar-coefficient path to within float rounding."""
+    I, J, K = 30, 28, 26
+    mu = _random_mu(I, J, K, 9)
+    runs = []
+    for before in (True, False):
+        s = _sim(I, J, K, dtype=np.float32, cpml=[6, 3, 1, 0.08, 1, 0.8], inset=8)
+        if before:
+            s.create_mu(*mu)
+        s.ready()
+        if not before:
+            s.create_mu(*mu)
+        _interior_random(s, ["Ex", "Ey", "Ez"], 7, 4)
+        for _ in range(5):
+            s.update_h_all(); s.update_h_pml(); s.update_e_all(); s.update_e_pml()
+        runs.append([s.field(c) for c in ("Ex", "Hy")])
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
+    ones = [np.ones(I * J * K, np.float32)] * 3
+    out = []
+    for with_mu in (True, False):
+        s = _sim(I, J, K, dtype=np.float32, cpml=[6, 3, 1, 0.08, 1, 0.8], inset=8)
+        if with_mu:
+            s.create_mu(*ones)
+        s.ready()
+        _interior_random(s, ["Ex", "Ey", "Ez"], 7, 4)
+        for _ in range(5):
+            s.update_h_all(); s.update_h_pml(); s.update_e_all(); s.update_e_pml()
+        out.append(s.field("Hz"))
+    np.testing.assert_allclose(out[0], out[1], rtol=0, atol=1e-6 * np.abs(out[1]).max())
+
+
 def test_yee_energy_conserved_without_loss():
     """A Gaussian E blob in vacuum keeps the discrete energy eps0 |E^n|^2 + mu0 H^{n-1/2}.H^{n+1/2}
     constant while it stays clear of the CPML slabs."""
