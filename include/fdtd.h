@@ -108,6 +108,16 @@ int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);
  * never takes effect there; here CreateMu may come any time after SetConst and takes effect at
  * ReadySimulation, or at once when that has run. */
 int fdtd_create_mu(fdtd_sim *s, const float *mu_x, const float *mu_y, const float *mu_z);
+/* Step currents: after every step of fdtd_step / fdtd_run, AddCurrents(n, pol, box) with the
+ * sample source[n] of SetSource -- the loop of GICsimulation.m:119-124 (UpdateH_All, UpdateH_PML,
+ * UpdateE_All, UpdateE_PML, AddCurrents).  pol 0 turns them off; box as for 'AddCurrents'.
+ * Probes sample a step before its currents, as in that loop. */
+int fdtd_set_step_currents(fdtd_sim *s, int32_t pol, const int32_t box[6]);
+/* Graph replay: fdtd_run records `steps` consecutive fused steps (with their step currents) into
+ * one HIP graph and replays it for runs without a plane wave, probes, snapshots or timing (every
+ * launch of such a step is the same, the current sample is read on the device); other steps run
+ * one by one.  steps: an even count, 0 = off (the default). */
+int fdtd_set_graph_steps(fdtd_sim *s, int32_t steps);
 /* Per-cell uint8 material index into a table of n rows [sigma_x, sigma_y, sigma_z, eps_x, eps_y,
  * eps_z] (n <= 256); replaces the six UpdateMaterial arrays. */
 int fdtd_set_material_index(fdtd_sim *s, const uint8_t *index, const float *table, int32_t n);

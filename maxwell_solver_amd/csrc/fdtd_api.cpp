@@ -135,9 +135,22 @@ struct fdtd_sim {
   int64_t kernel_launches = 0;
   std::vector<void *> allocs;
   size_t dev_bytes = 0;  // device bytes allocated so far (snapshots excluded)
+  // Step currents: AddCurrents(n, pol, box) after every step of fdtd_step / fdtd_run, the loop of
+  // GICsimulation.m:119-124 (pol 0: none).
+  int step_pol = 0, step_box[6] = {0, 0, 0, 0, 0, 0};
+  int currents_gen = 0, source_gen = 0;
+  // Graph replay of runs (fdtd_set_graph_steps): graph_steps steps recorded once into a HIP graph
+  // and replayed; the step currents of recorded launches read their sample from dSrc[*dStepN + q].
+  int graph_steps = 0;
+  float *dSrc = nullptr;
+  int dsrc_gen = -1;
+  int *dStepN = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gx_cur = -1, gx_args = -1, gx_steps = 0, gx_currents = -1;
 
   ~fdtd_sim() {
     if (stream) hipStreamSynchronize(stream);
+    if (gexec) hipGraphExecDestroy(gexec);
     for (void *p : allocs) hipFree(p);
     for (float *p : dSnaps) hipFree(p);
     for (hipEvent_t e : ev) hipEventDestroy(e);
@@ -660,6 +673,13 @@ struct fdtd_sim {
     return 0;
   }
 
+  int step_currents(int n) {
+    if (!step_pol) return 0;
+    if (n < 0 || n >= T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
+    HIPCHK(launch_add_currents(stream, g, fields(cur), coefs(), step_pol - 1, step_box, source[n], delX));
+    return 0;
+  }
+
   // One fused time step.  dry: launch the kernels (they read the current set and write the next)
   // but keep the current set and the probe recorder: the state is unchanged (used for timing).
   // rf (a run of steps): fold the next step's face sheets and this state's probe gather into the
@@ -690,7 +710,8 @@ struct fdtd_sim {
       if (pw) HIPCHK(launch_pw_e(stream, g, fields(cur), coefs(), *pw, n));
       HIPCHK(launch_update_e_pml(stream, g, fields(cur), psis(cur), coefs()));
       if (timed) HIPCHK(hipEventRecord(ev[(*ev_slot)++], stream));
-      return record_probes(cur);
+      if (int rc = record_probes(cur)) return rc;
+      return step_currents(n);
     }
     // timed: events before the step, around the row kernel, after the step
     hipEvent_t *e = timed ? &ev[*ev_slot] : nullptr;
@@ -703,7 +724,8 @@ struct fdtd_sim {
       if (timed) HIPCHK(hipEventRecord(e[3], stream));
       if (dry) return 0;
       cur = 1 - cur;
-      return record_probes(cur);
+      if (int rc = record_probes(cur)) return rc;
+      return step_currents(n);
     }
     StepFold fo;
     fo.sheets_ready = rf->sheets_n == n && rf->sheets_set == cur;
@@ -720,6 +742,73 @@ struct fdtd_sim {
     cur = 1 - cur;
     rf->probe_row = dPort && rec_step < T ? rec_step : -1;
     rec_step++;
+    if (step_pol) {  // the probes sample the step's E before its currents (as UpdateE_PML does)
+      if (int rc = flush_probes(*rf)) return rc;
+      return step_currents(n);
+    }
+    return 0;
+  }
+
+  // Steps [n0, n0 + count) as replays of one recorded graph of graph_steps steps (an even count, so
+  // a replay ends on the set it starts from), where every launch of a step is the same for all
+  // steps: no plane wave, probes, snapshots or timing, a grid the fused step covers.  Returns the
+  // steps run (a multiple of graph_steps), 0 when the run does not qualify.
+  int run_graphs(int n0, int count, int mode, int *done) {
+    *done = 0;
+    const int G = graph_steps;
+    if (G < 2 || count < G || dPort || timing) return 0;
+    if (step_pol && (n0 < 0 || n0 + count > T)) return 0;  // the step loop reports the bad index
+    const PWDev *pw = nullptr;
+    if (int rc = check_pw(n0, mode, &pw)) return rc;
+    if (pw) return 0;
+    FusedDev f[2];
+    ArgSlot *slot[2];
+    for (int w = 0; w < 2; ++w) {  // argument blocks uploaded before the recording
+      if (int rc = fused_slot(w, 0, nullptr, f[w], &slot[w])) return rc;
+      if (slot[w]->nwg < 0) return 0;
+    }
+    if (step_pol && dsrc_gen != source_gen) {
+      if (!dSrc && dalloc(&dSrc, (size_t)T)) return FDTD_ERR_NOMEM;
+      HIPCHK(hipMemcpy(dSrc, source.data(), (size_t)T * sizeof(float), hipMemcpyHostToDevice));
+      dsrc_gen = source_gen;
+    }
+    if (!dStepN && dalloc(&dStepN, 1)) return FDTD_ERR_NOMEM;
+    if (!gexec || gx_cur != cur || gx_args != args_gen || gx_steps != G || gx_currents != currents_gen) {
+      if (gexec) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipGraphExecDestroy(gexec));
+        gexec = nullptr;
+      }
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int c = cur;
+      hipError_t e = hipSuccess;
+      for (int q = 0; q < G && e == hipSuccess; ++q) {
+        e = launch_step_fused(stream, f[c], slot[c]->dptr, slot[c]->host.data(), slot[c]->nwg, n0 + q);
+        c = 1 - c;
+        if (e == hipSuccess && step_pol)
+          e = launch_add_currents(stream, g, fields(c), coefs(), step_pol - 1, step_box, 0.0f, delX, dSrc, dStepN, q);
+      }
+      if (e == hipSuccess) e = launch_advance(stream, dStepN, G);
+      hipGraph_t graph = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(stream, &graph);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return fail(FDTD_ERR_DEVICE, std::string("graph recording failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      const hipError_t e3 = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e3 != hipSuccess) {
+        gexec = nullptr;
+        return fail(FDTD_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(e3));
+      }
+      gx_cur = cur; gx_args = args_gen; gx_steps = G; gx_currents = currents_gen;
+    }
+    const int first = n0;
+    HIPCHK(hipMemcpy(dStepN, &first, sizeof(int), hipMemcpyHostToDevice));
+    int q = 0;
+    for (; q + G <= count; q += G) HIPCHK(hipGraphLaunch(gexec, stream));
+    rec_step += q;
+    *done = q;
     return 0;
   }
 };
@@ -841,7 +930,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       FusedDev f;
       fdtd_sim::ArgSlot *slot = nullptr;
       if (int rc = s->fused_slot(w, it.mode, pw, f, &slot)) return rc;
-      if (slot->nwg < 0) {
+      if (slot->nwg < 0 || s->step_pol) {  // per-op layouts and step currents run on their own
         ok = false;
         break;
       }
@@ -1174,6 +1263,26 @@ int fdtd_set_source(fdtd_sim *s, const float *src, int32_t update) {
   if (update ? !s->init_source : s->init_source) return 0;  // src/simulation.h:1070-1074, :1085-1089
   s->source.assign(src, src + s->T);
   s->init_source = true;
+  s->source_gen++;
+  return 0;
+}
+
+int fdtd_set_step_currents(fdtd_sim *s, int32_t pol, const int32_t box[6]) {
+  NEED(s);
+  if (pol < 0 || pol > 3) return fail(FDTD_ERR_INVALID, "step currents: pol must be 0..3");
+  if (pol && !box) return fail(FDTD_ERR_INVALID, "step currents: box required");
+  if (pol && !s->init_source) return fail(FDTD_ERR_STATE, "step currents before SetSource");
+  s->step_pol = pol;
+  for (int q = 0; q < 6; ++q) s->step_box[q] = pol ? box[q] : 0;
+  if (pol && (box[1] < box[0] || box[3] < box[2] || box[5] < box[4])) s->step_pol = 0;  // src/gpu.h:60-61
+  s->currents_gen++;
+  return 0;
+}
+
+int fdtd_set_graph_steps(fdtd_sim *s, int32_t steps) {
+  NEED(s);
+  if (steps < 0 || steps % 2) return fail(FDTD_ERR_INVALID, "graph steps must be an even count (0: off)");
+  s->graph_steps = steps;
   return 0;
 }
 
@@ -1565,9 +1674,11 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
       s->ev.push_back(e);
     }
   }
-  int slot = 0;
+  int slot = 0, q0 = 0;
+  if (snap_every <= 0)
+    if (int rc = s->run_graphs(n0, count, mode, &q0)) return rc;
   fdtd_sim::RunFold rf;
-  for (int q = 0; q < count; ++q) {
+  for (int q = q0; q < count; ++q) {
     if (int rc = s->step(n0 + q, mode, every > 0 && q % every == 0, &slot, false, &rf)) {
       s->flush_probes(rf);
       return rc;

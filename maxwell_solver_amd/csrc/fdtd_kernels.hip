@@ -348,16 +348,25 @@ __global__ void __launch_bounds__(256) k_pw_e(GridDev g, FieldSet F, CoefDev co,
 
 // K17 AddSource_GPU_3D_1DThread, src/gpu.h:55-82 (box already clipped to the grid)
 template <bool IDX>
+// The source sample is s, or, for launches recorded into a graph that is replayed for later
+// steps, src[*dn + q]: the step index of the replay's first step (*dn) plus this launch's offset.
 __global__ void __launch_bounds__(256) k_add_currents(GridDev g, FieldSet F, CoefDev co, int comp, int a0, int b0,
-                                                      int c0, int ni, int nj, int nk, float s, float dx) {
+                                                      int c0, int ni, int nj, int nk, float s, float dx,
+                                                      const float *src, const int *dn, int q) {
   const long tid = (long)blockIdx.x * 256 + threadIdx.x;
   if (tid >= (long)ni * nj * nk) return;
+  if (src) s = src[*dn + q];
   const int i = a0 + (int)(tid % ni), j = b0 + (int)((tid / ni) % nj), k = c0 + (int)(tid / ((long)ni * nj));
   const size_t id = (size_t)i + (size_t)j * g.I + (size_t)k * g.I * g.J;
   float ce[3], ch[3];
   cell_coef<IDX>(co, co.tab, id, i, g.I, ce, ch);
   float *E = F.f[comp];
   E[id] = E[id] - ch[comp] * s * dx / dx / dx;
+}
+
+// Advances the step index read by graph-recorded launches by the steps of one replay.
+__global__ void k_advance(int *dn, int steps) {
+  if (threadIdx.x == 0) dn[0] = dn[0] + steps;
 }
 
 // 1D incident line for all T steps in one workgroup (src/simulation.h:198-244 with the kernels
@@ -1658,15 +1667,20 @@ hipError_t launch_pw_e(hipStream_t st, const GridDev &g, const FieldSet &F, cons
 }
 
 hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, int comp,
-                               const int box[6], float s, float dx) {
+                               const int box[6], float s, float dx, const float *src, const int *dn, int q) {
   const int a0 = box[0] < 0 ? 0 : box[0], a1 = box[1] >= g.I ? g.I - 1 : box[1];
   const int b0 = box[2] < 0 ? 0 : box[2], b1 = box[3] >= g.J ? g.J - 1 : box[3];
   const int c0 = box[4] < 0 ? 0 : box[4], c1 = box[5] >= g.K ? g.K - 1 : box[5];
   if (a1 < a0 || b1 < b0 || c1 < c0) return hipSuccess;
   const int ni = a1 - a0 + 1, nj = b1 - b0 + 1, nk = c1 - c0 + 1;
   const unsigned nb = cdiv((long)ni * nj * nk, 256);
-  if (co.mat) hipLaunchKernelGGL(k_add_currents<true>, dim3(nb), dim3(256), 0, st, g, F, co, comp, a0, b0, c0, ni, nj, nk, s, dx);
-  else hipLaunchKernelGGL(k_add_currents<false>, dim3(nb), dim3(256), 0, st, g, F, co, comp, a0, b0, c0, ni, nj, nk, s, dx);
+  if (co.mat) hipLaunchKernelGGL(k_add_currents<true>, dim3(nb), dim3(256), 0, st, g, F, co, comp, a0, b0, c0, ni, nj, nk, s, dx, src, dn, q);
+  else hipLaunchKernelGGL(k_add_currents<false>, dim3(nb), dim3(256), 0, st, g, F, co, comp, a0, b0, c0, ni, nj, nk, s, dx, src, dn, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(hipStream_t st, int *dn, int steps) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, dn, steps);
   return hipGetLastError();
 }
 

@@ -47,8 +47,12 @@ hipError_t launch_update_h_pml(hipStream_t st, const GridDev &g, const FieldSet 
 hipError_t launch_update_e_pml(hipStream_t st, const GridDev &g, const FieldSet &F, const PsiSet &ps, const CoefDev &co);
 hipError_t launch_pw_h(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, const PWDev &pw, int n);
 hipError_t launch_pw_e(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, const PWDev &pw, int n);
+// src (optional): take the sample from src[*dn + q] instead of s (launches recorded into a graph)
 hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, int comp,
-                               const int box[6], float s, float dx);
+                               const int box[6], float s, float dx, const float *src = nullptr, const int *dn = nullptr,
+                               int q = 0);
+// *dn += steps (one thread): the step index of graph-recorded launches, advanced once per replay
+hipError_t launch_advance(hipStream_t st, int *dn, int steps);
 hipError_t launch_line(hipStream_t st, const LineDev &L);
 hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int nAnt, const int *comps, int ncomp,
                          float *port, int row, int T);

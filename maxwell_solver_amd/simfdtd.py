@@ -20,6 +20,7 @@ _lib = None
 EXPORTS = [
     "fdtd_last_error", "fdtd_last_error_id", "fdtd_version", "fdtd_create", "fdtd_destroy",
     "fdtd_set_const", "fdtd_set_source", "fdtd_set_antenna_locations", "fdtd_update_material", "fdtd_create_mu",
+    "fdtd_set_step_currents", "fdtd_set_graph_steps",
     "fdtd_ready", "fdtd_init_1d", "fdtd_update_coeffs", "fdtd_reset_fields", "fdtd_update_h_all",
     "fdtd_update_e_all", "fdtd_update_h_pml", "fdtd_update_e_pml", "fdtd_update_h_planewave",
     "fdtd_update_e_planewave", "fdtd_add_currents", "fdtd_output_ant", "fdtd_output_vis",
@@ -54,6 +55,7 @@ def lib():
             "fdtd_create": [C.POINTER(vp)], "fdtd_destroy": [vp],
             "fdtd_set_const": [vp, fp, ip], "fdtd_set_source": [vp, fp, i32],
             "fdtd_set_antenna_locations": [vp, ip], "fdtd_update_material": [vp, i32, fp], "fdtd_create_mu": [vp, fp, fp, fp],
+            "fdtd_set_step_currents": [vp, i32, ip], "fdtd_set_graph_steps": [vp, i32],
             "fdtd_ready": [vp], "fdtd_init_1d": [vp, C.c_float, fp], "fdtd_update_coeffs": [vp],
             "fdtd_reset_fields": [vp], "fdtd_update_h_all": [vp], "fdtd_update_e_all": [vp],
             "fdtd_update_h_pml": [vp], "fdtd_update_e_pml": [vp],
@@ -175,6 +177,15 @@ class Simulation:
         """CreateMu: per-cell relative permeabilities (three arrays of I*J*K floats)."""
         m = [_f32(v) for v in (mu_x, mu_y, mu_z)]
         self._chk(lib().fdtd_create_mu(self._h, *(_p(v, C.c_float) for v in m)))
+
+    def set_step_currents(self, pol, box=(0, 0, 0, 0, 0, 0)):
+        """AddCurrents(n, pol, box) after every step of step() / run() (pol 0: off)."""
+        b = _i32(box)
+        self._chk(lib().fdtd_set_step_currents(self._h, int(pol), _p(b, C.c_int32)))
+
+    def set_graph_steps(self, steps):
+        """Record `steps` fused steps into one HIP graph and replay it in qualifying runs (0: off)."""
+        self._chk(lib().fdtd_set_graph_steps(self._h, int(steps)))
 
     def set_material_index(self, index, table):
         idx = np.ascontiguousarray(np.asarray(index, dtype=np.uint8).reshape(-1))

@@ -118,6 +118,63 @@ def test_fused_run_with_mu_takes_per_op_path():
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+def _gic_oracle_steps(o, n0, count, box):
+    for n in range(n0, n0 + count):
+        o.update_h_all()
+        o.update_h_pml()
+        o.update_e_all()
+        o.update_e_pml()
+        o.add_currents(n, 1, box)
+
+
+def test_step_currents_graph_replays():
+    """GICsimulation.m:119-124 loop (no plane wave, AddCurrents after every step) as step currents:
+    runs replay one recorded graph of 8 steps (the current sample read on the device), with eager
+    steps and a re-recording between the replays; every stage equals the oracle's op sequence."""
+    import dataclasses
+    from maxwell_solver_amd import scenarios as S
+    sc = dataclasses.replace(_scenario(I=48, J=52, K=40, T=60, mode=0, cpml=CFG_CPML, spheres=False, layered=True),
+                             probes=None)
+    box = [1, 46, 26, 26, 26, 26]
+    g, o = _backends()
+    S.setup(g, sc, init_line=False)
+    S.setup(o, sc, init_line=False)
+    g.set_step_currents(1, box)
+    g.set_graph_steps(8)
+    g.run(0, 21, 0)           # two replays, five steps one by one
+    _gic_oracle_steps(o, 0, 21, box)
+    assert_close(compare_state(g, o), exact=True)
+    g.step(21, 0)             # the current set flips: the next run records again
+    _gic_oracle_steps(o, 21, 1, box)
+    g.run(22, 17, 0)          # two replays of the new recording, one step
+    _gic_oracle_steps(o, 22, 17, box)
+    assert_close(compare_state(g, o), exact=True)
+    g.run(39, 16, 0)          # replays of the same recording again, after eager work
+    _gic_oracle_steps(o, 39, 16, box)
+    assert_close(compare_state(g, o), exact=True)
+    g.set_graph_steps(0)
+    g.run(55, 5, 0)
+    _gic_oracle_steps(o, 55, 5, box)
+    assert_close(compare_state(g, o), exact=True)
+
+
+def test_step_currents_with_probes():
+    """Step currents in runs with probes (steps one by one): each record samples the step's E
+    before its currents, as UpdateE_PML closes a step before AddCurrents."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=44, K=40, T=40, mode=0, cpml=CFG_CPML)
+    box = [3, 60, 20, 20, 22, 22]
+    g, o = _backends()
+    S.setup(g, sc, init_line=False)
+    S.setup(o, sc, init_line=False)
+    g.set_step_currents(1, box)
+    g.set_graph_steps(8)      # not used: the run records probes
+    g.run(0, sc.T, 0)
+    _gic_oracle_steps(o, 0, sc.T, box)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def test_per_op_gic_currents():
     """GICsimulation.m:119-124 order (no plane wave, AddCurrents after UpdateE_PML)."""
     from maxwell_solver_amd import scenarios as S
