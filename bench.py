@@ -301,6 +301,63 @@ def cpu_baseline(sc, budget_s=15.0, max_steps=1000):
     }
 
 
+def run_gic(args):
+    """GICsimulation.m workload (250 x 280 x 50 cells, dx = 10 km, layered and graded ground
+    conductivity, an x-directed 100 kA line current added after every step, no plane wave): all
+    --gic-steps steps of the run with graph replay (--gic-graph steps per recorded graph), and a
+    sample of --gic-eager steps launched one by one for comparison."""
+    import numpy as np
+
+    from maxwell_solver_amd import Simulation
+    from maxwell_solver_amd import scenarios as S
+    steps, eager = args.gic_steps, args.gic_eager
+    sc = S.config_gic(T=args.warmup + steps + eager)
+    g = Simulation()
+    S.setup(g, sc, init_line=False)
+    g.set_graph_steps(args.gic_graph)
+    g.run(0, args.warmup, 0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run(args.warmup, steps, 0)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    g.set_graph_steps(0)
+    t0 = time.perf_counter()
+    g.run(args.warmup + steps, eager, 0)
+    g.synchronize()
+    dt_eager = time.perf_counter() - t0
+    ez = g.field("Ex")
+    finite = bool(np.isfinite(ez).all())
+    g.close()
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(sc.cells * steps / dt / 1e9, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (the GICsimulation.m geometry, conductivities and source)",
+        "config": {
+            "workload": f"GIC {sc.I}x{sc.J}x{sc.K}, dx 10 km, 10-cell CPML, {len(sc.mat_table)} conductivity "
+                        "rows (uint8 index), line current (AddCurrents Ex) after every step, no plane wave",
+            "grid": [sc.I, sc.J, sc.K],
+            "cells": sc.cells,
+            "graph_steps": args.gic_graph,
+            "eager_steps": eager,
+            "eager_value": round(sc.cells * eager / dt_eager / 1e9, 3),
+            "eager_ms_per_step": round(dt_eager * 1e3 / eager, 5),
+            "graph_vs_eager": round((dt_eager / eager) / (dt / steps), 3),
+            "fields_finite": finite,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,9 +370,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", choices=["grid", "batch", "serve"], default="grid",
+    ap.add_argument("--workload", choices=["grid", "batch", "serve", "gic"], default="grid",
                     help="grid: the 256^3 workload (default); batch: 96 independent grids, batched engine; "
-                         "serve: a stream of solve requests through the request server")
+                         "serve: a stream of solve requests through the request server; gic: the "
+                         "GICsimulation.m run (line current every step, graph replay)")
+    ap.add_argument("--gic-steps", type=int, default=300000)
+    ap.add_argument("--gic-eager", type=int, default=20000)
+    ap.add_argument("--gic-graph", type=int, default=64)
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--max-active", type=int, default=8)
     ap.add_argument("--steps-per-round", type=int, default=64)
@@ -330,6 +391,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "gic":
+        run_gic(args)
         return
     if args.workload == "batch":
         run_batch(args)

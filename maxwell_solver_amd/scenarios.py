@@ -101,6 +101,7 @@ class Scenario:
     snap_every: int = 0
     snap_comp: int = 3
     seed: int = 0
+    currents: tuple | None = None        # step currents (pol, box[6]): AddCurrents after every step
 
     @property
     def cells(self) -> int:
@@ -236,6 +237,47 @@ def config3(T: int = 4000, I: int = 384, J: int = 384, K: int = 256, lattice: in
     return Scenario("config3", I, J, K, T, dx, dt, fc, (P, 3.0, 5.0, 0.05, 1.0, 0.8), inset, 2, (1.0,),
                     gauss_pulse(T, dt, fc), mat_index=idx, mat_table=tab, probes=probes, probe_comps=(1, 5),
                     seed=seed)
+
+
+def config_gic(T: int = 300000) -> Scenario:
+    """GICsimulation.m: geomagnetically induced currents.  dx = 10 km, dt = 0.99 dx / (sqrt(3) c),
+    a 230 x 260 x 30 domain extended by a 10-cell CPML on every side (250 x 280 x 50 cells; the
+    script's SetConst passes the unextended size, :56, while its material arrays are extended,
+    :57-61 -- the extended grid is the one it describes), eps_r = 1, a layered and laterally
+    graded conductivity (:70-94), no plane wave, and an x-directed line current along
+    i = 1..249 at (j, k) = (139, 26) driven by a 100 kA Gaussian pulse (t0 = 5 s, FWHM 10 s)
+    through AddCurrents after every step (:119-124)."""
+    P = 10
+    I, J, K = 230 + 2 * P, 260 + 2 * P, 30 + 2 * P
+    dx = 10e3
+    dt = _dt(dx)
+    x1, x2 = 108, 135  # 1-based, :73-74
+    sx = np.zeros((K, J, I), np.float64)
+    sy = np.zeros_like(sx)
+    sz = np.zeros_like(sx)
+    for s in (sx, sy, sz):
+        s[0:16, :, 0:x1] = 0.01586
+        s[0:16, :, x2:] = 0.001
+    for s in (sx, sy):
+        s[16, :, 0:x1] = 0.01586 / 2
+        s[16, :, x2:] = 0.001 / 2
+    fact = 1.1
+    for i in range(x1, x2):  # 1-based x1+1 .. x2
+        sx[0:16, :, i] = 0.01586 / fact
+        sy[0:16, :, i] = 0.01586 / fact
+        sx[16, :, i] = 0.01586 / fact / 2
+        sy[16, :, i] = 0.01586 / fact / 2
+        sz[0:16, :, i] = 0.01586 / fact
+        fact *= 1.1
+    rows = np.stack([sx.reshape(-1), sy.reshape(-1), sz.reshape(-1)], axis=1).astype(np.float32)
+    table, index = np.unique(rows, axis=0, return_inverse=True)
+    mat_table = np.concatenate([table, np.ones((len(table), 3), np.float32)], axis=1)
+    t = np.arange(T) * dt
+    sig = 10.0 / 4 / 2.355
+    source = (100e3 * np.exp(-(t - 5.0) ** 2 / (2 * sig ** 2))).astype(np.float32)
+    return Scenario("gic", I, J, K, T, dx, dt, 5.5e9, (P, 3.0, 1.0, 0.08, 1.0, 0.8), 15, 0, (0.0,), source,
+                    mat_index=index.reshape(K, J, I).astype(np.uint8), mat_table=mat_table.astype(np.float32),
+                    currents=(1, (1, 249, 139, 139, 26, 26)))
 
 
 def config4(n_grids: int = 96, edge_lo: float = 32, edge_hi: float = 192) -> list:
@@ -424,3 +466,5 @@ def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = Tru
         backend.init_1d(1.0, sc.source)
     backend.update_coeffs()
     backend.reset()
+    if sc.currents is not None and hasattr(backend, "set_step_currents"):
+        backend.set_step_currents(sc.currents[0], list(sc.currents[1]))
