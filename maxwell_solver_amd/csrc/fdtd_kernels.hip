@@ -1781,10 +1781,11 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   }
   // rows per tile (a workgroup of R S waves, at most the row kernel's bound).  Interior part: two
   // rows (measured at 256^3: 180 us against 190 with one; the rows share their neighbour rows
-  // through LDS), four for one- and two-segment rows.  Slab part: one row where a row takes several
-  // segments (its heavier waves keep fewer workgroups resident; two rows cost it 35%), else two.
+  // through LDS; for the one- and two-segment rows of a batch of small grids two rows beat four
+  // by 3%).  Slab part: one row where a row takes several segments (its heavier waves keep fewer
+  // workgroups resident; two rows cost it 35%), else two.
   const int max_waves = kRowsMaxThreads[cpl] / 64;
-  int R0 = S >= 3 ? 2 : 4, R = S == 1 ? 2 : 1;
+  int R0 = 2, R = S == 1 ? 2 : 1;
   while (R0 > 1 && S * R0 > max_waves) R0 /= 2;
   while (R > 1 && S * R > max_waves) R /= 2;
   if (tune.tile_rows > 0) R = R0 = tune.tile_rows;
