@@ -735,6 +735,13 @@ struct fdtd_sim {
       fo.idx = dProbeIdx; fo.comps = dProbeComps; fo.port = dPort;
       fo.nAnt = nAnt; fo.ncomp = (int)probe_comps.size(); fo.row = rf->probe_row; fo.T = T;
     }
+    if (step_pol && !dPort) {  // with probes the currents follow the step's probe gather
+      if (n < 0 || n >= T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
+      fo.cur_comp = step_pol - 1;
+      for (int q = 0; q < 6; ++q) fo.cur_box[q] = step_box[q];
+      fo.cur_s = source[n];
+      fo.cur_dx = delX;
+    }
     HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr, &fo));
     if (timed) HIPCHK(hipEventRecord(e[3], stream));
     rf->sheets_n = fo.next_filled ? n + 1 : -1;
@@ -744,7 +751,7 @@ struct fdtd_sim {
     rec_step++;
     if (step_pol) {  // the probes sample the step's E before its currents (as UpdateE_PML does)
       if (int rc = flush_probes(*rf)) return rc;
-      return step_currents(n);
+      if (!fo.cur_done) return step_currents(n);
     }
     return 0;
   }
@@ -783,9 +790,15 @@ struct fdtd_sim {
       int c = cur;
       hipError_t e = hipSuccess;
       for (int q = 0; q < G && e == hipSuccess; ++q) {
-        e = launch_step_fused(stream, f[c], slot[c]->dptr, slot[c]->host.data(), slot[c]->nwg, n0 + q);
+        StepFold fo;  // the step currents ride in the slab part's launch where they can
+        if (step_pol) {
+          fo.cur_comp = step_pol - 1;
+          for (int b = 0; b < 6; ++b) fo.cur_box[b] = step_box[b];
+          fo.cur_dx = delX; fo.cur_src = dSrc; fo.cur_dn = dStepN; fo.cur_q = q;
+        }
+        e = launch_step_fused(stream, f[c], slot[c]->dptr, slot[c]->host.data(), slot[c]->nwg, n0 + q, nullptr, &fo);
         c = 1 - c;
-        if (e == hipSuccess && step_pol)
+        if (e == hipSuccess && step_pol && !fo.cur_done)
           e = launch_add_currents(stream, g, fields(c), coefs(), step_pol - 1, step_box, 0.0f, delX, dSrc, dStepN, q);
       }
       if (e == hipSuccess) e = launch_advance(stream, dStepN, G);

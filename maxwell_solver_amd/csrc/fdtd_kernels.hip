@@ -1483,8 +1483,18 @@ struct TailArgs {
   const int *pidx, *pcomps;
   float *port;
   int nAnt, ncomp, prow, T;
+  // step currents (nb_cur workgroups) on the new state: the box lies in the interior part's cells,
+  // which that launch has written; sample s, or src[*dn + q] in a recorded graph
+  FieldSet Fn;
+  CoefDev co;
+  int comp, a0, b0, c0, ni, nj, nk, q;
+  float s, dx;
+  const float *src;
+  const int *dn;
+  unsigned nb_cur;
 };
 
+template <bool IDX>
 __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
   const unsigned nthr = blockDim.x;
   if (blk < ta.nb_sheets) {
@@ -1496,10 +1506,25 @@ __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
     return;
   }
   blk -= ta.nb_sheets;
-  const uint32_t np = (uint32_t)ta.nAnt * ta.ncomp;
-  for (uint32_t t = blk * nthr + threadIdx.x; t < np; t += ta.nb_probes * nthr) {
-    const int q = t / ta.nAnt, a = t % ta.nAnt;
-    ta.port[((size_t)q * ta.T + ta.prow) * ta.nAnt + a] = ta.F.f[ta.pcomps[q]][ta.pidx[a]];
+  if (blk < ta.nb_probes) {
+    const uint32_t np = (uint32_t)ta.nAnt * ta.ncomp;
+    for (uint32_t t = blk * nthr + threadIdx.x; t < np; t += ta.nb_probes * nthr) {
+      const int q = t / ta.nAnt, a = t % ta.nAnt;
+      ta.port[((size_t)q * ta.T + ta.prow) * ta.nAnt + a] = ta.F.f[ta.pcomps[q]][ta.pidx[a]];
+    }
+    return;
+  }
+  blk -= ta.nb_probes;
+  // K17 AddSource_GPU_3D_1DThread (src/gpu.h:55-82), as k_add_currents
+  const float s = ta.src ? ta.src[*ta.dn + ta.q] : ta.s;
+  const long nc = (long)ta.ni * ta.nj * ta.nk;
+  for (long tid = (long)blk * nthr + threadIdx.x; tid < nc; tid += (long)ta.nb_cur * nthr) {
+    const int i = ta.a0 + (int)(tid % ta.ni), j = ta.b0 + (int)((tid / ta.ni) % ta.nj), k = ta.c0 + (int)(tid / ((long)ta.ni * ta.nj));
+    const size_t id = (size_t)i + (size_t)j * ta.g.I + (size_t)k * ta.g.I * ta.g.J;
+    float ce[3], ch[3];
+    cell_coef<IDX>(ta.co, ta.co.tab, id, i, ta.g.I, ce, ch);
+    float *E = ta.Fn.f[ta.comp];
+    E[id] = E[id] - ch[ta.comp] * s * ta.dx / ta.dx / ta.dx;
   }
 }
 
@@ -1509,7 +1534,7 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *_
                                                                const TailArgs ta) {
   const int b = blockIdx.x;
   if (PART == 1 && b >= nwg) {
-    tail_body(ta, b - nwg);
+    tail_body<IDX>(ta, b - nwg);
     return;
   }
   rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(b, nwg));
@@ -1751,7 +1776,7 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
   TailArgs ta{};
   if (tail && part != 0) ta = *tail;
-  const unsigned nt = ta.nb_sheets + ta.nb_probes;
+  const unsigned nt = ta.nb_sheets + ta.nb_probes + ta.nb_cur;
   const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
@@ -1991,6 +2016,7 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
   TailArgs ta{};
   if (fold) {
     fold->next_filled = false;
+    fold->cur_done = false;
     const unsigned nthr = 64u * a.S * a.R;
     if (nwg <= 0) {  // no row launch to carry the tail
       if (fold->row >= 0)
@@ -2007,6 +2033,23 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
       ta.F = fold->F; ta.pidx = fold->idx; ta.pcomps = fold->comps; ta.port = fold->port;
       ta.nAnt = fold->nAnt; ta.ncomp = fold->ncomp; ta.prow = fold->row; ta.T = fold->T;
       ta.nb_probes = std::min(cdiv((long)fold->nAnt * fold->ncomp, nthr), 64u);
+    }
+    if (fold->cur_comp >= 0) {
+      const GridDev &g = f.g;
+      const int *bx = fold->cur_box;
+      const int a0 = bx[0] < 0 ? 0 : bx[0], a1 = bx[1] >= g.I ? g.I - 1 : bx[1];
+      const int b0 = bx[2] < 0 ? 0 : bx[2], b1 = bx[3] >= g.J ? g.J - 1 : bx[3];
+      const int c0 = bx[4] < 0 ? 0 : bx[4], c1 = bx[5] >= g.K ? g.K - 1 : bx[5];
+      if (a1 < a0 || b1 < b0 || c1 < c0) {
+        fold->cur_done = true;  // an empty box adds nothing
+      } else if (b0 >= g.P + 1 && b1 <= g.J - g.P - 1 && c0 >= g.P && c1 <= g.K - g.P - 1) {
+        ta.g = g; ta.Fn = sa.nxt; ta.co = sa.co;
+        ta.comp = fold->cur_comp; ta.a0 = a0; ta.b0 = b0; ta.c0 = c0;
+        ta.ni = a1 - a0 + 1; ta.nj = b1 - b0 + 1; ta.nk = c1 - c0 + 1;
+        ta.s = fold->cur_s; ta.dx = fold->cur_dx; ta.src = fold->cur_src; ta.dn = fold->cur_dn; ta.q = fold->cur_q;
+        ta.nb_cur = std::min(cdiv((long)ta.ni * ta.nj * ta.nk, nthr), 64u);
+        fold->cur_done = true;
+      }
     }
   }
   if (nwg > 0) {

@@ -99,6 +99,14 @@ struct StepFold {
   const int *idx = nullptr, *comps = nullptr;
   float *port = nullptr;
   int nAnt = 0, ncomp = 0, row = -1, T = 0;
+  // Step currents (cur_comp >= 0: E component): AddCurrents on the new state after the step, taken
+  // into the slab part's launch when the box lies in the interior part's cells (cur_done reports
+  // it; otherwise the caller launches them).  Sample cur_s, or cur_src[*cur_dn + cur_q].
+  int cur_comp = -1, cur_box[6] = {0, 0, 0, 0, 0, 0}, cur_q = 0;
+  float cur_s = 0.0f, cur_dx = 0.0f;
+  const float *cur_src = nullptr;
+  const int *cur_dn = nullptr;
+  bool cur_done = false;
 };
 // ev_rows (optional): two events recorded on st around the row kernel's interior part
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,

@@ -127,7 +127,10 @@ def _gic_oracle_steps(o, n0, count, box):
         o.add_currents(n, 1, box)
 
 
-def test_step_currents_graph_replays():
+# current boxes: in the interior part's cells (applied inside the slab part's launch), in a y-slab
+# row and across z-slab planes (separate launches)
+@pytest.mark.parametrize("box", [[1, 46, 26, 26, 26, 26], [1, 46, 5, 5, 26, 26], [20, 21, 20, 30, 2, 35]])
+def test_step_currents_graph_replays(box):
     """GICsimulation.m:119-124 loop (no plane wave, AddCurrents after every step) as step currents:
     runs replay one recorded graph of 8 steps (the current sample read on the device), with eager
     steps and a re-recording between the replays; every stage equals the oracle's op sequence."""
@@ -135,7 +138,6 @@ def test_step_currents_graph_replays():
     from maxwell_solver_amd import scenarios as S
     sc = dataclasses.replace(_scenario(I=48, J=52, K=40, T=60, mode=0, cpml=CFG_CPML, spheres=False, layered=True),
                              probes=None)
-    box = [1, 46, 26, 26, 26, 26]
     g, o = _backends()
     S.setup(g, sc, init_line=False)
     S.setup(o, sc, init_line=False)
