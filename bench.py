@@ -370,8 +370,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", choices=["grid", "batch", "serve", "gic"], default="grid",
-                    help="grid: the 256^3 workload (default); batch: 96 independent grids, batched engine; "
+    ap.add_argument("--workload", choices=["grid", "layered", "batch", "serve", "gic"], default="grid",
+                    help="grid: the 256^3 workload (default); layered: the 384x384x256 lossy layered-ground "
+                         "workload (4000 steps with --steps 4000); batch: 96 independent grids, batched engine; "
                          "serve: a stream of solve requests through the request server; gic: the "
                          "GICsimulation.m run (line current every step, graph replay)")
     ap.add_argument("--gic-steps", type=int, default=300000)
@@ -406,10 +407,13 @@ def main():
     from maxwell_solver_amd import scenarios as S
 
     T = args.warmup + args.steps
-    sc = S.config2(T=max(T, 1))
-    if args.size != 256:
-        sc = S.small_variant(sc, args.size, T)
-        sc.snap_every = 250
+    if args.workload == "layered":
+        sc = S.config3(T=max(T, 1))
+    else:
+        sc = S.config2(T=max(T, 1))
+        if args.size != 256:
+            sc = S.small_variant(sc, args.size, T)
+            sc.snap_every = 250
     g = Simulation()
     S.setup(g, sc)
     g.set_tuning(args.rows_per_group, args.z_chunk, args.cols_per_lane)
@@ -452,10 +456,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded spheres, seeded plane-wave angles)",
+        "data": ("synthetic (seeded layer thicknesses, conductivities, permittivities and pulse frequency)"
+                 if args.workload == "layered" else "synthetic (seeded spheres, seeded plane-wave angles)"),
         "config": {
-            "workload": f"{sc.I}^3 FP32 Yee FDTD, 10-cell CPML, 40 dielectric spheres (uint8 index), oblique TF/SF "
-                        f"plane wave, 64 probes Ez/Hy, Ez snapshot every {sc.snap_every} steps",
+            "workload": (f"{sc.I}x{sc.J}x{sc.K} FP32 Yee FDTD, lossy layered ground (5 layers, uint8 index), "
+                         f"12-cell CPML, vertically incident x-polarised pulse (layered 1D line), "
+                         f"{len(sc.probes)} surface probes Ex/Hy" if args.workload == "layered" else
+                         f"{sc.I}^3 FP32 Yee FDTD, 10-cell CPML, 40 dielectric spheres (uint8 index), oblique TF/SF "
+                         f"plane wave, 64 probes Ez/Hy, Ez snapshot every {sc.snap_every} steps"),
             "grid": [sc.I, sc.J, sc.K],
             "cells": sc.cells,
             "time_steps": args.steps,
@@ -483,7 +491,9 @@ def main():
     }
     g.close()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(S.config2(T=1200) if args.size == 256 else sc, budget_s=args.cpu_budget)
+        base = (S.config3(T=600) if args.workload == "layered" else
+                S.config2(T=1200) if args.size == 256 else sc)
+        rec["cpu_baseline"] = cpu_baseline(base, budget_s=args.cpu_budget)
     print(json.dumps(rec), flush=True)
 
 

@@ -1730,14 +1730,15 @@ bool fused_supported(const GridDev &g) {
 
 // Row layout of the row kernel: CPL columns per lane (I a multiple of CPL, for aligned vector
 // accesses) and S segments of 64 CPL columns per row, with the fewest idle lanes, then the fewest
-// columns per lane (more waves in flight: measured faster at 256^3); at most 4 segments (rows up
-// to 1024 columns).  force_cpl != 0 restricts CPL.
+// columns per lane (more waves in flight: measured faster at 256^3); at most 8 segments with one
+// column per lane (rows up to 512 columns; 2 rows x 8 waves fill the 1024-thread workgroup) and 4
+// with two or four (rows up to 1024 columns).  force_cpl != 0 restricts CPL.
 static bool row_layout(int I, int force_cpl, int &cpl, int &S) {
   int best_waste = -1;
   for (int c : {1, 2, 4}) {
     if (I % c || (force_cpl && c != force_cpl)) continue;
     const int s = (I + 64 * c - 1) / (64 * c);
-    if (s > 4) continue;
+    if (s > (c == 1 ? 8 : 4)) continue;
     const int waste = s * 64 * c - I;
     if (best_waste < 0 || waste < best_waste) {
       best_waste = waste;

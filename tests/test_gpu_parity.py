@@ -212,13 +212,15 @@ def test_fused_step_axis_modes(mode, cpml):
 # workgroup; with 1 column per lane and I a multiple of 64 the split edge segment (both row ends in
 # one wave: I = 64, 128, 192, 256 take 1 to 4 segments), with the x faces of the TF/SF box in the
 # edge wave or (inset 40) in plain segments; -1 columns per lane: the same grids unsplit; rows of a
-# workgroup exchange their neighbour rows through LDS (up to 16 rows, 1024 threads).
+# workgroup exchange their neighbour rows through LDS (up to 16 rows, 1024 threads); rows of 300 to
+# 512 columns take 5 to 8 one-column segments (split when I is a multiple of 64), or 2 columns.
 @pytest.mark.parametrize("I,rows,z_chunk,cpl,inset", [
     (52, 0, 0, 0, 15), (52, 1, 5, 1, 15), (52, 2, 13, 4, 15), (52, 4, 3, 2, 15),
     (136, 0, 0, 1, 15), (200, 0, 7, 2, 15), (264, 0, 0, 4, 15), (264, 1, 4, 2, 15),
     (64, 0, 0, 1, 15), (64, 4, 6, 1, 13), (128, 0, 0, 1, 15), (128, 2, 5, 1, 40), (192, 1, 0, 1, 40),
     (192, 0, 3, 1, 15), (256, 0, 0, 1, 15), (256, 0, 9, 1, 40), (128, 0, 0, -1, 15), (256, 0, 0, -1, 15),
-    (64, 16, 0, 1, 15), (128, 8, 5, 1, 15), (52, 8, 4, 1, 15), (256, 4, 0, 1, 15), (200, 2, 0, 2, 15)])
+    (64, 16, 0, 1, 15), (128, 8, 5, 1, 15), (52, 8, 4, 1, 15), (256, 4, 0, 1, 15), (200, 2, 0, 2, 15),
+    (384, 0, 0, 0, 15), (320, 1, 0, 1, 40), (300, 0, 6, 0, 15), (512, 0, 0, 1, 15), (384, 0, 0, 2, 15)])
 def test_fused_tilings_oblique(I, rows, z_chunk, cpl, inset):
     """Oblique plane wave through every row layout: halo rows/planes recomputed from old data."""
     from maxwell_solver_amd import scenarios as S
@@ -495,6 +497,64 @@ def test_workload3_reduced():
     o.run(0, sc.T, sc.mode)
     assert_close(compare_state(g, o))
     assert rel_l2(g.probes(), o.probes()) <= 1e-5
+
+
+# FP32 error of each workload against an FP64 run of the oracle on the same seeded inputs (the 64^3
+# workload in full, the 256^3 and layered-ground workloads reduced): the probe series relative L2,
+# and the final fields' max-abs error against the largest recorded probe value (the 64^3 pulse has
+# left the box by step 600, so its final fields are residue and a relative L2 of them says nothing).
+@pytest.mark.parametrize("which", ["c1", "c2", "c3"])
+def test_fp32_error_against_fp64(which):
+    from maxwell_solver_amd import Simulation
+    from maxwell_solver_amd import scenarios as S
+    from oracle.oracle import OracleSim
+    sc = {"c1": lambda: S.config1(T=600), "c2": lambda: S.small_variant(S.config2(T=200), 64, 200),
+          "c3": lambda: S.config3(T=200, I=64, J=64, K=96, lattice=4)}[which]()
+    g, o = Simulation(), OracleSim(np.float64)
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, sc.T, sc.mode)
+    o.run(0, sc.T, sc.mode)
+    pg, po = g.probes().astype(np.float64), o.probes()
+    err_p = rel_l2(pg, po)
+    err_f = max(np.abs(g.field(n).astype(np.float64) - o.field(n)).max() for n in FIELDS)
+    peak = np.abs(po).max()
+    print(f"{which}: probes rel-L2 vs FP64 {err_p:.2e}, final-field max-abs / probe peak {err_f / peak:.2e}")
+    assert peak > 0
+    assert err_p <= 1e-5
+    assert err_f <= 1e-5 * peak
+
+
+def test_vacuum_energy_balance():
+    """Lossless vacuum (the 64^3 workload, pulse inside the TF box at step 150): the field energy in
+    the box's inner cells equals the energy of the 1D incident wave over the same cells (E^n and
+    H^{n-1/2} from the line histories), and the scattered-field region holds none of it."""
+    from maxwell_solver_amd import Simulation
+    from maxwell_solver_amd import scenarios as S
+    eps0, mu0 = 8.854187817e-12, 4e-7 * np.pi
+    sc = S.config1(T=600)
+    g = Simulation()
+    S.setup(g, sc)
+    n = 150
+    g.run(0, n, sc.mode)
+    F = {c: g.field(c).astype(np.float64) for c in FIELDS}
+    u = sum(eps0 * F[c] ** 2 for c in ("Ex", "Ey", "Ez")) + sum(mu0 * F[c] ** 2 for c in ("Hx", "Hy", "Hz"))
+    N, P, w = sc.I, sc.P, sc.inset
+    tf = np.zeros(u.shape, bool)
+    tf[w:N - w, w:N - w, w:N - w] = True
+    near = np.zeros(u.shape, bool)  # the box and the cells next to its faces
+    near[w - 2:N - w + 2, w - 2:N - w + 2, w - 2:N - w + 2] = True
+    sf = np.zeros(u.shape, bool)
+    sf[P:N - P, P:N - P, P:N - P] = True
+    sf &= ~near
+    assert u[sf].sum() <= 1e-9 * u[tf].sum()
+    a, b = w + 1, N - w - 1
+    u3 = u[a:b, a:b, a:b].sum()
+    E1, H1 = g.line(0)[n].astype(np.float64), g.line(1)[n - 1].astype(np.float64)
+    idx = np.arange(a, b) + E1.size - N
+    u1 = (b - a) ** 2 * (eps0 * (E1[idx] ** 2).sum() + mu0 * (H1[idx] ** 2).sum())
+    assert u1 > 0
+    assert abs(u3 - u1) <= 1e-5 * u1
 
 
 def test_error_paths():
