@@ -163,3 +163,43 @@ def test_gic_workload_transcribes_the_script():
     assert np.all(sc.mat_table[:, 3:] == 1.0)
     assert sc.currents == (1, (1, 249, 139, 139, 26, 26))
     assert abs(sc.dt - 0.99 * 10e3 / (np.sqrt(3) * S.C0)) < 1e-18
+
+
+def test_new_entry_points_validate_before_the_device():
+    """CreateMu, step currents, graph steps and plane-wave modes reject bad input on the host,
+    before any device call (these run without a GPU)."""
+    from maxwell_solver_amd import SimFDTDError, Simulation
+    s = Simulation()
+    n = 30 * 30 * 30
+    ones = np.ones(n, np.float32)
+    for call in (lambda: s.create_mu(ones, ones, ones), lambda: s.set_step_currents(1, [0, 1, 0, 1, 0, 1])):
+        with pytest.raises(SimFDTDError) as e:
+            call()  # before SetConst
+        assert e.value.identifier == "simFDTD:InvalidInput"
+    s.set_const([3e9, 0.01, 1.9e-11], [10, 30, 30, 30, 0])
+    bad = ones.copy()
+    bad[123] = 0.0
+    with pytest.raises(SimFDTDError):
+        s.create_mu(ones, bad, ones)  # mu must be positive
+    bad[123] = np.nan
+    with pytest.raises(SimFDTDError):
+        s.create_mu(bad, ones, ones)
+    s.create_mu(ones, ones, 2 * ones)  # taken before ReadySimulation (the fixed order)
+    with pytest.raises(SimFDTDError):
+        s.set_step_currents(1, [0, 1, 0, 1, 0, 1])  # before SetSource
+    s.set_source(np.zeros(10, np.float32))
+    with pytest.raises(SimFDTDError):
+        s.set_step_currents(4, [0, 1, 0, 1, 0, 1])  # pol must be 0..3
+    s.set_step_currents(1, [0, 1, 0, 1, 0, 1])
+    s.set_step_currents(0)
+    with pytest.raises(SimFDTDError):
+        s.set_graph_steps(7)  # an odd count would end a replay on the other state set
+    with pytest.raises(SimFDTDError):
+        s.set_graph_steps(-2)
+    s.set_graph_steps(16)
+    s.set_graph_steps(0)
+    for mode in (6, 7):
+        s.set_plane_wave(mode, [0.0])
+    with pytest.raises(SimFDTDError):
+        s.set_plane_wave(8, [0.0])
+    s.close()
