@@ -128,8 +128,10 @@ int fdtd_set_probe_components(fdtd_sim *s, const int32_t *comps, int32_t n);
  * pass of the row kernel, interior and CPML-slab parts), or the per-op kernels for grids outside
  * its layouts; mode 0 = no plane wave. */
 int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode);
-/* Steps n0 .. n0+count-1 (fused).  If snap_every > 0, component snap_comp (1..6) is copied to a
- * device snapshot buffer after every snap_every-th step (counted since ResetFieldAll). */
+/* Steps n0 .. n0+count-1 (fused).  Within a run the slab part's launch of step n also fills the
+ * face sheets of step n+1 and gathers the probes of step n-1, so a step is two launches.  If
+ * snap_every > 0, component snap_comp (1..6) is copied to a device snapshot buffer after every
+ * snap_every-th step (counted since ResetFieldAll). */
 int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_every, int32_t snap_comp);
 /* Number of snapshots held and their copy to host: out has count*I*J*K floats. */
 int fdtd_snapshot_count(fdtd_sim *s, int32_t *count);

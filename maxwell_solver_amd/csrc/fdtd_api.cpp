@@ -870,7 +870,7 @@ struct BatchPlan {
   struct Launch {
     int kind;  // 0 TF/SF face sheets, 1 row kernel part 0 (interior), 2 row kernel part 1 (slabs)
     RowsKey key;
-    bool idx = false, narrow = true;
+    bool idx = false;
     size_t tab = 0;  // element offset of the table in the device buffer
     int ng = 0, nwg = 0;
   };
@@ -921,8 +921,8 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
     int rows_nwg, rows_nwg1;
     RowsKey key;
     const void *slab_arg[2];
-    unsigned nb_slab, nb_sheets;
-    bool narrow, idx;
+    unsigned nb_sheets;
+    bool idx;
   };
   std::vector<Prep> preps;
   for (void *p : b->dslab) hipFree(p);
@@ -951,7 +951,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       pr.rows_nwg = rows_part_nwg(slot->host.data(), 0);
       pr.rows_nwg1 = rows_part_nwg(slot->host.data(), 1);
       pr.key = rows_key(slot->host.data());
-      make_slab_args(f, w ? h1.data() : h0.data(), &pr.nb_slab, &pr.nb_sheets, &pr.narrow);
+      make_slab_args(f, w ? h1.data() : h0.data(), &pr.nb_sheets);
       pr.idx = f.mat != nullptr;
     }
     if (!ok) {
@@ -989,8 +989,6 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       load[q] += (double)pr->it.s->IJK * pr->it.count;
     }
   }
-  bool narrow_all = true;
-  for (const auto &pr : preps) narrow_all = narrow_all && pr.narrow;
   std::vector<BatchGrid> tabs;
   std::vector<ProbeJob> jobs;
   std::vector<BatchPlan> plans(nsub);
@@ -1013,7 +1011,6 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
         L.kind = kind;
         L.key = key;
         L.idx = idx;
-        L.narrow = narrow_all;
         L.tab = tabs.size();
         L.ng = (int)grids.size();
         int off = 0;
@@ -1024,7 +1021,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
           bg.n0 = pr->it.n0 + t0;
           for (int w = 0; w < 2; ++w) bg.arg[w] = kind == 0 ? pr->slab_arg[w] : pr->rows_arg[w];
           tabs.push_back(bg);
-          off += kind == 0 ? (int)(pr->nb_slab + pr->nb_sheets) : kind == 1 ? pr->rows_nwg : pr->rows_nwg1;
+          off += kind == 0 ? (int)pr->nb_sheets : kind == 1 ? pr->rows_nwg : pr->rows_nwg1;
         }
         BatchGrid end;
         end.off = off;

@@ -1579,7 +1579,6 @@ struct SlabArgs {
   CoefDev co;
   PWDev pw;
   int n;
-  unsigned nbs;  // workgroups before the sheet fill (none: the row kernel covers every cell)
   float *xs;
   int xni, xnj, xnk;
 };
@@ -1593,8 +1592,7 @@ __device__ __forceinline__ void load_slab_args(SlabArgs &dst, const void *src) {
 
 // The TF/SF face sheets of time step n (one thread per sheet entry).
 __device__ __forceinline__ void sheets_body(const SlabArgs &a, unsigned blk, int n) {
-  if (blk < a.nbs) return;
-  const uint32_t t = (blk - a.nbs) * 256u + threadIdx.x;
+  const uint32_t t = blk * 256u + threadIdx.x;
   if (t < 8u * ((uint32_t)a.xnj * a.xnk + (uint32_t)a.xni * a.xnk + (uint32_t)a.xni * a.xnj)) {
     const float *rowE = a.pw.active_h ? a.pw.Eh + (size_t)n * a.pw.N1 : nullptr;
     const float *rowH = a.pw.active_e ? a.pw.Hh + (size_t)n * a.pw.N1 : nullptr;
@@ -1905,7 +1903,6 @@ static SlabArgs slab_args(const FusedDev &f, int n) {
   }
   for (int c = 0; c < 3; ++c) { pw.polE[c] = f.polE[c]; pw.polH[c] = f.polH[c]; }
   s.n = n;
-  s.nbs = 0;
   s.xs = f.xs;
   s.xni = g.i1 - g.i0 + 1;
   s.xnj = g.j1 - g.j0 + 1;
@@ -1929,16 +1926,12 @@ RowsKey rows_key(const void *hargs) {
 
 size_t slab_args_size() { return sizeof(SlabArgs); }
 
-void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow) {
-  SlabArgs sa = slab_args(f, 0);
-  const unsigned nb = 0;  // the row kernel covers the slab cells; this launch fills the face sheets
-  sa.nbs = nb;
+void make_slab_args(const FusedDev &f, void *out, unsigned *nb_sheets) {
+  const SlabArgs sa = slab_args(f, 0);
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
   std::memcpy(out, &sa, sizeof(sa));
-  *nb_slab = nb;
   *nb_sheets = sheets ? cdiv(nsheet, 256) : 0;
-  *narrow = (long)nb * 256 < (1L << 31);
 }
 
 template <bool IDX, bool BUF, int PART>
@@ -1995,8 +1988,7 @@ hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, 
 // fold->row) as tail workgroups.
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows, StepFold *fold) {
-  SlabArgs sa = slab_args(f, n);
-  sa.nbs = 0;
+  const SlabArgs sa = slab_args(f, n);
   const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
   const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
   const unsigned nbx = sheets ? cdiv(nsheet, 256) : 0;

@@ -146,10 +146,9 @@ struct RowsKey {
   }
 };
 RowsKey rows_key(const void *hargs);
-// Face-sheet argument block of f (host copy; n is supplied per step) with its workgroup counts:
-// nb_slab (always 0: the row kernel covers every cell) and the face-sheet workgroups.
+// Face-sheet argument block of f (host copy; n is supplied per step) and its workgroup count.
 size_t slab_args_size();
-void make_slab_args(const FusedDev &f, void *out, unsigned *nb_slab, unsigned *nb_sheets, bool *narrow);
+void make_slab_args(const FusedDev &f, void *out, unsigned *nb_sheets);
 hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
                              int step);
 // Workgroups of part 0 (interior rows and chunks) or part 1 (the slab rows and chunks) of the row kernel.
