@@ -102,8 +102,8 @@ int fdtd_set_tfsf(fdtd_sim *s, int32_t inset);
 int fdtd_set_plane_wave(fdtd_sim *s, int32_t mode, const double params[4]);
 /* 'CreateMu' src/simulation.h:28-68 (gateway src/simFDTD.cu:133-141): per-cell relative
  * permeabilities (three arrays of I*J*K floats, each > 0).  The H update then uses the per-cell
- * coefficients of K2 GPU_updateH_Full (src/gpu.h:4-30) instead of K1's scalar ones; the fused step
- * runs as the per-op sequence.  The reference's initialisation order is fixed: it sets init_mu
+ * coefficients of K2 GPU_updateH_Full (src/gpu.h:4-30) instead of K1's scalar ones, in the per-op
+ * kernels and in the fused step (12 more bytes per cell).  The reference's initialisation order is fixed: it sets init_mu
  * only before ReadySimulation but needs den_hx, which ReadySimulation fills (:30-32, :45), so it
  * never takes effect there; here CreateMu may come any time after SetConst and takes effect at
  * ReadySimulation, or at once when that has run. */

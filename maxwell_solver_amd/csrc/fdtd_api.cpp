@@ -641,7 +641,7 @@ struct fdtd_sim {
         for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
       }
       f.Chh = Chh; f.Che = Che; f.dx = delX;
-      f.mu = dChmu[0] != nullptr;
+      f.Chmu = dChmu[0];  // the three arrays are one block
     }
     if (slot.gen != args_gen) {
       slot.host.assign(step_args_size(), 0);
@@ -943,7 +943,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       FusedDev f;
       fdtd_sim::ArgSlot *slot = nullptr;
       if (int rc = s->fused_slot(w, it.mode, pw, f, &slot)) return rc;
-      if (slot->nwg < 0 || s->step_pol) {  // per-op layouts and step currents run on their own
+      if (slot->nwg < 0 || s->step_pol || s->dChmu[0]) {  // per-op layouts, step currents, mu run alone
         ok = false;
         break;
       }

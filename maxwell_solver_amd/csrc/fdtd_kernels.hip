@@ -451,6 +451,7 @@ struct StepArgs {
   const float *prm;    // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
   const float *xs;      // TF/SF face sheets (tfsf_sheet)
   float Chh, Che, dx;
+  const float *mu;      // per-cell H coefficients of CreateMu [3][IJK] (K2), or null
 };
 
 // Pointers taken from the argument block are generic; casting them to the global address space
@@ -692,8 +693,14 @@ __device__ __forceinline__ void tfsf_sheet(const GridDev &g, const PWDev &pw, co
 // same arrays.)
 
 // Loads of one plane of the march (see rows_body).
+template <int CPL, bool MU>
+struct RowPlaneMu {};  // per-cell H coefficients of a plane (mu runs only; empty otherwise)
 template <int CPL>
-struct RowPlane {
+struct RowPlaneMu<CPL, true> {
+  float mux[CPL], muy[CPL], muz[CPL], muxm[CPL], muzm[CPL];
+};
+template <int CPL, bool MU = false>
+struct RowPlane : RowPlaneMu<CPL, MU> {
   float nx[CPL], ny[CPL], nz[CPL], nym[CPL];     // E(k+1) of row j, Ey(k+1) of row j-1
   float exm[CPL], ezm[CPL], exq[CPL], ezq[CPL];  // Ex, Ez of rows j-1 and j+1 at plane k
   float hx[CPL], hy[CPL], hz[CPL];               // H of row j at plane k (old state)
@@ -708,7 +715,9 @@ struct RowPlane {
 // Workgroup size bound of the row kernel per columns per lane: wider lanes hold more registers.
 constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
 
-template <bool IDX, bool BUF, int CPL, int PART>
+// MU: per-cell H coefficients (CreateMu): H = Chh H + C(i,j,k) curl E, which is K2's H + C curl E
+// bit for bit since Chh is exactly 1.
+template <bool IDX, bool BUF, int CPL, int PART, bool MU = false>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
   // it is loaded with scalar loads into scalar registers (also where the block's address was
@@ -814,6 +823,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   // on the last, row j+1 is row j: values read there are not used); plane k+1 at IJ
   const Acc<BUF> Fc(pa->Fc, 6 * cs * 4), Fn(pa->Fn, 6 * cs * 4);
   const Acc<BUF> Mt(pa->mat, IDX ? IJK : 0), Cf(pa->coef, IDX ? 0 : 24 * IJK);
+  const Acc<BUF> Cm(MU ? pa->mu : nullptr, MU ? 12 * IJK : 0);
+  const O cm1 = (O)(IJK * 4), cm2 = 2 * cm1;
   const O c1 = (O)(cs * 4), c2 = 2 * c1, c3 = 3 * c1, c4 = 4 * c1, c5 = 5 * c1;
   const O r1 = uni(has_jm ? (O)(I * 4) : (O)0), r2 = uni(r1 + (j + 1 < J ? (O)(I * 4) : (O)0)), pz = (O)(IJ * 4);
 
@@ -917,7 +928,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     float *const sRq = sRow + (size_t)(4 * r) * Wrow + (on ? ib : 0);  // this row's LDS slots, column ib
 
     // ---- loads of plane index it ----
-    auto load = [&](RowPlane<CPL> &Lp, int it, auto own_tag) {
+    auto load = [&](RowPlane<CPL, MU> &Lp, int it, auto own_tag) {
       constexpr bool own = decltype(own_tag)::value;
       const int k = uni(k_first + it);
       const O o = o0 + (O)it * pz;
@@ -938,6 +949,10 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           Fc.ld(o, c5, Lp.hzm);
         }
         if (lastq) { Fc.ld(o2, 0, Lp.exq); Fc.ld(o2, c2, Lp.ezq); }
+        if constexpr (MU) {
+          Cm.ld(o1, 0, Lp.mux); Cm.ld(o1, cm1, Lp.muy); Cm.ld(o1, cm2, Lp.muz);
+          if (own && firstr) { Cm.ld(o, 0, Lp.muxm); Cm.ld(o, cm2, Lp.muzm); }
+        }
         if (IDX && own) Mt.ldm(o1 >> 2, 0, Lp.mt);
         // y-slab psi of rows j and j-1
         if (SLAB && ys) {
@@ -1000,7 +1015,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     };
 
     // ---- plane index it: H(k) of row j, Hx, Hz(k) of row j-1, then E(k) of row j ----
-    auto compute = [&](RowPlane<CPL> &Lp, int it, auto own_tag) {
+    auto compute = [&](RowPlane<CPL, MU> &Lp, int it, auto own_tag) {
       const int k = uni(k_first + it);
       constexpr bool own = decltype(own_tag)::value;  // a plane of the chunk (the plane below only feeds H(k-1))
       const O o1 = o0 + (O)it * pz + r1;
@@ -1048,9 +1063,11 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         if (h_jk && (!EDGE || bit(f, FS * c + F_H))) {
           const float dyz = EDGE ? (ez1 - ez[c]) * kH[c] : ez1 - ez[c];
           const float dzy = EDGE ? (ey1 - ey[c]) * kH[c] : ey1 - ey[c];
-          hx[c] = Chh * hx[c] + Che * (-(Lp.ezq[c] - ez[c]) * kyH + (Lp.ny[c] - ey[c]) * kzH);
-          hy[c] = Chh * hy[c] + Che * (-(Lp.nx[c] - ex[c]) * kzH + dyz);
-          hz[c] = Chh * hz[c] + Che * (-dzy + (Lp.exq[c] - ex[c]) * kyH);
+          float cx = Che, cy = Che, cz = Che;
+          if constexpr (MU) { cx = Lp.mux[c]; cy = Lp.muy[c]; cz = Lp.muz[c]; }
+          hx[c] = Chh * hx[c] + cx * (-(Lp.ezq[c] - ez[c]) * kyH + (Lp.ny[c] - ey[c]) * kzH);
+          hy[c] = Chh * hy[c] + cy * (-(Lp.nx[c] - ex[c]) * kzH + dyz);
+          hz[c] = Chh * hz[c] + cz * (-dzy + (Lp.exq[c] - ex[c]) * kyH);
         }
         if (EDGE && bit(f, FS * c + F_XH)) {
           if ((ah & 2) && xj0 && xk0) hy[c] = hy[c] + (Che * Lp.vhy) * sxh;
@@ -1159,8 +1176,10 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
           if (hm_k && (!EDGE || bit(f, FS * c + F_H))) {
             const float dzy = EDGE ? (eym1 - eym[c]) * kH[c] : eym1 - eym[c];
-            hxm[c] = Chh * hxm[c] + Che * (-(ez[c] - Lp.ezm[c]) * kyHm + (Lp.nym[c] - eym[c]) * kzH);
-            hzm[c] = Chh * hzm[c] + Che * (-dzy + (ex[c] - Lp.exm[c]) * kyHm);
+            float cx = Che, cz = Che;
+            if constexpr (MU) { cx = Lp.muxm[c]; cz = Lp.muzm[c]; }
+            hxm[c] = Chh * hxm[c] + cx * (-(ez[c] - Lp.ezm[c]) * kyHm + (Lp.nym[c] - eym[c]) * kzH);
+            hzm[c] = Chh * hzm[c] + cz * (-dzy + (ex[c] - Lp.exm[c]) * kyHm);
           }
           if (EDGE && bit(f, FS * c + F_XH) && (ah & 4) && xjm1 && xk1) hzm[c] = hzm[c] - (Che * Lp.vhzm) * sxh;
         }
@@ -1432,7 +1451,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       }
     };
 
-    RowPlane<CPL> A;
+    RowPlane<CPL, MU> A;
     // lanes outside the grid (edge waves) never load: their plane values stay zero
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
@@ -1442,6 +1461,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       A.qHxy[c] = A.qHzy[c] = A.qExy[c] = A.qEzy[c] = A.qHxym[c] = A.qHzym[c] = 0.0f;
       A.qHxz[c] = A.qHyz[c] = A.qExz[c] = A.qEyz[c] = A.qHxzm[c] = 0.0f;
       A.mt[c] = 0;
+      if constexpr (MU) A.mux[c] = A.muy[c] = A.muz[c] = A.muxm[c] = A.muzm[c] = 0.0f;
     }
     // The plane below the chunk (H only) is peeled off the march, so that every load of a march
     // iteration is consumed within it: a load left in flight across the back edge (the material
@@ -1529,7 +1549,7 @@ __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
 }
 
 // The row kernel's workgroups [0, nwg); the slab part's launch may carry tail workgroups after them.
-template <bool IDX, bool BUF, int CPL, int PART>
+template <bool IDX, bool BUF, int CPL, int PART, bool MU>
 __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n, int nwg,
                                                                const TailArgs ta) {
   const int b = blockIdx.x;
@@ -1537,7 +1557,7 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *_
     tail_body<IDX>(ta, b - nwg);
     return;
   }
-  rows_body<IDX, BUF, CPL, PART>(pa, xcd_remap(b, nwg));
+  rows_body<IDX, BUF, CPL, PART, MU>(pa, xcd_remap(b, nwg));
 }
 // The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
 // (measured 4% faster at 256^3 than the 68 the compiler picks freely).
@@ -1567,7 +1587,7 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows_batch(const Batch
   const int b = blockIdx.x, g = find_grid(tab, ng, b);
   const int par = (tab[g].cur0 + step) & 1;
   const int off = tab[g].off, nb = tab[g + 1].off - off;
-  rows_body<IDX, BUF, CPL, PART>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
+  rows_body<IDX, BUF, CPL, PART, false>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
 }
 
 
@@ -1753,15 +1773,25 @@ static unsigned rows_lds(const StepArgs &a, int part) {
   return R > 1 ? 16u * R * a.Wrow : 0u;
 }
 
-template <bool IDX, bool BUF, int PART>
-static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
+template <bool IDX, bool BUF, int PART, bool MU>
+static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
                           int n, const TailArgs &ta) {
   const dim3 grid(nwg + ntail);
   if (grid.x == 0) return;
-  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART>), grid, blk, lds, st, d, n, nwg, ta);
-  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART>), grid, blk, lds, st, d, n, nwg, ta);
-  else if (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF>), grid, blk, lds, st, d, n);
-  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART>), grid, blk, lds, st, d, n, nwg, ta);
+  if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART, MU>), grid, blk, lds, st, d, n, nwg, ta);
+  else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART, MU>), grid, blk, lds, st, d, n, nwg, ta);
+  else if constexpr (PART == 0 && !MU) hipLaunchKernelGGL((k_rows_c1<IDX, BUF>), grid, blk, lds, st, d, n);
+  // (with mu the interior part takes the register-unbounded kernel: held to 64 registers its
+  // march would spill)
+  else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART, MU>), grid, blk, lds, st, d, n, nwg, ta);
+}
+
+// mu: the per-cell H coefficients of CreateMu are in use
+template <bool IDX, bool BUF, int PART>
+static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
+                          int n, const TailArgs &ta, bool mu) {
+  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta);
+  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta);
 }
 
 // part: 0 the interior part, 1 the slab part (with the tail workgroups of ta, if any), -1 both
@@ -1771,7 +1801,9 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // buffer addressing when every array the kernel touches (a state set, a psi set, the material
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
-  const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
+  const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim &&
+                   (!a.mu || 12 * a.IJK < lim);
+  const bool mu = a.mu != nullptr;
   const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
   TailArgs ta{};
   if (tail && part != 0) ta = *tail;
@@ -1779,11 +1811,11 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
   }
   return hipGetLastError();
 }
@@ -1795,7 +1827,7 @@ size_t step_args_size() { return sizeof(StepArgs); }
 // not supported by the fused step.
 int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   const GridDev &g = f.g;
-  if (!fused_supported(g) || f.mu) return -1;
+  if (!fused_supported(g)) return -1;
   if (f.mat && f.ntab > 256) return -1;
   int cpl = 1, S = 1;
   if (!row_layout(g.I, tune.cpl, cpl, S)) return -1;
@@ -1853,6 +1885,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
   a.mat = f.mat; a.coef = f.coef; a.prm = f.prm;
   a.xs = f.xs;
+  a.mu = f.Chmu;
   a.Chh = f.Chh; a.Che = f.Che; a.dx = f.dx;
   std::memcpy(out, &a, sizeof(a));
   return a.nwg0 + a.nwg1;

@@ -81,7 +81,7 @@ struct FusedDev {
   int N1 = 0, act_h = 0, act_e = 0;
   float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
   float Chh = 1, Che = 0, dx = 0;
-  bool mu = false;  // per-cell mu active: outside the fused step (the per-op kernels run instead)
+  const float *Chmu = nullptr;  // per-cell H coefficients of CreateMu [3][I*J*K] (K2), or null
 };
 // Argument block of the fused kernel: built on the host (make_step_args returns the workgroup
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.

@@ -94,9 +94,10 @@ def test_per_op_with_mu(mode):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
-def test_fused_run_with_mu_takes_per_op_path():
-    """With per-cell mu the fused entry points (Run, Step, and a batch) run the per-op kernels in
-    the reference order: same bits as the oracle; CreateMu after ReadySimulation on the GPU."""
+def test_fused_run_with_mu():
+    """With per-cell mu the fused entry points (Run, Step) take the row kernel's per-cell H
+    coefficients and a batch runs the grid on its own: same bits as the oracle's K2; CreateMu after
+    ReadySimulation on the GPU."""
     from maxwell_solver_amd import Batch
     from maxwell_solver_amd import scenarios as S
     sc = _scenario(I=48, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5))
@@ -173,6 +174,24 @@ def test_step_currents_with_probes():
     g.set_graph_steps(8)      # not used: the run records probes
     g.run(0, sc.T, 0)
     _gic_oracle_steps(o, 0, sc.T, box)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+# row layouts with per-cell mu: one column per lane (split and unsplit), two and four per lane,
+# several rows per tile, a plain-wave grid and an oblique one
+@pytest.mark.parametrize("I,rows,cpl,mode", [(64, 0, 1, 5), (52, 0, 0, 3), (200, 0, 2, 5), (264, 1, 4, 5),
+                                             (128, 4, 1, 1), (128, 0, -1, 5), (384, 0, 0, 5)])
+def test_fused_layouts_with_mu(I, rows, cpl, mode):
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=I, J=37, K=45, T=40, mode=mode, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7) if mode == 5 else None)
+    mu = _mu(sc, 13)
+    g, o = _backends()
+    S.setup(g, sc, mu=mu)
+    S.setup(o, sc, mu=mu)
+    g.set_tuning(rows, 0, cpl)
+    g.run(0, sc.T, mode)
+    o.run(0, sc.T, mode)
     assert_close(compare_state(g, o), exact=True)
     np.testing.assert_array_equal(g.probes(), o.probes())
 
