@@ -517,18 +517,23 @@ struct Acc<true> {
   __device__ __forceinline__ float ld1(O v, O u) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)u, 0));
   }
+  // Stores are non-temporal (cache policy nt): nothing of the new state is read again within the
+  // step, and streaming its 370 MB per 256^3 step past the L2 keeps the halo rows the
+  // neighbouring tiles re-read resident (measured: interior part 177.0 -> 172.9 us, slab part
+  // 67.6 -> 64.4 us).
+  static constexpr int kStoreNT = 2;
   template <int CPL>
   __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
     if constexpr (CPL == 4) {
-      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, f4v{x[0], x[1], x[2], x[3]}), r, (int)v, (int)u, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, f4v{x[0], x[1], x[2], x[3]}), r, (int)v, (int)u, kStoreNT);
     } else if constexpr (CPL == 2) {
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, f2v{x[0], x[1]}), r, (int)v, (int)u, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, f2v{x[0], x[1]}), r, (int)v, (int)u, kStoreNT);
     } else {
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), r, (int)v, (int)u, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), r, (int)v, (int)u, kStoreNT);
     }
   }
   __device__ __forceinline__ void st1(O v, O u, float x) const {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)u, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)u, kStoreNT);
   }
   // CPL consecutive bytes at byte offset v + u
   template <int CPL>
@@ -564,18 +569,19 @@ struct Acc<false> {
     }
   }
   __device__ __forceinline__ float ld1(O v, O u) const { return *(const AS1 float *)(p + u + v); }
+  // non-temporal stores, as in buffer mode
   template <int CPL>
   __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
     AS1 char *q = p + u + v;
     if constexpr (CPL == 4) {
-      *(AS1 f4v *)q = f4v{x[0], x[1], x[2], x[3]};
+      __builtin_nontemporal_store(f4v{x[0], x[1], x[2], x[3]}, (AS1 f4v *)q);
     } else if constexpr (CPL == 2) {
-      *(AS1 f2v *)q = f2v{x[0], x[1]};
+      __builtin_nontemporal_store(f2v{x[0], x[1]}, (AS1 f2v *)q);
     } else {
-      *(AS1 float *)q = x[0];
+      __builtin_nontemporal_store(x[0], (AS1 float *)q);
     }
   }
-  __device__ __forceinline__ void st1(O v, O u, float x) const { *(AS1 float *)(p + u + v) = x; }
+  __device__ __forceinline__ void st1(O v, O u, float x) const { __builtin_nontemporal_store(x, (AS1 float *)(p + u + v)); }
   template <int CPL>
   __device__ __forceinline__ void ldm(O v, O u, int (&m)[CPL]) const {
     const AS1 uint8_t *q = (const AS1 uint8_t *)(p + u + v);
