@@ -517,23 +517,24 @@ struct Acc<true> {
   __device__ __forceinline__ float ld1(O v, O u) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)u, 0));
   }
-  // Stores are non-temporal (cache policy nt): nothing of the new state is read again within the
-  // step, and streaming its 370 MB per 256^3 step past the L2 keeps the halo rows the
-  // neighbouring tiles re-read resident (measured: interior part 177.0 -> 172.9 us, slab part
-  // 67.6 -> 64.4 us).
-  static constexpr int kStoreNT = 2;
-  template <int CPL>
+  // Store cache policy: NT (non-temporal) or write-back.  Nothing of the new state is read again
+  // within a step; when the grid's state is larger than the 256 MB Infinity Cache, streaming it
+  // past the L2 keeps the halo rows the neighbouring tiles re-read resident (256^3: interior part
+  // 177.0 -> 172.9 us, slab part 67.6 -> 64.4 us); when the state fits, write-back lets the next
+  // step read it from the Infinity Cache (250x280x50: 87.1 -> 71.3 us per step).
+  template <int CPL, bool NT = true>
   __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
+    constexpr int aux = NT ? 2 : 0;
     if constexpr (CPL == 4) {
-      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, f4v{x[0], x[1], x[2], x[3]}), r, (int)v, (int)u, kStoreNT);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, f4v{x[0], x[1], x[2], x[3]}), r, (int)v, (int)u, aux);
     } else if constexpr (CPL == 2) {
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, f2v{x[0], x[1]}), r, (int)v, (int)u, kStoreNT);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, f2v{x[0], x[1]}), r, (int)v, (int)u, aux);
     } else {
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), r, (int)v, (int)u, kStoreNT);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), r, (int)v, (int)u, aux);
     }
   }
   __device__ __forceinline__ void st1(O v, O u, float x) const {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)u, kStoreNT);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)v, (int)u, 2);
   }
   // CPL consecutive bytes at byte offset v + u
   template <int CPL>
@@ -569,8 +570,8 @@ struct Acc<false> {
     }
   }
   __device__ __forceinline__ float ld1(O v, O u) const { return *(const AS1 float *)(p + u + v); }
-  // non-temporal stores, as in buffer mode
-  template <int CPL>
+  // non-temporal stores (arrays over 4 GiB: the state is far larger than the Infinity Cache)
+  template <int CPL, bool NT = true>
   __device__ __forceinline__ void st(O v, O u, const float (&x)[CPL]) const {
     AS1 char *q = p + u + v;
     if constexpr (CPL == 4) {
@@ -723,7 +724,8 @@ constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
 
 // MU: per-cell H coefficients (CreateMu): H = Chh H + C(i,j,k) curl E, which is K2's H + C curl E
 // bit for bit since Chh is exactly 1.
-template <bool IDX, bool BUF, int CPL, int PART, bool MU = false>
+// WB: write-back stores (a state that fits in the Infinity Cache), else non-temporal.
+template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
   // it is loaded with scalar loads into scalar registers (also where the block's address was
@@ -1280,9 +1282,9 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
       if (own) {
         if (lon) {
-          Fn.st(o1, c3, hx);
-          Fn.st(o1, c4, hy);
-          Fn.st(o1, c5, hz);
+          Fn.template st<CPL, !WB>(o1, c3, hx);
+          Fn.template st<CPL, !WB>(o1, c4, hy);
+          Fn.template st<CPL, !WB>(o1, c5, hz);
         }
         // E(i, j, k): K3 GPU_updateE (src/gpu.h:745-762), TF/SF, CPML Y-Z-X (src/gpu.h:765-868)
         float ce[3][CPL], ch[3][CPL];
@@ -1417,9 +1419,9 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           }
         }
         if (lon) {
-          Fn.st(o1, 0, ux);
-          Fn.st(o1, c1, uy);
-          Fn.st(o1, c2, uz);
+          Fn.template st<CPL, !WB>(o1, 0, ux);
+          Fn.template st<CPL, !WB>(o1, c1, uy);
+          Fn.template st<CPL, !WB>(o1, c2, uz);
           // psi: the four members of each slab column in one 16-byte store; every element is written
           // each step (updated, or copied outside the masks)
           if (EDGE && bit(f, F_SLAB)) {
@@ -1428,7 +1430,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             for (int c = 0; c < CPL; ++c)
               if (slab_col(f, c)) {
                 const float v[4] = {Lp.pEy[c], Lp.pEz[c], Lp.pHy[c], Lp.pHz[c]};
-                Pn.template st<4>(so[c], pq, v);
+                Pn.template st<4, !WB>(so[c], pq, v);
               }
           }
           if (SLAB && ys) {
@@ -1436,7 +1438,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
               const float v[4] = {Lp.qExy[c], Lp.qEzy[c], Lp.qHxy[c], Lp.qHzy[c]};
-              Pn.template st<4>(oc + (O)(16 * c), u, v);
+              Pn.template st<4, !WB>(oc + (O)(16 * c), u, v);
             }
           }
           if (SLAB && zs) {
@@ -1444,7 +1446,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
               const float v[4] = {Lp.qExz[c], Lp.qEyz[c], Lp.qHxz[c], Lp.qHyz[c]};
-              Pn.template st<4>(oc + (O)(16 * c), u, v);
+              Pn.template st<4, !WB>(oc + (O)(16 * c), u, v);
             }
           }
         }
@@ -1555,7 +1557,7 @@ __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
 }
 
 // The row kernel's workgroups [0, nwg); the slab part's launch may carry tail workgroups after them.
-template <bool IDX, bool BUF, int CPL, int PART, bool MU>
+template <bool IDX, bool BUF, int CPL, int PART, bool MU, bool WB = false>
 __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n, int nwg,
                                                                const TailArgs ta) {
   const int b = blockIdx.x;
@@ -1563,14 +1565,14 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *_
     tail_body<IDX>(ta, b - nwg);
     return;
   }
-  rows_body<IDX, BUF, CPL, PART, MU>(pa, xcd_remap(b, nwg));
+  rows_body<IDX, BUF, CPL, PART, MU, WB>(pa, xcd_remap(b, nwg));
 }
 // The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
 // (measured 4% faster at 256^3 than the 68 the compiler picks freely).
-template <bool IDX, bool BUF>
+template <bool IDX, bool BUF, bool WB = false>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_rows_c1(const StepArgs *__restrict__ pa, int n) {
-  rows_body<IDX, BUF, 1, 0>(pa, xcd_remap(blockIdx.x, gridDim.x));
+  rows_body<IDX, BUF, 1, 0, false, WB>(pa, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Batched launches cover the workgroups of many grids: tab[g].off is grid g's first workgroup
@@ -1781,9 +1783,16 @@ static unsigned rows_lds(const StepArgs &a, int part) {
 
 template <bool IDX, bool BUF, int PART, bool MU>
 static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta) {
+                          int n, const TailArgs &ta, bool wb) {
   const dim3 grid(nwg + ntail);
   if (grid.x == 0) return;
+  if constexpr (BUF && !MU) {  // write-back variants: one column per lane (grids that fit the cache)
+    if (wb && cpl == 1) {
+      if constexpr (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF, true>), grid, blk, lds, st, d, n);
+      else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART, false, true>), grid, blk, lds, st, d, n, nwg, ta);
+      return;
+    }
+  }
   if (cpl == 4) hipLaunchKernelGGL((k_rows<IDX, BUF, 4, PART, MU>), grid, blk, lds, st, d, n, nwg, ta);
   else if (cpl == 2) hipLaunchKernelGGL((k_rows<IDX, BUF, 2, PART, MU>), grid, blk, lds, st, d, n, nwg, ta);
   else if constexpr (PART == 0 && !MU) hipLaunchKernelGGL((k_rows_c1<IDX, BUF>), grid, blk, lds, st, d, n);
@@ -1795,9 +1804,9 @@ static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3
 // mu: the per-cell H coefficients of CreateMu are in use
 template <bool IDX, bool BUF, int PART>
 static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta, bool mu) {
-  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta);
-  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta);
+                          int n, const TailArgs &ta, bool mu, bool wb) {
+  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb);
+  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb);
 }
 
 // part: 0 the interior part, 1 the slab part (with the tail workgroups of ta, if any), -1 both
@@ -1810,6 +1819,9 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim &&
                    (!a.mu || 12 * a.IJK < lim);
   const bool mu = a.mu != nullptr;
+  // the state (two field sets, two psi sets, materials) fits in the 256 MB Infinity Cache
+  const size_t state = 2 * 24 * a.cs + 2 * 16 * (a.sx + a.sy + a.sz) + (a.mat ? a.IJK : 24 * a.IJK);
+  const bool wb = state <= ((size_t)256 << 20);
   const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
   TailArgs ta{};
   if (tail && part != 0) ta = *tail;
@@ -1817,11 +1829,11 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
   }
   return hipGetLastError();
 }
