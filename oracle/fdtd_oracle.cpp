@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle: it may be built, loaded and
  * called only by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and only
- * as the checker (or the timed CPU baseline), never as a product code path.
+ * to check results against (or as the timed CPU baseline), never as a product code path.
  *
  * The reference has no CPU solver (src/cpucode/FunLib.cpp only generates a waveform), so this is
  * a restatement of the reference's device arithmetic (src/gpu.h) and host setup

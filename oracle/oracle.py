@@ -1,7 +1,7 @@
 """ctypes front-end of the CPU oracle (fdtd_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker.  It mirrors the op vocabulary of the reference MEX gateway
+cpu_baseline leg, to check results against and as the timed CPU baseline.  It mirrors the op vocabulary of the reference MEX gateway
 (src/simFDTD.cu:19-187) so that one driver can replay the same op sequence on the oracle and on
 the HIP library.
 """

@@ -434,7 +434,7 @@ def test_oblique_leakage_small_and_axis_case_exact():
         assert peak > 0.01 and leak <= bound * peak, (ang, leak / peak)
 
 
-# ---- an independent vectorised statement of one step ----------
+# ---- an independent vectorised statement of one step (src/gpu.h:592-601, 745-762, 604-868) ----
 # Written from the specification with whole-array slices instead of the oracle's per-cell loops:
 # a wrong psi index, slab mirror, mask or PML order in the oracle makes the two disagree.
 PSI_NAMES = ["Exy", "Exz", "Eyx", "Eyz", "Ezx", "Ezy", "Hxy", "Hxz", "Hyx", "Hyz", "Hzx", "Hzy"]
