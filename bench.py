@@ -274,30 +274,88 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(sc, budget_s=15.0, max_steps=1000):
-    """The CPU oracle (float32, OpenMP over k planes) on the same workload, bounded to ~budget_s."""
+def host_cpu():
+    """CPU model, physical core count and the logical CPUs this process may use (/proc/cpuinfo)."""
+    model, cores, phys, core = None, set(), None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + [""]:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not line.strip():
+                    if core is not None:
+                        cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model or "unknown", len(cores) or None, usable
+
+
+def cpu_baseline(sc, budget_s=15.0):
+    """The CPU oracle (float32, OpenMP over k planes) on the same workload, on this host's cores:
+    built for this CPU (-march=native) when g++ can, timed with 1 thread and with all the threads
+    the process may use (at most OMP_NUM_THREADS when set), each the median of 3 timed samples of a
+    few steps; about budget_s seconds in all."""
+    import statistics
+    import tempfile
+
     import numpy as np
 
-    from oracle.oracle import OracleSim
+    from oracle import oracle as orc
     from maxwell_solver_amd import scenarios as S
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = OracleSim(np.float32)
+    model, phys, usable = host_cpu()
+    build = "-O3 -march=native -fopenmp -ffp-contract=off (built on this host)"
+    tmp = tempfile.mkdtemp(prefix="oracle_native_")
+    try:
+        orc.use_library(orc.build_native(tmp))
+    except Exception as e:  # no compiler or no -march=native: the in-tree x86-64-v3 build
+        build = f"-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (in-tree build; native build failed: {type(e).__name__})"
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(usable, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else usable
+    o = orc.OracleSim(np.float32)
     S.setup(o, sc)
-    steps, t_tot = 0, 0.0
-    chunk = 2
-    while steps < max_steps and t_tot < budget_s:
+    n = 0
+
+    def rate(nthreads, steps):
+        nonlocal n
+        orc.set_threads(nthreads)
+        if n + steps > sc.T:  # the source has T samples: reuse the time indices (timing only)
+            n = 0
         t0 = time.perf_counter()
-        o.run(steps, chunk, sc.mode)
-        t_tot += time.perf_counter() - t0
-        steps += chunk
+        o.run(n, steps, sc.mode)
+        dt = time.perf_counter() - t0
+        n += steps
+        return sc.cells * steps / dt / 1e9, dt
+
+    per = budget_s / 6.0  # three samples each of two thread counts
+    out = {}
+    for label, nt in (("all", threads), ("one", 1)):
+        r0, dt0 = rate(nt, 1)  # calibration step (also warms the caches)
+        steps = max(1, min(sc.T, int(per / dt0)))
+        runs = [rate(nt, steps)[0] for _ in range(3)]
+        out[label] = (statistics.median(runs), runs, steps)
     o.close()
     return {
-        "value": round(sc.cells * steps / t_tot / 1e9, 4),
+        "value": round(out["all"][0], 4),
         "unit": "Gcell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} steps of the {sc.I}x{sc.J}x{sc.K} workload ({sc.cells * steps / 1e9:.2f} G cell-updates, "
-                  f"{t_tot:.1f} s, float32, {threads} OpenMP threads)",
+        "sample": f"{sc.I}x{sc.J}x{sc.K} workload, median of 3 samples of {out['all'][2]} steps with {threads} "
+                  f"OpenMP threads and of {out['one'][2]} steps with 1 thread (float32)",
+        "one_core_value": round(out["one"][0], 4),
+        "all_core_runs": [round(x, 4) for x in out["all"][1]],
+        "one_core_runs": [round(x, 4) for x in out["one"][1]],
+        "cpu_model": model,
+        "physical_cores": phys,
+        "logical_cpus_usable": usable,
+        "build": build,
     }
 
 
@@ -368,7 +426,7 @@ def main():
     ap.add_argument("--cols-per-lane", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["grid", "layered", "batch", "serve", "gic"], default="grid",
                     help="grid: the 256^3 workload (default); layered: the 384x384x256 lossy layered-ground "
@@ -492,7 +550,7 @@ def main():
     g.close()
     if not args.no_cpu:
         base = (S.config3(T=600) if args.workload == "layered" else
-                S.config2(T=1200) if args.size == 256 else sc)
+                S.config2(T=2000) if args.size == 256 else sc)
         rec["cpu_baseline"] = cpu_baseline(base, budget_s=args.cpu_budget)
     print(json.dumps(rec), flush=True)
 

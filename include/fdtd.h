@@ -21,7 +21,10 @@
 extern "C" {
 #endif
 
-typedef struct fdtd_sim fdtd_sim; /* one simulation; reference: class simulation, src/simulation.h:24 */
+typedef struct fdtd_sim fdtd_sim; /* one simulation; reference: class simulation, src/simulation.h:24.
+                                   * Every entry point rejects a handle that fdtd_create did not return
+                                   * or fdtd_destroy has destroyed (simFDTD:InvalidInput); the reference
+                                   * leaves that undefined (src/simFDTD.cu:32-52). */
 typedef struct fdtd_batch fdtd_batch; /* a set of simulations advanced together (no reference counterpart) */
 
 enum {
@@ -82,7 +85,8 @@ int fdtd_update_e_planewave(fdtd_sim *s, const int32_t time[2]);
 /* 'AddCurrents': time = [n, pol], box = [i0,i1,j0,j1,k0,k1]; src/simulation.h:256-286 (gpu.h:55-82) */
 int fdtd_add_currents(fdtd_sim *s, const int32_t time[2], const int32_t box[6]);
 /* 'OutputAnt': out has `size` floats, filled with the probe record [ncomp][T][nAnt];
- * src/simulation.h:621-638 */
+ * src/simulation.h:621-638.  Without antennas (or before ReadySimulation) it prints "Antenna data
+ * is not initialized." to stderr and leaves out untouched, as the reference (:634-637). */
 int fdtd_output_ant(fdtd_sim *s, float *out, uint32_t size);
 /* 'OutputVis': flag 1..3 = Ex,Ey,Ez, 4..6 = Hx,Hy,Hz, 0/-1/-2 = sigma x/y/z; out has `size` >= I*J*K
  * floats; src/simulation.h:572-619 */
@@ -136,6 +140,10 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
 /* Number of snapshots held and their copy to host: out has count*I*J*K floats. */
 int fdtd_snapshot_count(fdtd_sim *s, int32_t *count);
 int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size);
+/* Device bytes the simulation holds now (state, materials, tables, line histories, probe record,
+ * argument blocks and snapshot buffers).  Repeated UpdateCuda, Init1D and SetAntennaLocations calls
+ * reuse or replace their buffers, so the figure does not grow with them. */
+int fdtd_device_bytes(fdtd_sim *s, uint64_t *bytes);
 /* Wait for all queued work of this simulation. */
 int fdtd_synchronize(fdtd_sim *s);
 /* Read any internal array to host for verification: what = 0..5 fields, 6..11 Ce/Ch per cell,
@@ -207,6 +215,11 @@ int fdtd_server_submit(fdtd_server *srv, fdtd_sim *s, float rate, const float *p
 int fdtd_server_round(fdtd_server *srv, int32_t *active, int32_t *queued);
 int fdtd_server_collect(fdtd_server *srv, int64_t *ids, double *times, int32_t max, int32_t *n);
 int fdtd_server_release(fdtd_server *srv, int64_t id);
+/* Error of a request whose set-up failed at admission (ReadySimulation / Init1D): *code = its
+ * FDTD_ERR_* (0 for a request admitted normally) and its message in msg (msg_len bytes, may be
+ * NULL).  Such a request is completed at once: collect returns it with admission time -1, it is
+ * charged nothing against the budget, and release frees what its set-up allocated. */
+int fdtd_server_request_error(fdtd_server *srv, int64_t id, int32_t *code, char *msg, int32_t msg_len);
 /* Device bytes a request is charged while admitted (its upper bound, fixed at submission). */
 int fdtd_server_request_bytes(fdtd_server *srv, int64_t id, uint64_t *bytes);
 /* Bytes charged to admitted requests now and at most so far; cell updates done. */

@@ -15,6 +15,8 @@
 #include <string>
 #include <unordered_map>
 #include <map>
+#include <mutex>
+#include <set>
 #include <vector>
 
 #include "../../include/fdtd.h"
@@ -133,8 +135,9 @@ struct fdtd_sim {
   std::vector<hipEvent_t> ev;
   double kernel_ms = 0, rows_ms = 0;  // summed step / row-kernel event times of the last timed run
   int64_t kernel_launches = 0;
-  std::vector<void *> allocs;
-  size_t dev_bytes = 0;  // device bytes allocated so far (snapshots excluded)
+  std::map<void *, size_t> allocs;  // device allocations and their bytes
+  size_t dev_bytes = 0;  // device bytes held (snapshots excluded)
+  std::vector<void *> line_allocs;  // buffers of the last Init1D (line histories, its uploads, tables)
   // Step currents: AddCurrents(n, pol, box) after every step of fdtd_step / fdtd_run, the loop of
   // GICsimulation.m:119-124 (pol 0: none).
   int step_pol = 0, step_box[6] = {0, 0, 0, 0, 0, 0};
@@ -151,7 +154,7 @@ struct fdtd_sim {
   ~fdtd_sim() {
     if (stream) hipStreamSynchronize(stream);
     if (gexec) hipGraphExecDestroy(gexec);
-    for (void *p : allocs) hipFree(p);
+    for (auto &kv : allocs) hipFree(kv.first);
     for (float *p : dSnaps) hipFree(p);
     for (hipEvent_t e : ev) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
@@ -163,10 +166,30 @@ struct fdtd_sim {
     if (n == 0) n = 1;
     hipError_t e = hipMalloc(&q, n * sizeof(Tp));
     if (e != hipSuccess) return fail(FDTD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    allocs.push_back(q);
+    allocs[q] = n * sizeof(Tp);
     dev_bytes += n * sizeof(Tp);
     *p = (Tp *)q;
     return 0;
+  }
+  // Frees a buffer of dalloc once the simulation's queued work is done, and sets *p to null.
+  template <typename Tp>
+  void dfree(Tp **p) {
+    if (!*p) return;
+    auto it = allocs.find((void *)*p);
+    if (it != allocs.end()) {
+      if (stream) hipStreamSynchronize(stream);
+      hipFree(it->first);
+      dev_bytes -= it->second;
+      allocs.erase(it);
+    }
+    *p = nullptr;
+  }
+  // The line buffers of the previous Init1D (before a new one replaces them).
+  void free_line_buffers() {
+    for (void *q : line_allocs) dfree(&q);
+    line_allocs.clear();
+    for (int m = 0; m < 8; ++m) dfree(&dPwTab[m]);
+    dE1h = dH1h = dEinv = dHinv = nullptr;
   }
 
   // Upper bound of the device bytes the simulation holds once ready, with its 1D line and after
@@ -377,6 +400,9 @@ struct fdtd_sim {
     if (!init_antenna || !init_ready) return 0;
     if (int rc = resolve_antenna()) return rc;
     port_len = probe_comps.size() * (size_t)T * nAnt;
+    dfree(&dPort);
+    dfree(&dProbeIdx);
+    dfree(&dProbeComps);
     if (int rc = dalloc(&dPort, port_len)) return rc;
     HIPCHK(hipMemsetAsync(dPort, 0, port_len * sizeof(float), stream));
     if (int rc = upload(&dProbeIdx, coax)) return rc;
@@ -384,13 +410,23 @@ struct fdtd_sim {
     return 0;
   }
 
+  // Materials to the device (ReadySimulation, UpdateCuda).  The buffers of the mode in use are
+  // allocated once and copied into afterwards, as the reference copies into its arrays
+  // (src/simulation.h:1116-1127); switching between the index and the per-cell form frees the
+  // other form's buffers.
   int upload_coefficients() {
     args_gen++;
     if (index_mode) {
-      if (int rc = upload(&dMat, cell_mat)) return rc;
-      if (int rc = upload(&dTab, coef_tab)) return rc;
+      dfree(&dCoefFull);
+      for (int c = 0; c < 3; ++c) dCe[c] = dCh[c] = nullptr;
+      if (!dMat && dalloc(&dMat, IJK)) return FDTD_ERR_NOMEM;
+      if (!dTab && dalloc(&dTab, (size_t)256 * 6)) return FDTD_ERR_NOMEM;
+      HIPCHK(hipMemcpyAsync(dMat, cell_mat.data(), IJK, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dTab, coef_tab.data(), coef_tab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     } else {
-      if (int rc = dalloc(&dCoefFull, 6 * IJK)) return rc;
+      dfree(&dMat);
+      dfree(&dTab);
+      if (!dCoefFull && dalloc(&dCoefFull, 6 * IJK)) return FDTD_ERR_NOMEM;
       for (int c = 0; c < 3; ++c) {
         dCe[c] = dCoefFull + c * IJK;
         dCh[c] = dCoefFull + (3 + c) * IJK;
@@ -480,6 +516,8 @@ struct fdtd_sim {
     if (int rc = upload(&dpw, pw)) return rc;
     if (int rc = dalloc(dEh, (size_t)T * N)) return rc;
     if (int rc = dalloc(dHh, (size_t)T * N)) return rc;
+    for (void *q : {(void *)dCe1, (void *)dCb1, (void *)dikE1, (void *)dikH1, (void *)dpw, (void *)*dEh, (void *)*dHh})
+      line_allocs.push_back(q);
     LineDev L;
     L.N = N; L.T = T; L.P = cp.P; L.src = src; L.Ch = Ch1; L.ds = ds;
     L.Ce = dCe1; L.Cb = dCb1; L.ikE = dikE1; L.ikH = dikH1;
@@ -492,7 +530,8 @@ struct fdtd_sim {
     if (!init_ready) return fail(FDTD_ERR_STATE, "Init1D before ReadySimulation");
     std::vector<float> pw((size_t)T);
     for (int i = 0; i < T; ++i) pw[i] = pwsrc[i] * rate;  // src/simulation.h:188-192
-    dE1h = dH1h = dEinv = dHinv = nullptr;
+    have_1d = false;
+    free_line_buffers();  // a repeated Init1D replaces the previous line
     if (pw_mode == 5) {
       N1 = obl.N1;
       line_src = obl.src;
@@ -1117,6 +1156,8 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
 // retires the requests that reached their last step.  Requests admitted in one round join the
 // running ones at the next round boundary; each request's result is that of fdtd_run alone.
 // ------------------------------------------------------------------------------------------
+static void destroy_sim(fdtd_sim *s);
+
 struct fdtd_server {
   struct Req {
     int64_t id = 0;
@@ -1126,6 +1167,8 @@ struct fdtd_server {
     int count = 0, mode = 0, done = 0;
     size_t bytes = 0;
     double t_arrive = 0, t_admit = -1, t_done = -1;
+    int err = 0;          // FDTD_ERR_* of a failed admission (0: none)
+    std::string err_msg;  // its message
   };
   size_t budget = 0, in_use = 0, peak = 0;
   int max_active = 0, steps_per_round = 0;
@@ -1142,7 +1185,7 @@ struct fdtd_server {
 
   ~fdtd_server() {
     for (auto &kv : live) {
-      delete kv.second->s;
+      destroy_sim(kv.second->s);
       delete kv.second;
     }
   }
@@ -1165,7 +1208,18 @@ struct fdtd_server {
     while (!pending.empty() && (int)active.size() < max_active && in_use + pending.front()->bytes <= budget) {
       Req *r = pending.front();
       pending.pop_front();
-      if (int rc = admit(r)) return rc;
+      if (int rc = admit(r)) {
+        // a request whose set-up fails at admission (ReadySimulation or Init1D: a TF/SF inset too
+        // large for its grid, device memory, ...) completes at once as failed: it is charged
+        // nothing, collect returns it (admission time -1), fdtd_server_request_error reports the
+        // error, and release frees what its set-up allocated
+        r->err = rc;
+        r->err_msg = fdtd_last_error();
+        r->t_done = now();
+        r->pw.clear();
+        r->pw.shrink_to_fit();
+        finished.push_back(r);
+      }
     }
     if (active.empty()) return 0;
     batch.items.clear();
@@ -1195,11 +1249,41 @@ struct fdtd_server {
 };
 
 // ------------------------------------------------------------------------------------------
+// Live simulation handles.  The reference's handle is a raw pointer in a uint64 and a deleted one
+// is undefined behaviour (src/simFDTD.cu:32-52); here every entry point checks that the handle
+// was created by fdtd_create and not yet destroyed, and rejects it with simFDTD:InvalidInput
+// otherwise.
+// ------------------------------------------------------------------------------------------
+namespace {
+std::mutex g_live_mu;
+std::set<const fdtd_sim *> g_live;
+
+void register_sim(const fdtd_sim *s) {
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  g_live.insert(s);
+}
+bool sim_live(const fdtd_sim *s) {
+  std::lock_guard<std::mutex> lk(g_live_mu);
+  return g_live.count(s) != 0;
+}
+}  // namespace
+
+// Destroys a simulation and forgets its handle (fdtd_destroy, and the request server's release).
+static void destroy_sim(fdtd_sim *s) {
+  {
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    g_live.erase(s);
+  }
+  delete s;
+}
+
+// ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
-#define NEED(s)                                                     \
-  do {                                                              \
-    if (!(s)) return fail(FDTD_ERR_INVALID, "null simulation handle"); \
+#define NEED(s)                                                                                     \
+  do {                                                                                              \
+    if (!(s)) return fail(FDTD_ERR_INVALID, "null simulation handle");                              \
+    if (!sim_live(s)) return fail(FDTD_ERR_INVALID, "invalid simulation handle (deleted or never created)"); \
   } while (0)
 #define NEED_READY(s)                                                          \
   do {                                                                         \
@@ -1216,12 +1300,13 @@ const char *fdtd_version(void) { return "maxwell-mi355x fdtd 0.1 (gfx950)"; }
 int fdtd_create(fdtd_sim **out) {
   if (!out) return fail(FDTD_ERR_INVALID, "null output pointer");
   *out = new fdtd_sim();
+  register_sim(*out);
   return 0;
 }
 
 int fdtd_destroy(fdtd_sim *s) {
   NEED(s);
-  delete s;
+  destroy_sim(s);
   return 0;
 }
 
@@ -1284,7 +1369,9 @@ int fdtd_set_step_currents(fdtd_sim *s, int32_t pol, const int32_t box[6]) {
   if (pol && !s->init_source) return fail(FDTD_ERR_STATE, "step currents before SetSource");
   s->step_pol = pol;
   for (int q = 0; q < 6; ++q) s->step_box[q] = pol ? box[q] : 0;
-  if (pol && (box[1] < box[0] || box[3] < box[2] || box[5] < box[4])) s->step_pol = 0;  // src/gpu.h:60-61
+  // a box that writes nothing (empty, or a negative start: src/gpu.h:55-82) turns them off
+  if (pol && (box[1] < box[0] || box[3] < box[2] || box[5] < box[4] || box[0] < 0 || box[2] < 0 || box[4] < 0))
+    s->step_pol = 0;
   s->currents_gen++;
   return 0;
 }
@@ -1453,7 +1540,8 @@ int fdtd_add_currents(fdtd_sim *s, const int32_t time[2], const int32_t box[6]) 
   const int n = time[0], pol = time[1];
   if (n < 0 || n >= s->T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
   if (pol < 1 || pol > 3) return 0;
-  if (box[1] < box[0] || box[3] < box[2] || box[5] < box[4]) return 0;  // src/gpu.h:60-61
+  // an empty box or a negative start writes nothing (src/gpu.h:55-82; see current_box)
+  if (box[1] < box[0] || box[3] < box[2] || box[5] < box[4] || box[0] < 0 || box[2] < 0 || box[4] < 0) return 0;
   const int b[6] = {box[0], box[1], box[2], box[3], box[4], box[5]};
   HIPCHK(launch_add_currents(s->stream, s->g, s->fields(s->cur), s->coefs(), pol - 1, b, s->source[n], s->delX));
   return 0;
@@ -1591,10 +1679,24 @@ int fdtd_server_release(fdtd_server *srv, int64_t id) {
   if (r->t_done < 0) return fail(FDTD_ERR_STATE, "server: request not completed");
   for (const auto &f : srv->finished)
     if (f == r) return fail(FDTD_ERR_STATE, "server: request not collected");
-  delete r->s;
-  srv->in_use -= r->bytes;
+  destroy_sim(r->s);
+  if (!r->err) srv->in_use -= r->bytes;  // a failed admission was never charged
   srv->live.erase(it);
   delete r;
+  return 0;
+}
+
+int fdtd_server_request_error(fdtd_server *srv, int64_t id, int32_t *code, char *msg, int32_t msg_len) {
+  if (!srv || !code) return fail(FDTD_ERR_INVALID, "null server handle or output");
+  auto it = srv->live.find(id);
+  if (it == srv->live.end()) return fail(FDTD_ERR_INVALID, "server: unknown request id");
+  const fdtd_server::Req *r = it->second;
+  *code = r->err;
+  if (msg && msg_len > 0) {
+    const size_t n = std::min(r->err_msg.size(), (size_t)msg_len - 1);
+    std::memcpy(msg, r->err_msg.data(), n);
+    msg[n] = 0;
+  }
   return 0;
 }
 
@@ -1733,6 +1835,13 @@ int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size) {
   return 0;
 }
 
+int fdtd_device_bytes(fdtd_sim *s, uint64_t *bytes) {
+  NEED(s);
+  if (!bytes) return fail(FDTD_ERR_INVALID, "null output");
+  *bytes = s->dev_bytes + s->dSnaps.size() * s->IJK * sizeof(float);
+  return 0;
+}
+
 int fdtd_synchronize(fdtd_sim *s) {
   NEED(s);
   if (s->stream) HIPCHK(hipStreamSynchronize(s->stream));
@@ -1740,9 +1849,14 @@ int fdtd_synchronize(fdtd_sim *s) {
 }
 
 int fdtd_output_ant(fdtd_sim *s, float *out, uint32_t size) {
-  NEED_READY(s);
+  NEED(s);
   if (!out) return fail(FDTD_ERR_INVALID, "OutAntSize required.");
-  if (!s->dPort) return fail(FDTD_ERR_STATE, "Antenna data is not initialized.");
+  if (!s->init_ready || !s->dPort) {
+    // src/simulation.h:634-637: without antennas the reference prints this and leaves the output
+    // as it is (the gateway's freshly created, zero-filled array)
+    std::fprintf(stderr, "Antenna data is not initialized.\n");
+    return 0;
+  }
   const size_t n = std::min((size_t)size, s->port_len);
   HIPCHK(hipMemcpyAsync(out, s->dPort, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));

@@ -1661,6 +1661,21 @@ __global__ void __launch_bounds__(64) k_probes_batch(const ProbeJob *__restrict_
 // ------------------------------------------------------------------------------------------
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Cells K17 AddSource_GPU_3D_1DThread writes for an AddCurrents box [i0,i1,j0,j1,k0,k1]
+// (src/gpu.h:55-82 via src/simulation.h:256-286): the box, with its upper ends clipped to the
+// grid (the kernel's i < Imax test, gpu.h:75).  The reference hands the int box to the kernel as
+// size_t, so a negative start either makes i1 < i0 (the kernel returns, gpu.h:60-61) or puts
+// every index past the grid: a box with a negative start, or an empty box, writes nothing.
+bool current_box(const GridDev &g, const int box[6], int &a0, int &a1, int &b0, int &b1, int &c0, int &c1) {
+  if (box[1] < box[0] || box[3] < box[2] || box[5] < box[4]) return false;
+  if (box[0] < 0 || box[2] < 0 || box[4] < 0) return false;
+  a0 = box[0]; b0 = box[2]; c0 = box[4];
+  a1 = box[1] >= g.I ? g.I - 1 : box[1];
+  b1 = box[3] >= g.J ? g.J - 1 : box[3];
+  c1 = box[5] >= g.K ? g.K - 1 : box[5];
+  return a0 <= a1 && b0 <= b1 && c0 <= c1;
+}
+
 hipError_t launch_update_h_all(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co) {
   if (g.I < 2 || g.J < 2 || g.K < 2) return hipSuccess;
   dim3 grid(cdiv(g.I - 1, 64), cdiv(g.J - 1, 4), g.K - 1), blk(64, 4, 1);
@@ -1725,10 +1740,8 @@ hipError_t launch_pw_e(hipStream_t st, const GridDev &g, const FieldSet &F, cons
 
 hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, int comp,
                                const int box[6], float s, float dx, const float *src, const int *dn, int q) {
-  const int a0 = box[0] < 0 ? 0 : box[0], a1 = box[1] >= g.I ? g.I - 1 : box[1];
-  const int b0 = box[2] < 0 ? 0 : box[2], b1 = box[3] >= g.J ? g.J - 1 : box[3];
-  const int c0 = box[4] < 0 ? 0 : box[4], c1 = box[5] >= g.K ? g.K - 1 : box[5];
-  if (a1 < a0 || b1 < b0 || c1 < c0) return hipSuccess;
+  int a0, a1, b0, b1, c0, c1;
+  if (!current_box(g, box, a0, a1, b0, b1, c0, c1)) return hipSuccess;
   const int ni = a1 - a0 + 1, nj = b1 - b0 + 1, nk = c1 - c0 + 1;
   const unsigned nb = cdiv((long)ni * nj * nk, 256);
   if (co.mat) hipLaunchKernelGGL(k_add_currents<true>, dim3(nb), dim3(256), 0, st, g, F, co, comp, a0, b0, c0, ni, nj, nk, s, dx, src, dn, q);
@@ -2086,11 +2099,8 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
     }
     if (fold->cur_comp >= 0) {
       const GridDev &g = f.g;
-      const int *bx = fold->cur_box;
-      const int a0 = bx[0] < 0 ? 0 : bx[0], a1 = bx[1] >= g.I ? g.I - 1 : bx[1];
-      const int b0 = bx[2] < 0 ? 0 : bx[2], b1 = bx[3] >= g.J ? g.J - 1 : bx[3];
-      const int c0 = bx[4] < 0 ? 0 : bx[4], c1 = bx[5] >= g.K ? g.K - 1 : bx[5];
-      if (a1 < a0 || b1 < b0 || c1 < c0) {
+      int a0, a1, b0, b1, c0, c1;
+      if (!current_box(g, fold->cur_box, a0, a1, b0, b1, c0, c1)) {
         fold->cur_done = true;  // an empty box adds nothing
       } else if (b0 >= g.P + 1 && b1 <= g.J - g.P - 1 && c0 >= g.P && c1 <= g.K - g.P - 1) {
         ta.g = g; ta.Fn = sa.nxt; ta.co = sa.co;

@@ -51,6 +51,9 @@ hipError_t launch_pw_e(hipStream_t st, const GridDev &g, const FieldSet &F, cons
 hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet &F, const CoefDev &co, int comp,
                                const int box[6], float s, float dx, const float *src = nullptr, const int *dn = nullptr,
                                int q = 0);
+// Cells an AddCurrents box writes (upper ends clipped to the grid); false when it writes none
+// (an empty box, or a negative start, as the reference's size_t box, src/gpu.h:55-82)
+bool current_box(const GridDev &g, const int box[6], int &a0, int &a1, int &b0, int &b1, int &c0, int &c1);
 // *dn += steps (one thread): the step index of graph-recorded launches, advanced once per replay
 hipError_t launch_advance(hipStream_t st, int *dn, int steps);
 hipError_t launch_line(hipStream_t st, const LineDev &L);

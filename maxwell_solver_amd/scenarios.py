@@ -187,7 +187,7 @@ def config2(T: int = 2000, N: int = 256, n_spheres: int = 40, n_probes: int = 64
     theta, phi, psi = rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)
     return Scenario("config2", N, N, N, T, dx, dt, f0, CONFIG_CPML, inset, 5, (theta, phi, psi),
                     gauss_sine(T, dt, f0), mat_index=idx, mat_table=tab,
-                    probes=_probes(rng, n_probes, N, N, N, P), probe_comps=(3, 5), snap_every=250 if T >= 250 else 0,
+                    probes=_probes(rng, n_probes, N, N, N, P), probe_comps=(3, 5), snap_every=250,
                     snap_comp=3, seed=seed)
 
 

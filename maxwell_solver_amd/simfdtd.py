@@ -31,6 +31,7 @@ EXPORTS = [
     "fdtd_batch_add", "fdtd_batch_clear", "fdtd_batch_run", "fdtd_batch_set_strategy", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
+    "fdtd_server_request_error", "fdtd_device_bytes",
 ]
 
 
@@ -81,6 +82,8 @@ def lib():
             "fdtd_server_release": [vp, C.c_int64],
             "fdtd_server_stats": [vp, u64p, u64p, C.POINTER(C.c_int64)],
             "fdtd_server_request_bytes": [vp, C.c_int64, u64p],
+            "fdtd_server_request_error": [vp, C.c_int64, ip, C.c_char_p, i32],
+            "fdtd_device_bytes": [vp, u64p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -246,6 +249,11 @@ class Simulation:
     def synchronize(self):
         self._chk(lib().fdtd_synchronize(self._h))
 
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        self._chk(lib().fdtd_device_bytes(self._h, C.byref(b)))
+        return b.value
+
     def set_tuning(self, rows_per_group=0, z_chunk=0, cols_per_lane=0):
         self._chk(lib().fdtd_set_tuning(self._h, int(rows_per_group), int(z_chunk), int(cols_per_lane)))
 
@@ -407,6 +415,14 @@ class Server:
         Simulation._chk(lib().fdtd_server_release(self._h, int(rid)))
         sim = self._reqs.pop(rid)
         sim._h = None
+
+    def request_error(self, rid: int):
+        """(FDTD_ERR_* code, message) of a request whose set-up failed at admission; (0, "")
+        otherwise."""
+        code = C.c_int32()
+        msg = C.create_string_buffer(512)
+        Simulation._chk(lib().fdtd_server_request_error(self._h, int(rid), C.byref(code), msg, 512))
+        return code.value, msg.value.decode()
 
     def request_bytes(self, rid: int) -> int:
         b = C.c_uint64()

@@ -32,6 +32,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <omp.h>
+
 namespace orc {
 
 // src/simulation.h:12-21.  pi is the fallback value simulation.h uses when <math.h> does not
@@ -1050,12 +1052,18 @@ struct Sim : Base {
     if (mode < 1 || mode > 3) return 0;
     long a0 = ind[0], a1 = ind[1], b0 = ind[2], b1 = ind[3], c0 = ind[4], c1 = ind[5];
     if (a1 < a0 || b1 < b0 || c1 < c0) return 0;
+    // The reference takes the box as int (simulation.h:260-266) and hands it to the kernel as
+    // size_t (gpu.h:56-58): a negative start becomes a huge unsigned value, so either i1 < i0
+    // holds and the kernel returns (gpu.h:60-61), or every index fails the i < Imax test
+    // (gpu.h:75).  A negative start on any axis therefore adds nothing.
+    if (a0 < 0 || b0 < 0 || c0 < 0) return 0;
     R *E = F[mode - 1].data();
     const R *Cxh = Ch[mode - 1].data();
     const R s = source[(size_t)n];
-    for (long z = std::max(c0, 0L); z <= c1 && z < (long)K; ++z)
-      for (long y = std::max(b0, 0L); y <= b1 && y < (long)J; ++y)
-        for (long x = std::max(a0, 0L); x <= a1 && x < (long)I; ++x) {
+    // the upper ends are clipped per cell (gpu.h:75)
+    for (long z = c0; z <= c1 && z < (long)K; ++z)
+      for (long y = b0; y <= b1 && y < (long)J; ++y)
+        for (long x = a0; x <= a1 && x < (long)I; ++x) {
           size_t id = IDX((size_t)x, (size_t)y, (size_t)z);
           E[id] = E[id] - Cxh[id] * s * delX / delX / delX;
         }
@@ -1105,6 +1113,14 @@ static void widen_out(const std::vector<R> &v, double *out) {
 extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
+
+// OpenMP threads of the parallel loops (the CPU baseline times 1 thread and all of them).
+int orc_set_threads(int n) {
+  if (n < 1) return 1;
+  omp_set_num_threads(n);
+  return 0;
+}
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 void *orc_create(int dbl) {
   Handle *h = new Handle;

@@ -32,12 +32,43 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def build_native(out_dir: str) -> str:
+    """Compile the oracle for this host's CPU (-march=native; otherwise the flags of the Makefile)
+    into out_dir; returns the library path.  Used by the CPU baseline of bench.py."""
+    out = os.path.join(out_dir, "liboracle_native.so")
+    cmd = [os.environ.get("CXX", "g++"), "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+           "-fPIC", "-std=c++17", "-shared", "-o", out, os.path.join(_HERE, "fdtd_oracle.cpp")]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return out
+
+
+def use_library(path: str) -> None:
+    """Load the oracle from `path` instead of the in-tree build (before the first use)."""
+    global _LIB_PATH, _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
+
+
+def set_threads(n: int) -> None:
+    if lib().orc_set_threads(int(n)):
+        raise ValueError("thread count must be >= 1")
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_max_threads.argtypes = []
+        L.orc_max_threads.restype = C.c_int
         vp, i32, f32p, i32p, f64p = C.c_void_p, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_double)
         L.orc_last_error.restype = C.c_char_p
         L.orc_create.restype = vp

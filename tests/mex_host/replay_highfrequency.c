@@ -70,7 +70,31 @@ int main(int argc, char **argv) {
     if (op("SetConst", h, 2, a, NULL)) { printf("FAIL setconst %s\n", g_err_msg); bad = 1; }
     bad |= expect_err("setconst twice", op("setconst", h, 2, a, NULL), "simFDTD:AlreadyInitialized");
     bad |= expect_err("update before ready", op("UpdateE_All", h, 0, NULL, NULL), "simFDTD:InvalidInput");
+    {
+      /* OutputAnt without antennas: the reference prints a message and returns the zero-filled
+       * array (src/simulation.h:634-637) */
+      mxArray *sz = vec(mxUINT32_CLASS, 1), *out = NULL;
+      *(unsigned *)mxGetData(sz) = 10;
+      mxArray *a[1] = {sz};
+      if (op("OutputAnt", h, 1, a, &out) || !out || mxGetNumberOfElements(out) != 10) {
+        printf("FAIL OutputAnt without antennas: %s\n", g_err_msg);
+        bad = 1;
+      } else {
+        const float *v = (const float *)mxGetData(out);
+        for (int q = 0; q < 10; ++q)
+          if (v[q] != 0.0f) { printf("FAIL OutputAnt without antennas: nonzero\n"); bad = 1; break; }
+        printf("ok OutputAnt without antennas -> zeros\n");
+      }
+    }
     if (op("DeleteSimulation", h, 0, NULL, NULL)) bad = 1;
+    /* a deleted handle is rejected (the reference leaves its use undefined) */
+    bad |= expect_err("use after delete", op("UpdateH_All", h, 0, NULL, NULL), "simFDTD:InvalidInput");
+    bad |= expect_err("delete twice", op("DeleteSimulation", h, 0, NULL, NULL), "simFDTD:InvalidInput");
+    {
+      mxArray *bogus = vec(mxUINT64_CLASS, 1);
+      *(unsigned long long *)mxGetData(bogus) = 0x12345678ull;
+      bad |= expect_err("never created", op("ResetFieldAll", bogus, 0, NULL, NULL), "simFDTD:InvalidInput");
+    }
     printf(bad ? "ERRORS FAILED\n" : "ERRORS OK\n");
     return bad;
   }
@@ -103,6 +127,15 @@ int main(int argc, char **argv) {
     OK(op("UpdateMaterial", h, 2, a, NULL));
   }
   OK(op("ReadySimulation", h, 0, NULL, NULL));
+  {
+    /* no antennas were set: OutputAnt leaves the zero-filled array (src/simulation.h:634-637) */
+    mxArray *sz = vec(mxUINT32_CLASS, 1), *ant = NULL;
+    *(unsigned *)mxGetData(sz) = 16;
+    mxArray *a[1] = {sz};
+    OK(op("OutputAnt", h, 1, a, &ant));
+    for (int q = 0; q < 16; ++q)
+      if (((const float *)mxGetData(ant))[q] != 0.0f) { fprintf(stderr, "OutputAnt without antennas: nonzero\n"); return 1; }
+  }
   { const float one = 1.0f; mxArray *a[2] = {f32s(1, &one), f32s(T, src)}; OK(op("Init1D", h, 2, a, NULL)); }
   OK(op("UpdateCuda", h, 0, NULL, NULL));
   OK(op("ResetFieldAll", h, 0, NULL, NULL));

@@ -130,7 +130,8 @@ def _gic_oracle_steps(o, n0, count, box):
 
 # current boxes: in the interior part's cells (applied inside the slab part's launch), in a y-slab
 # row and across z-slab planes (separate launches)
-@pytest.mark.parametrize("box", [[1, 46, 26, 26, 26, 26], [1, 46, 5, 5, 26, 26], [20, 21, 20, 30, 2, 35]])
+@pytest.mark.parametrize("box", [[1, 46, 26, 26, 26, 26], [1, 46, 5, 5, 26, 26], [20, 21, 20, 30, 2, 35],
+                                 [-3, 46, 26, 26, 26, 26]])
 def test_step_currents_graph_replays(box):
     """GICsimulation.m:119-124 loop (no plane wave, AddCurrents after every step) as step currents:
     runs replay one recorded graph of 8 steps (the current sample read on the device), with eager
@@ -196,14 +197,20 @@ def test_fused_layouts_with_mu(I, rows, cpl, mode):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
-def test_per_op_gic_currents():
+# a line, a block past the upper ends (clipped), and boxes with a negative start (the reference's
+# size_t box writes nothing) -- the last two with a line that does write, so the state is not zero
+@pytest.mark.parametrize("box", [[1, 30, 20, 20, 12, 12], [30, 60, 40, 70, 40, 60], [-1, 30, 20, 20, 12, 12],
+                                 [4, 9, 20, 22, -2, 12]])
+def test_per_op_gic_currents(box):
     """GICsimulation.m:119-124 order (no plane wave, AddCurrents after UpdateE_PML)."""
     from maxwell_solver_amd import scenarios as S
     sc = _scenario(mode=0, cpml=CFG_CPML)
     g, o = _backends()
     S.setup(g, sc, init_line=False)
     S.setup(o, sc, init_line=False)
-    box = [1, 30, 20, 20, 12, 12]
+    if min(box[0], box[2], box[4]) < 0:
+        for b in (g, o):
+            b.add_currents(0, 3, [10, 12, 11, 11, 9, 9])
     _per_op_steps(g, sc.T, 0, currents=(1, box))
     _per_op_steps(o, sc.T, 0, currents=(1, box))
     assert_close(compare_state(g, o), exact=True)
@@ -590,3 +597,83 @@ def test_error_paths():
     with pytest.raises(SimFDTDError):
         s.update_h_planewave(0, 3)  # before Init1D
     s.update_h_planewave(0, 9)  # unknown modes are ignored, as the reference
+
+
+@pytest.mark.parametrize("how", ["step", "run1", "run"])
+def test_probe_record_is_the_field_after_each_step(how):
+    """The GPU recorder against the GPU's own fields: record row n of component c and probe a is
+    OutputVis(c) at the cell (P + x, P + y, P + z) after step n -- for single fused steps, runs of
+    one step and one long run (whose slab-part launches gather the previous step's probes)."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=40, K=44, mode=5, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5))
+    g, _ = _backends()
+    S.setup(g, sc)
+    P, comps = sc.P, sc.probe_comps
+    loc = np.asarray(sc.probes)
+    cells = (P + loc[:, 0]) + (P + loc[:, 1]) * sc.I + (P + loc[:, 2]) * sc.I * sc.J
+    seen = []
+    if how == "run":
+        g.run(0, 25, 5)
+        seen = [{c: g.output_vis(c)[cells] for c in comps}]
+    else:
+        for n in range(25):
+            if how == "step":
+                g.step(n, 5)
+            else:
+                g.run(n, 1, 5)
+            seen.append({c: g.output_vis(c)[cells] for c in comps})
+    rec = g.probes()
+    rows = range(24, 25) if how == "run" else range(25)
+    for q, c in enumerate(comps):
+        for m, n in enumerate(rows):
+            np.testing.assert_array_equal(rec[q, n], seen[m][c])
+        assert not rec[q, 25:].any()
+    assert np.abs(rec).max() > 0
+
+
+def test_repeated_update_cuda_and_init_1d_reuse_device_buffers():
+    """UpdateCuda copies into the material buffers it has (the reference copies into its arrays,
+    src/simulation.h:1116-1127) and a repeated Init1D or SetAntennaLocations replaces its buffers:
+    the device bytes stay constant and the results stay equal to the oracle's."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(mode=3, cpml=CFG_CPML)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.run(0, 10, 3)
+    o.run(0, 10, 3)
+    base = g.device_bytes()
+    for _ in range(4):
+        for b in (g, o):
+            b.update_coeffs()
+        assert g.device_bytes() == base
+    for b in (g, o):
+        b.init_1d(1.0, sc.source)
+    assert g.device_bytes() == base
+    loc = np.asarray(sc.probes, dtype=np.int32).T.reshape(-1)
+    g.set_antenna_locations(loc)
+    assert g.device_bytes() == base
+    g.run(10, 20, 3)
+    o.run(10, 20, 3)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes()[:, 10:], o.probes()[:, 10:])
+
+
+# TF/SF box faces at or inside the CPML (inset <= P): the corrections then fall in slab rows, slab
+# columns and slab planes, next to the inline CPML, in the same order as the per-op sequence
+@pytest.mark.parametrize("inset,mode", [(10, 3), (6, 3), (10, 5), (4, 5), (8, 1)])
+def test_tfsf_box_inside_the_cpml(inset, mode):
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=44, K=48, mode=mode, cpml=CFG_CPML, inset=inset, spheres=False,
+                   oblique=(0.8, 2.0, 0.5) if mode == 5 else None)
+    for fused in (False, True):
+        g, o = _backends()
+        S.setup(g, sc)
+        S.setup(o, sc)
+        if fused:
+            g.run(0, sc.T, mode)
+        else:
+            _per_op_steps(g, sc.T, mode)
+        o.run(0, sc.T, mode)
+        assert_close(compare_state(g, o), exact=True)
+        np.testing.assert_array_equal(g.probes(), o.probes())

@@ -149,6 +149,57 @@ def test_server_argument_and_state_checks():
     srv.close()  # destroys the queued simulation it owns
 
 
+def test_failed_admission_is_reported_and_releasable():
+    """A request whose set-up fails at admission (here a TF/SF inset too large for its grid, which
+    ReadySimulation's host part rejects before any device call) completes at once as failed: it is
+    charged nothing, collect returns it with admission time -1, its error is reported, it can be
+    released, and the requests queued behind it are still admitted."""
+    import dataclasses
+    from maxwell_solver_amd import Server, Simulation
+    from maxwell_solver_amd import scenarios as S
+    sc, steps, _ = _requests(1)[0]
+    bad = dataclasses.replace(sc, inset=sc.I // 2)
+    srv = Server(hbm_budget=1 << 40, max_active=1, steps_per_round=8)
+    ids = []
+    for q in range(2):
+        sim = Simulation()
+        S.setup_host(sim, bad)
+        ids.append(srv.submit(sim, steps, bad.mode, source=bad.source))
+    active, queued = srv.round()  # both admissions fail before touching the device
+    assert (active, queued) == (0, 0)
+    done = srv.collect()
+    assert [d[0] for d in done] == ids
+    for rid, ta, tadm, tdone in done:
+        assert tadm == -1 and tdone >= ta
+        code, msg = srv.request_error(rid)
+        assert code == 1 and "inset" in msg
+        srv.release(rid)
+    assert srv.stats()[0] == 0 and srv.stats()[1] == 0
+    srv.close()
+
+
+def test_stale_and_unknown_handles_are_rejected():
+    """Every entry point rejects a handle that was destroyed or never created."""
+    import ctypes as C
+    from maxwell_solver_amd import SimFDTDError, Simulation
+    from maxwell_solver_amd.simfdtd import lib
+    s = Simulation()
+    s.set_const([3e9, 0.01, 1.9e-11], [10, 40, 40, 40, 0])
+    h = s._h
+    s.close()
+    for fn in (lambda: lib().fdtd_update_h_all(h), lambda: lib().fdtd_destroy(h),
+               lambda: lib().fdtd_reset_fields(C.c_void_p(0x1234560))):
+        assert fn() != 0
+        assert lib().fdtd_last_error_id().decode() == "simFDTD:InvalidInput"
+    t = Simulation()
+    t.set_const([3e9, 0.01, 1.9e-11], [10, 40, 40, 40, 0])
+    with pytest.raises(SimFDTDError):
+        t.set_const([3e9, 0.01, 1.9e-11], [10, 40, 40, 40, 0])
+    out = t.output_ant(8)  # no antennas (and not ready): the reference returns the zero array
+    assert out.shape == (8,) and not out.any()
+    t.close()
+
+
 def test_served_workload_generator_is_seeded():
     from maxwell_solver_amd import scenarios as S
     a, b = S.config5(8), S.config5(8)
