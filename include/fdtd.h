@@ -137,6 +137,17 @@ int fdtd_step(fdtd_sim *s, int32_t n, int32_t mode);
  * snap_every > 0, component snap_comp (1..6) is copied to a device snapshot buffer after every
  * snap_every-th step (counted since ResetFieldAll). */
 int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_every, int32_t snap_comp);
+/* fdtd_run without any allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Steps n0 .. n0+count-1 exactly as fdtd_run (same kernels and results) with no
+ * snapshots, timing or graph replay; the TF/SF face sheets and the row kernel's argument blocks
+ * live in a file-scope device array of the library (48 MiB + 8 KiB per GPU, held until the process
+ * ends) instead of buffers of the simulation.  Limit: a grid whose face sheets exceed 24 MiB per
+ * buffer (8 (nj nk + ni nk + ni nj) floats for the TF/SF box extents ni, nj, nk; about a 512^2 box
+ * face) is refused with FDTD_ERR_INVALID before any work.  All calls on one GPU share that array:
+ * calls that overlap on one GPU (other threads or streams) race.  The simulation's state (both
+ * state sets) is allocated by ReadySimulation. */
+int fdtd_run_noalloc(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode);
 /* Number of snapshots held and their copy to host: out has count*I*J*K floats. */
 int fdtd_snapshot_count(fdtd_sim *s, int32_t *count);
 int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size);

@@ -93,6 +93,7 @@ struct fdtd_sim {
 
   // ---- device ----
   hipStream_t stream = nullptr;
+  int device = 0;  // the HIP device of ReadySimulation
   float *dFb[2] = {}, *dPsiB[2] = {};  // packed field / psi sets
   float *dF[2][6] = {};
   float *dPsi[2][12] = {};
@@ -131,6 +132,17 @@ struct fdtd_sim {
   };
   ArgSlot argslots[2][8];
   int args_gen = 0;
+  // fdtd_run_noalloc: while set, the fused step takes its TF/SF face sheets and its row-kernel
+  // argument blocks from the GPU's device scratch (scratch_region) instead of the simulation's own
+  // buffers, and builds the argument blocks in na_host; nothing is allocated.
+  struct NoAlloc {
+    float *xs[2];        // face-sheet buffers of state sets 0 and 1
+    char *args[2];       // argument blocks of state sets 0 and 1 (device)
+    bool uploaded[2];    // argument block of the set built and copied in this call
+    int nwg[2];
+  };
+  NoAlloc *na = nullptr;
+  alignas(16) char na_host[2][kScratchArgBytes];
   int timing = 0;  // HIP events around every timing-th step of fdtd_run (0: none)
   std::vector<hipEvent_t> ev;
   double kernel_ms = 0, rows_ms = 0;  // summed step / row-kernel event times of the last timed run
@@ -443,7 +455,10 @@ struct fdtd_sim {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FDTD_ERR_DEVICE, "no HIP device");
     HIPCHK(hipStreamCreate(&stream));
+    HIPCHK(hipGetDevice(&device));
     if (int rc = alloc_set(0)) return rc;
+    // both state sets of the fused step's ping-pong at set-up, so that no step allocates
+    if (int rc = ensure_second()) return rc;
     if (int rc = upload_coefficients()) return rc;
     // parameter table: bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
     std::vector<float> prm;
@@ -655,9 +670,40 @@ struct fdtd_sim {
 
   // The fused step's description and row-kernel argument block for current set `which` and
   // plane wave pw (null: none), built and uploaded when anything it depends on changed.
+  // Floats of one face-sheet buffer of the TF/SF box (0 when the box is empty).
+  size_t sheet_floats() const {
+    if (g.i1 < g.i0 || g.j1 < g.j0 || g.k1 < g.k0) return 0;
+    const size_t ni = g.i1 - g.i0 + 1, nj = g.j1 - g.j0 + 1, nk = g.k1 - g.k0 + 1;
+    return 8 * (nj * nk + ni * nk + ni * nj);
+  }
+
+  // The row-kernel argument block of set `which` in an allocation-free run: built into na_host and
+  // copied to the device scratch once per call (the scratch is shared by every simulation of the
+  // GPU, so a previous call may have left another simulation's block there).
+  int noalloc_args(int which, const FusedDev &f, const void **dargs, const void **hargs, int *nwg) {
+    if (!na->uploaded[which]) {
+      std::memset(na_host[which], 0, sizeof(na_host[which]));
+      na->nwg[which] = make_step_args(f, tune, na_host[which]);
+      if (na->nwg[which] < 0 && (tune.cpl || tune.tile_rows))
+        return fail(FDTD_ERR_INVALID, "launch tuning not supported for this grid");
+      if (na->nwg[which] >= 0)
+        HIPCHK(hipMemcpyAsync(na->args[which], na_host[which], step_args_size(), hipMemcpyHostToDevice, stream));
+      na->uploaded[which] = true;
+    }
+    *dargs = na->args[which];
+    *hargs = na_host[which];
+    *nwg = na->nwg[which];
+    return 0;
+  }
+
   int fused_slot(int which, int mode, const PWDev *pw, FusedDev &f, ArgSlot **out) {
     if (int rc = ensure_second()) return rc;
     const int slot_mode = pw ? mode : 0;
+    if (na) {  // allocation-free run: the face sheets are the device scratch's
+      fill_fused(which, mode, pw, pw ? na->xs[which] : nullptr, f);
+      *out = nullptr;
+      return 0;
+    }
     if (pw && !dXs[0] && g.i1 >= g.i0 && g.j1 >= g.j0 && g.k1 >= g.k0) {
       const size_t ni = g.i1 - g.i0 + 1, nj = g.j1 - g.j0 + 1, nk = g.k1 - g.k0 + 1;
       const size_t nxs = 8 * (nj * nk + ni * nk + ni * nj);
@@ -665,23 +711,7 @@ struct fdtd_sim {
       args_gen++;
     }
     ArgSlot &slot = argslots[which][slot_mode];
-    {
-      f = FusedDev();
-      f.g = g;
-      f.Fc = dFb[which]; f.Fn = dFb[1 - which]; f.Pc = dPsiB[which]; f.Pn = dPsiB[1 - which];
-      f.cs = cs;
-      f.mat = index_mode ? dMat : nullptr;
-      f.coef = index_mode ? dTab : dCoefFull;
-      f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
-      f.prm = dPrm;
-      if (pw) {
-        f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1; f.xs = dXs[which];
-        f.act_h = pw->active_h; f.act_e = pw->active_e;
-        for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
-      }
-      f.Chh = Chh; f.Che = Che; f.dx = delX;
-      f.Chmu = dChmu[0];  // the three arrays are one block
-    }
+    fill_fused(which, mode, pw, dXs[which], f);
     if (slot.gen != args_gen) {
       slot.host.assign(step_args_size(), 0);
       slot.nwg = make_step_args(f, tune, slot.host.data());
@@ -695,6 +725,26 @@ struct fdtd_sim {
     }
     *out = &slot;
     return 0;
+  }
+
+  // The fused step's description for current set `which`, plane wave pw (null: none) and face
+  // sheets xs.
+  void fill_fused(int which, int mode, const PWDev *pw, float *xs, FusedDev &f) const {
+    f = FusedDev();
+    f.g = g;
+    f.Fc = dFb[which]; f.Fn = dFb[1 - which]; f.Pc = dPsiB[which]; f.Pn = dPsiB[1 - which];
+    f.cs = cs;
+    f.mat = index_mode ? dMat : nullptr;
+    f.coef = index_mode ? dTab : dCoefFull;
+    f.ntab = index_mode ? (int)(coef_tab.size() / 6) : 0;
+    f.prm = dPrm;
+    if (pw) {
+      f.pwtab = dPwTab[mode]; f.Eh = pw->Eh; f.Hh = pw->Hh; f.N1 = pw->N1; f.xs = xs;
+      f.act_h = pw->active_h; f.act_e = pw->active_e;
+      for (int c = 0; c < 3; ++c) { f.polE[c] = pw->polE[c]; f.polH[c] = pw->polH[c]; }
+    }
+    f.Chh = Chh; f.Che = Che; f.dx = delX;
+    f.Chmu = dChmu[0];  // the three arrays are one block
   }
 
   // State carried from step to step by a run of consecutive steps (fdtd_run): which face sheets
@@ -730,8 +780,16 @@ struct fdtd_sim {
     FusedDev f;
     ArgSlot *slotp = nullptr;
     if (int rc = fused_slot(cur, mode, pw, f, &slotp)) return rc;
-    ArgSlot &slot = *slotp;
-    if (slot.nwg < 0) {
+    const void *dargs = nullptr, *hargs = nullptr;
+    int nwg = -1;
+    if (na) {
+      if (int rc = noalloc_args(cur, f, &dargs, &hargs, &nwg)) return rc;
+    } else {
+      dargs = slotp->dptr;
+      hargs = slotp->host.data();
+      nwg = slotp->nwg;
+    }
+    if (nwg < 0) {
       if (dry) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
       if (rf) {
         if (int rc = flush_probes(*rf)) return rc;
@@ -759,7 +817,7 @@ struct fdtd_sim {
       *ev_slot += 4;
     }
     if (!rf || dry) {
-      HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr));
+      HIPCHK(launch_step_fused(stream, f, dargs, hargs, nwg, n, e ? e + 1 : nullptr));
       if (timed) HIPCHK(hipEventRecord(e[3], stream));
       if (dry) return 0;
       cur = 1 - cur;
@@ -768,7 +826,7 @@ struct fdtd_sim {
     }
     StepFold fo;
     fo.sheets_ready = rf->sheets_n == n && rf->sheets_set == cur;
-    fo.xs_next = n + 1 < T ? dXs[1 - cur] : nullptr;
+    fo.xs_next = n + 1 < T ? (na ? na->xs[1 - cur] : dXs[1 - cur]) : nullptr;
     if (rf->probe_row >= 0) {
       fo.F = fields(cur);
       fo.idx = dProbeIdx; fo.comps = dProbeComps; fo.port = dPort;
@@ -781,7 +839,7 @@ struct fdtd_sim {
       fo.cur_s = source[n];
       fo.cur_dx = delX;
     }
-    HIPCHK(launch_step_fused(stream, f, slot.dptr, slot.host.data(), slot.nwg, n, e ? e + 1 : nullptr, &fo));
+    HIPCHK(launch_step_fused(stream, f, dargs, hargs, nwg, n, e ? e + 1 : nullptr, &fo));
     if (timed) HIPCHK(hipEventRecord(e[3], stream));
     rf->sheets_n = fo.next_filled ? n + 1 : -1;
     rf->sheets_set = 1 - cur;
@@ -1817,6 +1875,44 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
     s->kernel_launches = ntimed;
   }
   return 0;
+}
+
+int fdtd_run_noalloc(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode) {
+  NEED_READY(s);
+  if (count < 0) return fail(FDTD_ERR_INVALID, "negative step count");
+  if (s->na) return fail(FDTD_ERR_STATE, "an allocation-free run of this simulation is in progress");
+  const PWDev *pw = nullptr;
+  if (count > 0)
+    if (int rc = s->check_pw(n0, mode, &pw)) return rc;
+  if (pw && s->sheet_floats() * sizeof(float) > kScratchSheetBytes)
+    return fail(FDTD_ERR_INVALID, "run_noalloc: the TF/SF face sheets of this grid (" +
+                                      std::to_string(s->sheet_floats() * sizeof(float)) +
+                                      " bytes) exceed the scratch's " + std::to_string(kScratchSheetBytes) +
+                                      " bytes per buffer");
+  static_assert(kScratchArgBytes >= 1024, "argument block scratch too small");
+  if (step_args_size() > kScratchArgBytes) return fail(FDTD_ERR_INVALID, "run_noalloc: argument block too large");
+  int prev = 0;
+  HIPCHK(hipGetDevice(&prev));
+  if (prev != s->device) HIPCHK(hipSetDevice(s->device));
+  void *base = nullptr;
+  const hipError_t e = scratch_region(&base);
+  if (e != hipSuccess) {
+    if (prev != s->device) hipSetDevice(prev);
+    return fail(FDTD_ERR_DEVICE, std::string("scratch_region: ") + hipGetErrorString(e));
+  }
+  char *b = (char *)base;
+  fdtd_sim::NoAlloc na{{(float *)b, (float *)(b + kScratchSheetBytes)},
+                       {b + 2 * kScratchSheetBytes, b + 2 * kScratchSheetBytes + kScratchArgBytes},
+                       {false, false},
+                       {-1, -1}};
+  s->na = &na;
+  fdtd_sim::RunFold rf;
+  int slot = 0, rc = 0;
+  for (int q = 0; q < count && !rc; ++q) rc = s->step(n0 + q, mode, false, &slot, false, &rf);
+  const int rc2 = s->flush_probes(rf);
+  s->na = nullptr;
+  if (prev != s->device) hipSetDevice(prev);
+  return rc ? rc : rc2;
 }
 
 int fdtd_snapshot_count(fdtd_sim *s, int32_t *count) {

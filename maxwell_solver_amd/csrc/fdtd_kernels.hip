@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "fdtd_kernels.hpp"
@@ -1767,6 +1768,33 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
   if (nAnt <= 0 || ncomp <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_probes, dim3(cdiv(nAnt, 64), ncomp), dim3(64), 0, st, F, idx, nAnt, comps, ncomp, port, row, T);
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// The device scratch of fdtd_run_noalloc: one file-scope array per GPU, held until the process
+// ends.  Host code never passes `scratch` itself (in host code the name stands for a host-side
+// shadow, not a device address): scratch_region() looks the array's device address up once per
+// device with hipGetSymbolAddress and every user takes that address.
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+hipError_t scratch_region(void **ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
 }
 
 bool fused_supported(const GridDev &g) {

@@ -116,6 +116,15 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
                              hipEvent_t *ev_rows = nullptr, StepFold *fold = nullptr);
 bool fused_supported(const GridDev &g);
 
+// Device scratch of the allocation-free run (fdtd_run_noalloc): kScratchBytes per GPU, a file-scope
+// device array; scratch_region returns its device address on the current device (looked up once
+// per device).  Layout: two TF/SF face-sheet buffers of kScratchSheetBytes each, then the row
+// kernel's argument blocks of the two state sets (kScratchArgBytes each).
+constexpr size_t kScratchSheetBytes = (size_t)24 << 20;
+constexpr size_t kScratchArgBytes = 4096;
+constexpr size_t kScratchBytes = 2 * kScratchSheetBytes + 2 * kScratchArgBytes;
+hipError_t scratch_region(void **ptr);
+
 // ---- batched fused step: one launch per kernel covers many grids ----
 // Per grid of a batched launch: first workgroup, current set and time index at batch step 0, and
 // the argument block (device copy) to use while the current set is 0 or 1.  A table has ng + 1

@@ -31,7 +31,7 @@ EXPORTS = [
     "fdtd_batch_add", "fdtd_batch_clear", "fdtd_batch_run", "fdtd_batch_set_strategy", "fdtd_set_timing", "fdtd_kernel_time", "fdtd_row_kernel_time",
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
-    "fdtd_server_request_error", "fdtd_device_bytes",
+    "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
             "fdtd_server_request_bytes": [vp, C.c_int64, u64p],
             "fdtd_server_request_error": [vp, C.c_int64, ip, C.c_char_p, i32],
             "fdtd_device_bytes": [vp, u64p],
+            "fdtd_run_noalloc": [vp, i32, i32, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -245,6 +246,10 @@ class Simulation:
     def run(self, n0, count, mode, snap_every=0, snap_comp="Ez"):
         comp = FIELD[snap_comp] + 1 if isinstance(snap_comp, str) else int(snap_comp)
         self._chk(lib().fdtd_run(self._h, int(n0), int(count), int(mode), int(snap_every), comp))
+
+    def run_noalloc(self, n0, count, mode):
+        """fdtd_run_noalloc: the steps of run() with no allocation (device scratch of the library)."""
+        self._chk(lib().fdtd_run_noalloc(self._h, int(n0), int(count), int(mode)))
 
     def synchronize(self):
         self._chk(lib().fdtd_synchronize(self._h))
