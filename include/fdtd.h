@@ -155,6 +155,13 @@ int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size);
  * argument blocks and snapshot buffers).  Repeated UpdateCuda, Init1D and SetAntennaLocations calls
  * reuse or replace their buffers, so the figure does not grow with them. */
 int fdtd_device_bytes(fdtd_sim *s, uint64_t *bytes);
+/* Measurement (no counterpart in the reference): the stream ceiling of this grid -- a kernel that
+ * reads the six field components and the material byte of every cell from one state set and
+ * writes the six components to the other (49 bytes per cell, the fused step's compulsory traffic
+ * with no stencil), in the fused step's layout; *ms = its average time over reps launches.  It
+ * overwrites the fields (call ResetFieldAll before stepping again).  Needs the uint8 material
+ * index form. */
+int fdtd_stream_ceiling(fdtd_sim *s, int32_t reps, double *ms);
 /* Wait for all queued work of this simulation. */
 int fdtd_synchronize(fdtd_sim *s);
 /* Read any internal array to host for verification: what = 0..5 fields, 6..11 Ce/Ch per cell,

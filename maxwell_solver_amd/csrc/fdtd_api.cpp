@@ -1931,6 +1931,28 @@ int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size) {
   return 0;
 }
 
+int fdtd_stream_ceiling(fdtd_sim *s, int32_t reps, double *ms) {
+  NEED_READY(s);
+  if (reps < 1 || !ms) return fail(FDTD_ERR_INVALID, "stream ceiling: reps >= 1 and an output required");
+  if (!s->dMat) return fail(FDTD_ERR_STATE, "stream ceiling: needs the uint8 material index form");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = launch_stream(s->stream, s->dFb[s->cur], s->dFb[1 - s->cur], s->dMat, s->cs, s->IJK);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+  for (int q = 0; q < reps && e == hipSuccess; ++q)
+    e = launch_stream(s->stream, s->dFb[(s->cur + q) & 1], s->dFb[(s->cur + q + 1) & 1], s->dMat, s->cs, s->IJK);
+  if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(FDTD_ERR_DEVICE, std::string("stream ceiling: ") + hipGetErrorString(e));
+  *ms = t / reps;
+  return 0;
+}
+
 int fdtd_device_bytes(fdtd_sim *s, uint64_t *bytes) {
   NEED(s);
   if (!bytes) return fail(FDTD_ERR_INVALID, "null output");

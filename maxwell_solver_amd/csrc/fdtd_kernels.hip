@@ -1771,6 +1771,34 @@ hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int 
 }
 
 // ------------------------------------------------------------------------------------------
+// Stream ceiling: the fused step's compulsory traffic with no stencil -- per cell the six field
+// components and the material byte read from one state set and the six components written to the
+// other (49 bytes), one pass, the row kernel's layout (components cs floats apart) and store
+// policy.  Its rate bounds what any single-pass step can reach on this GPU.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stream(const float *__restrict__ Fc, float *__restrict__ Fn,
+                                               const uint8_t *__restrict__ mat, size_t cs, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t id = (size_t)blockIdx.x * 256 + threadIdx.x; id < n; id += stride) {
+    const float m = (float)mat[id];
+    const float ex = Fc[id], ey = Fc[cs + id], ez = Fc[2 * cs + id];
+    const float hx = Fc[3 * cs + id], hy = Fc[4 * cs + id], hz = Fc[5 * cs + id];
+    __builtin_nontemporal_store(ex + m, Fn + id);
+    __builtin_nontemporal_store(ey + m, Fn + cs + id);
+    __builtin_nontemporal_store(ez + m, Fn + 2 * cs + id);
+    __builtin_nontemporal_store(hx + m, Fn + 3 * cs + id);
+    __builtin_nontemporal_store(hy + m, Fn + 4 * cs + id);
+    __builtin_nontemporal_store(hz + m, Fn + 5 * cs + id);
+  }
+}
+
+hipError_t launch_stream(hipStream_t st, const float *Fc, float *Fn, const uint8_t *mat, size_t cs, size_t n) {
+  const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)256 * 64);
+  hipLaunchKernelGGL(k_stream, dim3(nb), dim3(256), 0, st, Fc, Fn, mat, cs, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
 // The device scratch of fdtd_run_noalloc: one file-scope array per GPU, held until the process
 // ends.  Host code never passes `scratch` itself (in host code the name stands for a host-side
 // shadow, not a device address): scratch_region() looks the array's device address up once per

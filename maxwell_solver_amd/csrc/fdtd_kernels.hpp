@@ -115,6 +115,8 @@ struct StepFold {
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows = nullptr, StepFold *fold = nullptr);
 bool fused_supported(const GridDev &g);
+// The stream ceiling: Fn[c][id] = Fc[c][id] + mat[id] for the six components (cs apart), n cells.
+hipError_t launch_stream(hipStream_t st, const float *Fc, float *Fn, const uint8_t *mat, size_t cs, size_t n);
 
 // Device scratch of the allocation-free run (fdtd_run_noalloc): kScratchBytes per GPU, a file-scope
 // device array; scratch_region returns its device address on the current device (looked up once

@@ -32,6 +32,7 @@ EXPORTS = [
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
+    "fdtd_stream_ceiling",
 ]
 
 
@@ -85,6 +86,7 @@ def lib():
             "fdtd_server_request_error": [vp, C.c_int64, ip, C.c_char_p, i32],
             "fdtd_device_bytes": [vp, u64p],
             "fdtd_run_noalloc": [vp, i32, i32, i32],
+            "fdtd_stream_ceiling": [vp, i32, dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -250,6 +252,12 @@ class Simulation:
     def run_noalloc(self, n0, count, mode):
         """fdtd_run_noalloc: the steps of run() with no allocation (device scratch of the library)."""
         self._chk(lib().fdtd_run_noalloc(self._h, int(n0), int(count), int(mode)))
+
+    def stream_ceiling(self, reps=20):
+        """Average ms of the 49-byte-per-cell stream kernel over this grid (overwrites the fields)."""
+        ms = C.c_double()
+        self._chk(lib().fdtd_stream_ceiling(self._h, int(reps), C.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         self._chk(lib().fdtd_synchronize(self._h))
