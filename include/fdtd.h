@@ -117,10 +117,11 @@ int fdtd_create_mu(fdtd_sim *s, const float *mu_x, const float *mu_y, const floa
  * UpdateE_All, UpdateE_PML, AddCurrents).  pol 0 turns them off; box as for 'AddCurrents'.
  * Probes sample a step before its currents, as in that loop. */
 int fdtd_set_step_currents(fdtd_sim *s, int32_t pol, const int32_t box[6]);
-/* Graph replay: fdtd_run records `steps` consecutive fused steps (with their step currents) into
- * one HIP graph and replays it for runs without a plane wave, probes, snapshots or timing (every
- * launch of such a step is the same, the current sample is read on the device); other steps run
- * one by one.  steps: an even count, 0 = off (the default). */
+/* Graph replay: fdtd_run records `steps` consecutive fused steps into one HIP graph and replays it
+ * (the time index, probe record row and current sample of each recorded launch are read on the
+ * device from a counter the replay advances), for runs with or without a plane wave and probes;
+ * not with snapshots or timing, nor probes together with step currents; other steps run one by
+ * one.  steps: an even count, 0 = off (the default). */
 int fdtd_set_graph_steps(fdtd_sim *s, int32_t steps);
 /* Per-cell uint8 material index into a table of n rows [sigma_x, sigma_y, sigma_z, eps_x, eps_y,
  * eps_z] (n <= 256); replaces the six UpdateMaterial arrays. */
@@ -186,6 +187,9 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
  * without that split.  An unsupported choice makes
  * the next step fail with FDTD_ERR_INVALID. */
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
+/* The launch tuning in force: out = [rows per group, z chunk, columns per lane (-1: one column
+ * without the split edge segment), 0]; 0 means automatic. */
+int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]);
 /* Batched engine (no counterpart in the reference; for many independent grids, each much smaller
  * than the GPU).  fdtd_batch_add queues `count` steps of a ready simulation from time index n0
  * with plane-wave mode `mode`; fdtd_batch_run advances every queued simulation to the end of its
@@ -201,8 +205,11 @@ int fdtd_batch_clear(fdtd_batch *b);
 int fdtd_batch_run(fdtd_batch *b);
 /* How fdtd_batch_run advances the grids: 1 = merged launches (each kernel of the step launched once
  * for all grids, in up to four interleaved sub-batches); 2 = one stream per grid (each grid's
- * steps queued on its own stream); 0 (default) = merged launches for the grids under 2^18 cells
- * and one stream each for the larger ones, both at once.  Results are the same either way. */
+ * steps queued on its own stream); 3 = device work queue (as 1, but each row-kernel launch is a
+ * persistent launch of as many workgroups as fit at once, which pull (grid, z-chunk tile) work
+ * items from an atomic counter in device memory until none is left); 0 (default) = merged launches
+ * for the grids under 2^18 cells and one stream each for the larger ones, both at once.  Results
+ * are the same either way. */
 int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy);
 /* Request server (no counterpart in the reference; the served form of the solver).  Solves arrive
  * one at a time and are advanced together with continuous batching on one GPU.

@@ -161,7 +161,8 @@ struct fdtd_sim {
   int dsrc_gen = -1;
   int *dStepN = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int gx_cur = -1, gx_args = -1, gx_steps = 0, gx_currents = -1;
+  int gx_cur = -1, gx_args = -1, gx_steps = 0, gx_currents = -1, gx_mode = -1;
+  bool gx_probes = false;
 
   ~fdtd_sim() {
     if (stream) hipStreamSynchronize(stream);
@@ -825,6 +826,7 @@ struct fdtd_sim {
       return step_currents(n);
     }
     StepFold fo;
+    fo.T = T;
     fo.sheets_ready = rf->sheets_n == n && rf->sheets_set == cur;
     fo.xs_next = n + 1 < T ? (na ? na->xs[1 - cur] : dXs[1 - cur]) : nullptr;
     if (rf->probe_row >= 0) {
@@ -857,18 +859,19 @@ struct fdtd_sim {
   // a replay ends on the set it starts from), where every launch of a step is the same for all
   // steps: no plane wave, probes, snapshots or timing, a grid the fused step covers.  Returns the
   // steps run (a multiple of graph_steps), 0 when the run does not qualify.
-  int run_graphs(int n0, int count, int mode, int *done) {
+  int run_graphs(int n0, int count, int mode, int *done, RunFold &rf) {
     *done = 0;
     const int G = graph_steps;
-    if (G < 2 || count < G || dPort || timing) return 0;
-    if (step_pol && (n0 < 0 || n0 + count > T)) return 0;  // the step loop reports the bad index
+    if (G < 2 || count < G || timing) return 0;
+    if (dPort && step_pol) return 0;  // probes then sample between a step and its currents: eager
+    if ((step_pol || mode) && (n0 < 0 || n0 + count > T)) return 0;  // the step loop reports the bad index
     const PWDev *pw = nullptr;
     if (int rc = check_pw(n0, mode, &pw)) return rc;
-    if (pw) return 0;
+    const int gmode = pw ? mode : 0;
     FusedDev f[2];
     ArgSlot *slot[2];
-    for (int w = 0; w < 2; ++w) {  // argument blocks uploaded before the recording
-      if (int rc = fused_slot(w, 0, nullptr, f[w], &slot[w])) return rc;
+    for (int w = 0; w < 2; ++w) {  // argument blocks (and face-sheet buffers) before the recording
+      if (int rc = fused_slot(w, mode, pw, f[w], &slot[w])) return rc;
       if (slot[w]->nwg < 0) return 0;
     }
     if (step_pol && dsrc_gen != source_gen) {
@@ -876,8 +879,10 @@ struct fdtd_sim {
       HIPCHK(hipMemcpy(dSrc, source.data(), (size_t)T * sizeof(float), hipMemcpyHostToDevice));
       dsrc_gen = source_gen;
     }
-    if (!dStepN && dalloc(&dStepN, 1)) return FDTD_ERR_NOMEM;
-    if (!gexec || gx_cur != cur || gx_args != args_gen || gx_steps != G || gx_currents != currents_gen) {
+    if (!dStepN && dalloc(&dStepN, 3)) return FDTD_ERR_NOMEM;
+    const bool probes = dPort != nullptr;
+    if (!gexec || gx_cur != cur || gx_args != args_gen || gx_steps != G || gx_currents != currents_gen ||
+        gx_mode != gmode || gx_probes != probes) {
       if (gexec) {
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipGraphExecDestroy(gexec));
@@ -887,8 +892,23 @@ struct fdtd_sim {
       int c = cur;
       hipError_t e = hipSuccess;
       for (int q = 0; q < G && e == hipSuccess; ++q) {
-        StepFold fo;  // the step currents ride in the slab part's launch where they can
-        if (step_pol) {
+        // step q of a replay: its time index and record row are dStepN[0] + q and dStepN[1] + q;
+        // its face sheets were filled by the step before (the previous replay's last step, or the
+        // k_sheets launch before the first replay), it fills those of the next time index, and it
+        // gathers the probes of the state it reads (the previous step's record)
+        StepFold fo;
+        fo.sheets_ready = true;
+        fo.xs_next = pw ? dXs[1 - c] : nullptr;
+        fo.gidx = dStepN;
+        fo.gq = q;
+        fo.T = T;
+        if (probes) {
+          fo.probe_rel = true;
+          fo.F = fields(c);
+          fo.idx = dProbeIdx; fo.comps = dProbeComps; fo.port = dPort;
+          fo.nAnt = nAnt; fo.ncomp = (int)probe_comps.size();
+        }
+        if (step_pol) {  // the step currents ride in the slab part's launch where they can
           fo.cur_comp = step_pol - 1;
           for (int b = 0; b < 6; ++b) fo.cur_box[b] = step_box[b];
           fo.cur_dx = delX; fo.cur_src = dSrc; fo.cur_dn = dStepN; fo.cur_q = q;
@@ -911,14 +931,22 @@ struct fdtd_sim {
         gexec = nullptr;
         return fail(FDTD_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(e3));
       }
-      gx_cur = cur; gx_args = args_gen; gx_steps = G; gx_currents = currents_gen;
+      gx_cur = cur; gx_args = args_gen; gx_steps = G; gx_currents = currents_gen; gx_mode = gmode; gx_probes = probes;
     }
-    const int first = n0;
-    HIPCHK(hipMemcpy(dStepN, &first, sizeof(int), hipMemcpyHostToDevice));
+    // the first step's sheets (unless the run before filled them), then the replays
+    if (pw && !(rf.sheets_n == n0 && rf.sheets_set == cur)) HIPCHK(launch_sheets(stream, f[cur], n0));
+    if (int rc = flush_probes(rf)) return rc;  // a record still pending from before
+    const int idx[3] = {n0, rec_step, rec_step};
+    HIPCHK(hipMemcpy(dStepN, idx, sizeof(idx), hipMemcpyHostToDevice));
     int q = 0;
     for (; q + G <= count; q += G) HIPCHK(hipGraphLaunch(gexec, stream));
     rec_step += q;
     *done = q;
+    // what the last replayed step left for the steps after it: the next step's sheets (filled in
+    // the current set's buffer) and its own record, still to be gathered
+    rf.sheets_n = pw && n0 + q < T ? n0 + q : -1;
+    rf.sheets_set = cur;
+    rf.probe_row = probes && rec_step - 1 < T ? rec_step - 1 : -1;
     return 0;
   }
 };
@@ -941,7 +969,8 @@ struct fdtd_batch {
     int n0, count, mode;
   };
   std::vector<Item> items;
-  int strategy = 0;  // 0 automatic, 1 batched launches, 2 one stream per grid
+  int strategy = 0;  // 0 automatic, 1 batched launches, 2 one stream per grid, 3 device work queue
+  int *dqueue = nullptr;  // per sub-batch stream: two slots of 8 (one per XCD) ticket counters
   // sub-batches run on their own streams (the GPU has four hardware queues per process), so the
   // kernels of one sub-batch overlap those of another
   static constexpr int kStreams = 4;
@@ -956,6 +985,7 @@ struct fdtd_batch {
         hipStreamDestroy(st);
       }
     if (dtab) hipFree(dtab);
+    if (dqueue) hipFree(dqueue);
     for (void *p : dslab) hipFree(p);
   }
 };
@@ -1167,6 +1197,12 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
     b->dtab_bytes = tab_bytes + job_bytes;
   }
   hipStream_t s0 = b->streams[0];
+  const bool queue = b->strategy == 3;
+  if (queue) {
+    if (!b->dqueue && hipMalloc(&b->dqueue, 16 * fdtd_batch::kStreams * sizeof(int)) != hipSuccess)
+      return fail(FDTD_ERR_NOMEM, "batch: work queues");
+    HIPCHK(hipMemsetAsync(b->dqueue, 0, 16 * fdtd_batch::kStreams * sizeof(int), s0));
+  }
   if (tab_bytes) HIPCHK(hipMemcpyAsync(b->dtab, tabs.data(), tab_bytes, hipMemcpyHostToDevice, s0));
   if (job_bytes) HIPCHK(hipMemcpyAsync((char *)b->dtab + tab_bytes, jobs.data(), job_bytes, hipMemcpyHostToDevice, s0));
   HIPCHK(hipStreamSynchronize(s0));
@@ -1174,6 +1210,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
   const ProbeJob *dj = (const ProbeJob *)((char *)b->dtab + tab_bytes);
   // interleave the sub-batches step by step so every stream stays fed
   std::vector<size_t> epi(nsub, 0);
+  std::vector<int> qslot(nsub, 0);  // queue launches so far per stream (their slots alternate)
   int tmax = 0;
   for (const auto &pr : preps) tmax = std::max(tmax, pr.it.count);
   for (int t = 0; t < tmax; ++t) {
@@ -1186,8 +1223,17 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       hipStream_t st = b->streams[q];
       for (const auto &L : ep.launches)
         if (L.kind == 0) HIPCHK(launch_sheets_batch(st, dt + L.tab, L.ng, L.nwg, step));
-      for (const auto &L : ep.launches)
-        if (L.kind >= 1) HIPCHK(launch_rows_batch(st, L.key, L.kind - 1, dt + L.tab, L.ng, L.nwg, step));
+      for (const auto &L : ep.launches) {
+        if (L.kind < 1) continue;
+        if (queue) {
+          if (L.nwg <= 0) continue;
+          HIPCHK(launch_rows_queue(st, L.key, L.kind - 1, dt + L.tab, L.ng, L.nwg, step, b->dqueue + 16 * q, qslot[q] & 1,
+                                   nsub));
+          qslot[q]++;
+        } else {
+          HIPCHK(launch_rows_batch(st, L.key, L.kind - 1, dt + L.tab, L.ng, L.nwg, step));
+        }
+      }
       HIPCHK(launch_probes_batch(st, dj + ep.probes, ep.nprobe, ep.max_ant, step));
     }
   }
@@ -1640,7 +1686,7 @@ int fdtd_batch_clear(fdtd_batch *b) {
 
 int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy) {
   if (!b) return fail(FDTD_ERR_INVALID, "null batch handle");
-  if (strategy < 0 || strategy > 2) return fail(FDTD_ERR_INVALID, "strategy must be 0, 1 or 2");
+  if (strategy < 0 || strategy > 3) return fail(FDTD_ERR_INVALID, "strategy must be 0, 1, 2 or 3");
   b->strategy = strategy;
   return 0;
 }
@@ -1845,9 +1891,9 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
     }
   }
   int slot = 0, q0 = 0;
-  if (snap_every <= 0)
-    if (int rc = s->run_graphs(n0, count, mode, &q0)) return rc;
   fdtd_sim::RunFold rf;
+  if (snap_every <= 0)
+    if (int rc = s->run_graphs(n0, count, mode, &q0, rf)) return rc;
   for (int q = q0; q < count; ++q) {
     if (int rc = s->step(n0 + q, mode, every > 0 && q % every == 0, &slot, false, &rf)) {
       s->flush_probes(rf);
@@ -2095,6 +2141,16 @@ int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_
   s->tune.cpl = cols_per_lane < 0 ? 1 : cols_per_lane;
   s->tune.no_split = cols_per_lane < 0;
   s->args_gen++;
+  return 0;
+}
+
+int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]) {
+  NEED(s);
+  if (!out) return fail(FDTD_ERR_INVALID, "null output");
+  out[0] = s->tune.tile_rows;
+  out[1] = s->tune.z_chunk;
+  out[2] = s->tune.no_split ? -1 : s->tune.cpl;
+  out[3] = 0;
   return 0;
 }
 

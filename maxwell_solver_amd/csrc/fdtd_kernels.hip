@@ -17,7 +17,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "fdtd_kernels.hpp"
@@ -365,9 +367,14 @@ __global__ void __launch_bounds__(256) k_add_currents(GridDev g, FieldSet F, Coe
   E[id] = E[id] - ch[comp] * s * dx / dx / dx;
 }
 
-// Advances the step index read by graph-recorded launches by the steps of one replay.
+// Advances the indices read by graph-recorded launches by the steps of one replay: dn[0] the time
+// index, dn[1] the probe record row of the replay's first step (dn[2], the first record row of the
+// run, stays).
 __global__ void k_advance(int *dn, int steps) {
-  if (threadIdx.x == 0) dn[0] = dn[0] + steps;
+  if (threadIdx.x == 0) {
+    dn[0] = dn[0] + steps;
+    dn[1] = dn[1] + steps;
+  }
 }
 
 // 1D incident line for all T steps in one workgroup (src/simulation.h:198-244 with the kernels
@@ -1518,6 +1525,9 @@ struct TailArgs {
   const int *pidx, *pcomps;
   float *port;
   int nAnt, ncomp, prow, T;
+  // graph-recorded launches: nsh and prow are offsets from idx[0] (time index) and idx[1] (record
+  // row); sheets of time indices >= T and record rows < idx[2] (before the run) are skipped
+  const int *idx;
   // step currents (nb_cur workgroups) on the new state: the box lies in the interior part's cells,
   // which that launch has written; sample s, or src[*dn + q] in a recorded graph
   FieldSet Fn;
@@ -1533,19 +1543,23 @@ template <bool IDX>
 __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
   const unsigned nthr = blockDim.x;
   if (blk < ta.nb_sheets) {
+    const int nsh = ta.idx ? ta.idx[0] + ta.nsh : ta.nsh;
+    if (nsh >= ta.T) return;  // past the line history: a step after the last has no sheets
     const uint32_t ns = 8u * ((uint32_t)ta.xnj * ta.xnk + (uint32_t)ta.xni * ta.xnk + (uint32_t)ta.xni * ta.xnj);
-    const float *rowE = ta.pw.active_h ? ta.pw.Eh + (size_t)ta.nsh * ta.pw.N1 : nullptr;
-    const float *rowH = ta.pw.active_e ? ta.pw.Hh + (size_t)ta.nsh * ta.pw.N1 : nullptr;
+    const float *rowE = ta.pw.active_h ? ta.pw.Eh + (size_t)nsh * ta.pw.N1 : nullptr;
+    const float *rowH = ta.pw.active_e ? ta.pw.Hh + (size_t)nsh * ta.pw.N1 : nullptr;
     for (uint32_t t = blk * nthr + threadIdx.x; t < ns; t += ta.nb_sheets * nthr)
       tfsf_sheet(ta.g, ta.pw, rowE, rowH, ta.xs, ta.xni, ta.xnj, ta.xnk, t);
     return;
   }
   blk -= ta.nb_sheets;
   if (blk < ta.nb_probes) {
+    const int prow = ta.idx ? ta.idx[1] + ta.prow : ta.prow;
+    if (prow < 0 || prow >= ta.T || (ta.idx && prow < ta.idx[2])) return;
     const uint32_t np = (uint32_t)ta.nAnt * ta.ncomp;
     for (uint32_t t = blk * nthr + threadIdx.x; t < np; t += ta.nb_probes * nthr) {
       const int q = t / ta.nAnt, a = t % ta.nAnt;
-      ta.port[((size_t)q * ta.T + ta.prow) * ta.nAnt + a] = ta.F.f[ta.pcomps[q]][ta.pidx[a]];
+      ta.port[((size_t)q * ta.T + prow) * ta.nAnt + a] = ta.F.f[ta.pcomps[q]][ta.pidx[a]];
     }
     return;
   }
@@ -1605,6 +1619,33 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows_batch(const Batch
   rows_body<IDX, BUF, CPL, PART, false>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
 }
 
+
+// The same tiles pulled from device work queues by resident workgroups.  Workgroups are dealt to
+// the 8 XCDs round-robin, and the XCD of workgroup w (w mod 8) keeps a queue of its own: the
+// work items b = 8 t + x for tickets t = 0, 1, ... of counter queue[8 slot + x], the block
+// indices a merged launch would run on that XCD, so a tile's halo rows stay in its XCD's L2.  A
+// workgroup takes tickets until they pass nitems.  Workgroup 0 clears the counters of slot ^ 1
+// for the launch after this one (launches on one stream alternate the slots; the previous launch
+// has finished with that slot).
+template <bool IDX, bool BUF, int CPL, int PART>
+__global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows_queue(const BatchGrid *__restrict__ tab, int ng, int step,
+                                                                     int *queue, int slot, int nitems) {
+  __shared__ int s_item;
+  if (blockIdx.x == 0 && threadIdx.x < 8) queue[8 * (slot ^ 1) + threadIdx.x] = 0;
+  const int x = blockIdx.x & 7;
+  int *q = queue + 8 * slot + x;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = 8 * atomicAdd(q, 1) + x;
+    __syncthreads();
+    const int b = uni(s_item);
+    if (b >= nitems) return;  // the same ticket in every wave: the whole workgroup leaves
+    const int g = find_grid(tab, ng, b);
+    const int par = (tab[g].cur0 + step) & 1;
+    const int off = tab[g].off, nb = tab[g + 1].off - off;
+    rows_body<IDX, BUF, CPL, PART, false>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
+    __syncthreads();  // every wave is done with the tile's LDS (and has read s_item) before the next ticket
+  }
+}
 
 // Argument block of the TF/SF face-sheet fill (k_sheets), one per (state parity, plane-wave mode).
 struct SlabArgs {
@@ -2080,6 +2121,62 @@ static void launch_rows_batch_p(hipStream_t st, const RowsKey &key, dim3 grid, d
   }
 }
 
+// Resident workgroups of a queue launch: the occupancy of the instantiation at this workgroup size
+// times the CU count (looked up once per instantiation and size).
+template <typename F>
+static int queue_capacity(F kernel, int threads, unsigned lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void *, int, unsigned>, int> cache;
+  const auto key = std::make_tuple((const void *)kernel, threads, lds);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess || per < 1)
+    return 0;
+  return cache[key] = cus * per;
+}
+
+template <bool IDX, bool BUF, int CPL, int PART>
+static hipError_t launch_rows_queue_k(hipStream_t st, int threads, unsigned lds, const BatchGrid *tab, int ng, int nwg,
+                                      int step, int *queue, int slot, int share) {
+  auto k = k_rows_queue<IDX, BUF, CPL, PART>;
+  const int cap = queue_capacity(k, threads, lds);
+  if (cap <= 0) return hipErrorInvalidConfiguration;
+  // this launch's share of the resident workgroups (launches of other streams run beside it), a
+  // multiple of 8 so that every XCD queue has workgroups
+  const int want = std::max(8, cap / std::max(share, 1) / 8 * 8);
+  const dim3 grid(std::min((nwg + 7) / 8 * 8, want)), blk(threads);
+  hipLaunchKernelGGL(k, grid, blk, lds, st, tab, ng, step, queue, slot, nwg);
+  return hipGetLastError();
+}
+
+template <bool IDX, bool BUF, int PART>
+static hipError_t launch_rows_queue_c(hipStream_t st, int cpl, int threads, unsigned lds, const BatchGrid *tab, int ng,
+                                      int nwg, int step, int *queue, int slot, int share) {
+  if (cpl == 4) return launch_rows_queue_k<IDX, BUF, 4, PART>(st, threads, lds, tab, ng, nwg, step, queue, slot, share);
+  if (cpl == 2) return launch_rows_queue_k<IDX, BUF, 2, PART>(st, threads, lds, tab, ng, nwg, step, queue, slot, share);
+  return launch_rows_queue_k<IDX, BUF, 1, PART>(st, threads, lds, tab, ng, nwg, step, queue, slot, share);
+}
+
+hipError_t launch_rows_queue(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
+                             int step, int *queue, int slot, int share) {
+  if (nwg <= 0) return hipSuccess;
+  const int threads = part == 0 ? key.threads0 : key.threads;
+  const unsigned lds = part == 0 ? key.lds0 : key.lds;
+  if (key.idx) {
+    if (key.buf) return part == 0 ? launch_rows_queue_c<true, true, 0>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share)
+                                  : launch_rows_queue_c<true, true, 1>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share);
+    return part == 0 ? launch_rows_queue_c<true, false, 0>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share)
+                     : launch_rows_queue_c<true, false, 1>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share);
+  }
+  if (key.buf) return part == 0 ? launch_rows_queue_c<false, true, 0>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share)
+                                : launch_rows_queue_c<false, true, 1>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share);
+  return part == 0 ? launch_rows_queue_c<false, false, 0>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share)
+                   : launch_rows_queue_c<false, false, 1>(st, key.cpl, threads, lds, tab, ng, nwg, step, queue, slot, share);
+}
+
 hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
                              int step) {
   if (nwg <= 0) return hipSuccess;
@@ -2103,6 +2200,15 @@ hipError_t launch_sheets_batch(hipStream_t st, const BatchGrid *tab, int ng, int
 hipError_t launch_probes_batch(hipStream_t st, const ProbeJob *jobs, int njobs, int max_ant, int step) {
   if (njobs <= 0 || max_ant <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_probes_batch, dim3(cdiv(max_ant, 64), njobs), dim3(64), 0, st, jobs, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_sheets(hipStream_t st, const FusedDev &f, int n) {
+  const SlabArgs sa = slab_args(f, n);
+  const bool sheets = f.xs && (f.act_h | f.act_e) && sa.xni > 0 && sa.xnj > 0 && sa.xnk > 0;
+  if (!sheets) return hipSuccess;
+  const long nsheet = 8L * ((long)sa.xnj * sa.xnk + (long)sa.xni * sa.xnk + (long)sa.xni * sa.xnj);
+  hipLaunchKernelGGL(k_sheets, dim3(cdiv(nsheet, 256)), dim3(256), 0, st, sa);
   return hipGetLastError();
 }
 
@@ -2142,15 +2248,19 @@ hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *darg
         return launch_probes(st, fold->F, fold->idx, fold->nAnt, fold->comps, fold->ncomp, fold->port, fold->row, fold->T);
       return hipGetLastError();
     }
+    ta.idx = fold->gidx;
+    ta.T = fold->T;
     if (sheets && fold->xs_next) {
       ta.g = sa.g; ta.pw = sa.pw;
-      ta.xs = fold->xs_next; ta.xni = sa.xni; ta.xnj = sa.xnj; ta.xnk = sa.xnk; ta.nsh = n + 1;
+      ta.xs = fold->xs_next; ta.xni = sa.xni; ta.xnj = sa.xnj; ta.xnk = sa.xnk;
+      ta.nsh = fold->gidx ? fold->gq + 1 : n + 1;
       ta.nb_sheets = std::min(cdiv(nsheet, nthr), 1024u);
       fold->next_filled = true;
     }
-    if (fold->row >= 0 && fold->row < fold->T && fold->nAnt > 0 && fold->ncomp > 0) {
+    const bool rec = fold->gidx ? fold->probe_rel : (fold->row >= 0 && fold->row < fold->T);
+    if (rec && fold->nAnt > 0 && fold->ncomp > 0) {
       ta.F = fold->F; ta.pidx = fold->idx; ta.pcomps = fold->comps; ta.port = fold->port;
-      ta.nAnt = fold->nAnt; ta.ncomp = fold->ncomp; ta.prow = fold->row; ta.T = fold->T;
+      ta.nAnt = fold->nAnt; ta.ncomp = fold->ncomp; ta.prow = fold->gidx ? fold->gq - 1 : fold->row;
       ta.nb_probes = std::min(cdiv((long)fold->nAnt * fold->ncomp, nthr), 64u);
     }
     if (fold->cur_comp >= 0) {

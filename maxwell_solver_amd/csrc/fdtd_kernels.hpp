@@ -54,7 +54,8 @@ hipError_t launch_add_currents(hipStream_t st, const GridDev &g, const FieldSet 
 // Cells an AddCurrents box writes (upper ends clipped to the grid); false when it writes none
 // (an empty box, or a negative start, as the reference's size_t box, src/gpu.h:55-82)
 bool current_box(const GridDev &g, const int box[6], int &a0, int &a1, int &b0, int &b1, int &c0, int &c1);
-// *dn += steps (one thread): the step index of graph-recorded launches, advanced once per replay
+// dn[0] += steps, dn[1] += steps (one thread): the time index and record row of graph-recorded
+// launches, advanced once per replay
 hipError_t launch_advance(hipStream_t st, int *dn, int steps);
 hipError_t launch_line(hipStream_t st, const LineDev &L);
 hipError_t launch_probes(hipStream_t st, const FieldSet &F, const int *idx, int nAnt, const int *comps, int ncomp,
@@ -110,11 +111,19 @@ struct StepFold {
   const float *cur_src = nullptr;
   const int *cur_dn = nullptr;
   bool cur_done = false;
+  // Launches recorded into a graph of several steps (gidx set): the step is the gq-th of the
+  // recording; the sheets filled are those of time index gidx[0] + gq + 1 (none past T) and, with
+  // probe_rel, the record row gathered is gidx[1] + gq - 1 (none before gidx[2]).
+  const int *gidx = nullptr;
+  int gq = 0;
+  bool probe_rel = false;
 };
 // ev_rows (optional): two events recorded on st around the row kernel's interior part
 hipError_t launch_step_fused(hipStream_t st, const FusedDev &f, const void *dargs, const void *hargs, int nwg, int n,
                              hipEvent_t *ev_rows = nullptr, StepFold *fold = nullptr);
 bool fused_supported(const GridDev &g);
+// The TF/SF face sheets of time index n into f.xs (nothing without a plane wave).
+hipError_t launch_sheets(hipStream_t st, const FusedDev &f, int n);
 // The stream ceiling: Fn[c][id] = Fc[c][id] + mat[id] for the six components (cs apart), n cells.
 hipError_t launch_stream(hipStream_t st, const float *Fc, float *Fn, const uint8_t *mat, size_t cs, size_t n);
 
@@ -165,6 +174,11 @@ size_t slab_args_size();
 void make_slab_args(const FusedDev &f, void *out, unsigned *nb_sheets);
 hipError_t launch_rows_batch(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
                              int step);
+// The same tiles pulled from device work queues (one per XCD) by resident workgroups, 1/share of
+// those that fit at once: queue[8 slot .. 8 slot + 7] count tickets (zero at launch; the launch
+// clears those of slot ^ 1 for the next one).
+hipError_t launch_rows_queue(hipStream_t st, const RowsKey &key, int part, const BatchGrid *tab, int ng, int nwg,
+                             int step, int *queue, int slot, int share);
 // Workgroups of part 0 (interior rows and chunks) or part 1 (the slab rows and chunks) of the row kernel.
 int rows_part_nwg(const void *hargs, int part);
 hipError_t launch_sheets_batch(hipStream_t st, const BatchGrid *tab, int ng, int nwg, int step);

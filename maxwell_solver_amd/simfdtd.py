@@ -32,7 +32,7 @@ EXPORTS = [
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
-    "fdtd_stream_ceiling",
+    "fdtd_stream_ceiling", "fdtd_get_tuning",
 ]
 
 
@@ -87,6 +87,7 @@ def lib():
             "fdtd_device_bytes": [vp, u64p],
             "fdtd_run_noalloc": [vp, i32, i32, i32],
             "fdtd_stream_ceiling": [vp, i32, dp],
+            "fdtd_get_tuning": [vp, ip],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -267,6 +268,12 @@ class Simulation:
         self._chk(lib().fdtd_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    def get_tuning(self):
+        """(rows per group, z chunk, columns per lane) in force; 0 = automatic."""
+        out = np.zeros(4, np.int32)
+        self._chk(lib().fdtd_get_tuning(self._h, _p(out, C.c_int32)))
+        return tuple(int(v) for v in out[:3])
+
     def set_tuning(self, rows_per_group=0, z_chunk=0, cols_per_lane=0):
         self._chk(lib().fdtd_set_tuning(self._h, int(rows_per_group), int(z_chunk), int(cols_per_lane)))
 
@@ -364,7 +371,7 @@ class Batch:
         Simulation._chk(lib().fdtd_batch_run(self._h))
 
     def set_strategy(self, strategy):
-        """0 automatic, 1 merged launches, 2 one stream per grid."""
+        """0 automatic, 1 merged launches, 2 one stream per grid, 3 device work queue."""
         Simulation._chk(lib().fdtd_batch_set_strategy(self._h, int(strategy)))
 
     def close(self):

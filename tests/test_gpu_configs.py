@@ -124,6 +124,29 @@ def test_workload4_batch_equals_each_grid_alone(batch96):
         a.close()
 
 
+def test_workload4_device_queue_equals_default(batch96):
+    """The whole 96-grid batch again through the device work queue (persistent launches pulling
+    (grid, tile) items from an atomic counter): every grid equals the default engine's result."""
+    from maxwell_solver_amd import Batch, Simulation
+    from maxwell_solver_amd import scenarios as S
+    work, sims = batch96
+    b = Batch()
+    b.set_strategy(3)
+    mine = []
+    for sc, steps in work:
+        g = Simulation()
+        S.setup(g, sc)
+        mine.append(g)
+        b.add(g, 0, steps, sc.mode)
+    b.run()
+    b.close()
+    for (sc, steps), g, ref in zip(work, mine, sims):
+        np.testing.assert_array_equal(g.probes(), ref.probes(), err_msg=sc.name)
+        np.testing.assert_array_equal(g.field("Ez"), ref.field("Ez"), err_msg=sc.name)
+        np.testing.assert_array_equal(g.psi(2), ref.psi(2), err_msg=sc.name)
+        g.close()
+
+
 def test_workload4_sample_is_stratified():
     from maxwell_solver_amd import scenarios as S
     work = S.config4()

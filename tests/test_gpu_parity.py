@@ -342,7 +342,7 @@ def test_autotune_leaves_state_and_keeps_parity():
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
-@pytest.mark.parametrize("strategy", [0, 1, 2])
+@pytest.mark.parametrize("strategy", [0, 1, 2, 3])
 def test_batch_of_grids_matches_each_alone(strategy):
     """Grids of different sizes, row layouts, materials, plane-wave modes and step counts advanced
     together by the batched engine equal the oracle run of each grid alone, bit for bit (fields,
@@ -677,3 +677,31 @@ def test_tfsf_box_inside_the_cpml(inset, mode):
         o.run(0, sc.T, mode)
         assert_close(compare_state(g, o), exact=True)
         np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+@pytest.mark.parametrize("mode,graph", [(3, 8), (5, 8), (5, 2), (1, 16)])
+def test_plane_wave_runs_replay_graphs(mode, graph):
+    """Runs with a plane wave and probes replay a recorded graph of several fused steps (the time
+    index of the face sheets and the probe record row read on the device): runs of several replays
+    plus eager steps, single steps and runs between them, a recording per state set, and a run that
+    ends on the last time index; fields, psi and every probe record equal the oracle's."""
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=40, K=44, T=60, mode=mode, cpml=CFG_CPML, oblique=(0.8, 2.0, 0.5) if mode == 5 else None)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_graph_steps(graph)
+    n = 0
+    for kind, count in (("run", 21), ("step", 1), ("run", 17), ("run", 3), ("step", 1)):
+        if kind == "run":
+            g.run(n, count, mode)
+        else:
+            g.step(n, mode)
+        n += count
+    o.run(0, n, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+    g.run(n, sc.T - n, mode)  # to the last time index (no sheets past it)
+    o.run(n, sc.T - n, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
