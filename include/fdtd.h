@@ -250,9 +250,18 @@ int fdtd_server_request_bytes(fdtd_server *srv, int64_t id, uint64_t *bytes);
 /* Bytes charged to admitted requests now and at most so far; cell updates done. */
 int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_peak, int64_t *cell_updates);
 /* Time the fused step at time index n, plane-wave mode `mode`, for each available columns-per-lane
- * and z-chunk choice (reps steps each) and keep the fastest (*best_ms: its average step time).
- * The steps are run dry: the fields, psi, probe record and time step are unchanged. */
+ * and z-chunk choice and two tile heights (reps steps each, in the launch form of a run) and keep
+ * the fastest, unless the automatic choice is within 1% of it (*best_ms: its average step time).
+ * The steps are run dry: the fields, psi, probe record and time step are unchanged.  The choice
+ * is also entered in the size-class table (below). */
 int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms);
+/* The size-class tuning table: the autotuner's choices per class of grid (I; floor(log2 J);
+ * floor(log2 K); P; index-form materials; plane wave; per-cell mu), shared by the simulations of
+ * the process and read from / written to the JSON file FDTD_TUNE_CACHE names (if any).  fdtd_run
+ * and the batched engine consult it for a simulation whose tuning is automatic: policy 1 (the
+ * default) applies the class's entry when there is one; policy 2 also autotunes a class that has
+ * none (at the run's first time index) and adds it; 0 keeps the automatic choice. */
+int fdtd_set_autotune(fdtd_sim *s, int32_t policy);
 /* Average device time of the fused step over the last fdtd_run with timing on: HIP events on the
  * simulation's stream around every `every`-th step of the run (steps 0, every, 2 every, ...;
  * every = 1: all steps, 0: off).  Each pair of events costs the stream a few microseconds, so a

@@ -8,8 +8,9 @@
 
 #include <algorithm>
 #include <chrono>
-#include <deque>
 #include <cmath>
+#include <cstdlib>
+#include <deque>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <set>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fdtd.h"
@@ -123,6 +125,8 @@ struct fdtd_sim {
   int nsnap = 0;
   PWDev pwdev[8];
   StepTuning tune;
+  bool tune_user = false;  // fdtd_set_tuning chose the tuning: the size-class table leaves it
+  int tune_policy = 1;     // fdtd_set_autotune: 0 heuristics, 1 size-class table, 2 table + autotune on a miss
   // device argument blocks of the fused kernel per (state parity, plane-wave mode); rebuilt when
   // args_gen moves (any buffer or tuning change)
   struct ArgSlot {
@@ -855,6 +859,34 @@ struct fdtd_sim {
     return 0;
   }
 
+  // Average device time of `reps` steps at time index n in the launch form of a run (the row
+  // kernel's two launches, the slab part filling the next step's face sheets), with the tuning in
+  // force, run dry: the kernels read the current set and write the other, the current set and the
+  // probe record stay as they are.  One untimed step first (it also fills the sheets of step n).
+  int time_variant(int n, int mode, int reps, hipEvent_t e0, hipEvent_t e1, double *ms) {
+    const PWDev *pw = nullptr;
+    if (int rc = check_pw(n, mode, &pw)) return rc;
+    FusedDev f;
+    ArgSlot *slot = nullptr;
+    if (int rc = fused_slot(cur, mode, pw, f, &slot)) return rc;
+    if (slot->nwg < 0) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
+    HIPCHK(launch_step_fused(stream, f, slot->dptr, slot->host.data(), slot->nwg, n));
+    HIPCHK(hipEventRecord(e0, stream));
+    for (int q = 0; q < reps; ++q) {
+      StepFold fo;
+      fo.T = T;
+      fo.sheets_ready = true;
+      fo.xs_next = pw ? dXs[1 - cur] : nullptr;
+      HIPCHK(launch_step_fused(stream, f, slot->dptr, slot->host.data(), slot->nwg, n, nullptr, &fo));
+    }
+    HIPCHK(hipEventRecord(e1, stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = (double)t / reps;
+    return 0;
+  }
+
   // Steps [n0, n0 + count) as replays of one recorded graph of graph_steps steps (an even count, so
   // a replay ends on the set it starts from), where every launch of a step is the same for all
   // steps: no plane wave, probes, snapshots or timing, a grid the fused step covers.  Returns the
@@ -1012,11 +1044,14 @@ struct BatchPlan {
 
 }  // namespace
 
+static int apply_tuning_class(fdtd_sim *s, int n, int mode);
+
 // One stream per grid: every grid's steps are queued at once on its own stream and the GPU
 // overlaps the grids as its hardware queues allow (the caller waits for the streams).
 static int batch_queue_streams(const std::vector<fdtd_batch::Item> &items) {
   for (const auto &it : items) {
     if (!it.s->init_ready) return fail(FDTD_ERR_STATE, "batch: ReadySimulation has not been called");
+    if (int rc = apply_tuning_class(it.s, it.n0, it.mode)) return rc;
     int slot = 0;
     for (int q = 0; q < it.count; ++q)
       if (int rc = it.s->step(it.n0 + q, it.mode, false, &slot)) return rc;
@@ -1062,6 +1097,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
     const PWDev *pw = nullptr;
     if (int rc = s->check_pw(it.n0, it.mode, &pw)) return rc;
     if (pw && it.n0 + it.count > s->T) return fail(FDTD_ERR_INVALID, "batch: steps beyond the source length T");
+    if (int rc = apply_tuning_class(s, it.n0, it.mode)) return rc;
     Prep pr;
     pr.it = it;
     bool ok = true;
@@ -1820,48 +1856,191 @@ int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_
   return 0;
 }
 
-int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms) {
-  NEED_READY(s);
-  if (reps < 1) return fail(FDTD_ERR_INVALID, "reps must be >= 1");
+// ------------------------------------------------------------------------------------------
+// Size-class tuning table: the autotuner's choices per class of grid, shared by the process's
+// simulations and kept in a JSON file when FDTD_TUNE_CACHE names one.  A class is (I exactly: it
+// fixes the row layouts; floor(log2 J) and floor(log2 K): the parallelism the z-chunk depth
+// trades against; P; index-form materials; plane wave; per-cell mu).
+// ------------------------------------------------------------------------------------------
+namespace {
+struct TuneClass {
+  int I = 0, Jl = 0, Kl = 0, P = 0, idx = 0, pw = 0, mu = 0;
+  bool operator<(const TuneClass &o) const {
+    return std::tie(I, Jl, Kl, P, idx, pw, mu) < std::tie(o.I, o.Jl, o.Kl, o.P, o.idx, o.pw, o.mu);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneClass, StepTuning> g_tune;
+std::string g_tune_loaded = "\x01";  // the FDTD_TUNE_CACHE path last read (none yet)
+
+int ilog2(int v) {
+  int r = 0;
+  while (v > 1) { v >>= 1; ++r; }
+  return r;
+}
+
+TuneClass tune_class(const fdtd_sim *s, int mode) {
+  TuneClass c;
+  c.I = s->I; c.Jl = ilog2(s->J); c.Kl = ilog2(s->K); c.P = s->cp.P;
+  c.idx = s->index_mode ? 1 : 0;
+  c.pw = mode >= 1 && mode <= 7 ? 1 : 0;
+  c.mu = s->dChmu[0] ? 1 : 0;
+  return c;
+}
+
+std::string tune_path() {
+  const char *p = std::getenv("FDTD_TUNE_CACHE");
+  return p ? std::string(p) : std::string();
+}
+
+// The file: {"classes": [{"I": 256, "log2J": 8, "log2K": 8, "P": 10, "index": 1, "plane_wave": 1,
+// "mu": 0, "rows": 0, "z_chunk": 8, "cols_per_lane": 1}, ...]} (integers only).
+void tune_load_locked() {
+  const std::string path = tune_path();
+  if (path == g_tune_loaded) return;  // read once per path (its entries join the table)
+  g_tune_loaded = path;
+  if (path.empty()) return;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  std::string txt;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) txt.append(buf, n);
+  std::fclose(f);
+  size_t pos = txt.find("\"classes\"");
+  while (pos != std::string::npos) {
+    const size_t a = txt.find('{', pos), b = a == std::string::npos ? a : txt.find('}', a);
+    if (a == std::string::npos || b == std::string::npos) break;
+    std::map<std::string, int> kv;
+    size_t q = a;
+    while (true) {
+      const size_t k0 = txt.find('"', q + 1);
+      if (k0 == std::string::npos || k0 > b) break;
+      const size_t k1 = txt.find('"', k0 + 1);
+      const size_t colon = txt.find(':', k1);
+      if (k1 == std::string::npos || colon == std::string::npos || colon > b) break;
+      kv[txt.substr(k0 + 1, k1 - k0 - 1)] = (int)std::strtol(txt.c_str() + colon + 1, nullptr, 10);
+      q = txt.find_first_of(",}", colon);
+      if (q == std::string::npos || q >= b) break;
+    }
+    TuneClass c;
+    c.I = kv["I"]; c.Jl = kv["log2J"]; c.Kl = kv["log2K"]; c.P = kv["P"];
+    c.idx = kv["index"]; c.pw = kv["plane_wave"]; c.mu = kv["mu"];
+    StepTuning t;
+    t.tile_rows = kv["rows"]; t.z_chunk = kv["z_chunk"];
+    const int cpl = kv["cols_per_lane"];
+    t.cpl = cpl < 0 ? 1 : cpl;
+    t.no_split = cpl < 0;
+    if (c.I > 0) g_tune[c] = t;
+    pos = b;
+  }
+}
+
+void tune_save_locked() {
+  const std::string path = tune_path();
+  if (path.empty()) return;
+  const std::string tmp = path + ".tmp";
+  FILE *f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;
+  std::fprintf(f, "{\"classes\": [");
+  bool first = true;
+  for (const auto &kv : g_tune) {
+    const TuneClass &c = kv.first;
+    const StepTuning &t = kv.second;
+    std::fprintf(f, "%s\n  {\"I\": %d, \"log2J\": %d, \"log2K\": %d, \"P\": %d, \"index\": %d, \"plane_wave\": %d, "
+                 "\"mu\": %d, \"rows\": %d, \"z_chunk\": %d, \"cols_per_lane\": %d}",
+                 first ? "" : ",", c.I, c.Jl, c.Kl, c.P, c.idx, c.pw, c.mu, t.tile_rows, t.z_chunk,
+                 t.no_split ? -1 : t.cpl);
+    first = false;
+  }
+  std::fprintf(f, "\n]}\n");
+  std::fclose(f);
+  std::rename(tmp.c_str(), path.c_str());
+}
+
+// The autotuner: every columns-per-lane and z-chunk choice and two tile heights, each timed over
+// `reps` dry steps in a run's launch form; a choice replaces the automatic one only when it is
+// faster by more than 1% (timing noise would otherwise pick between equals).
+int autotune_sim(fdtd_sim *s, int n, int mode, int reps, double *best_ms) {
   const StepTuning saved = s->tune;
-  StepTuning best = saved;
-  double best_t = -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  double base = -1, best_t = -1;
+  StepTuning best;
   int rc = 0;
-  for (int cpl : {1, 2, 4}) {
-    for (int kc : {4, 8, 16}) {
-      s->tune = StepTuning();
-      s->tune.cpl = cpl;
-      s->tune.z_chunk = kc;
-      s->args_gen++;
-      int slot = 0;
-      if (s->step(n, mode, false, &slot, true)) continue;  // layout not available for this grid
-      if ((rc = hipEventRecord(e0, s->stream) == hipSuccess ? 0 : FDTD_ERR_DEVICE)) break;
-      for (int q = 0; q < reps && !rc; ++q) rc = s->step(n, mode, false, &slot, true);
-      if (rc || hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-        rc = rc ? rc : FDTD_ERR_DEVICE;
-        break;
+  for (int rows : {0, 4})
+    for (int cpl : {0, 1, 2, 4})
+      for (int kc : {0, 4, 8, 16}) {
+        s->tune = StepTuning();
+        s->tune.tile_rows = rows;
+        s->tune.cpl = cpl;
+        s->tune.z_chunk = kc;
+        s->args_gen++;
+        double ms = 0;
+        const int r = s->time_variant(n, mode, reps, e0, e1, &ms);
+        if (r == FDTD_ERR_DEVICE) { rc = r; break; }
+        if (r) continue;  // a layout this grid does not take
+        if (rows == 0 && cpl == 0 && kc == 0) base = ms;
+        if (best_t < 0 || ms < best_t) { best_t = ms; best = s->tune; }
       }
-      float ms = 0;
-      hipEventElapsedTime(&ms, e0, e1);
-      const double t = ms / reps;
-      if (best_t < 0 || t < best_t) {
-        best_t = t;
-        best = s->tune;
-      }
-    }
-    if (rc) break;
-  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  s->tune = best_t < 0 ? saved : best;
+  if (rc) { s->tune = saved; s->args_gen++; return rc; }
+  if (best_t < 0) { s->tune = saved; s->args_gen++; return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts"); }
+  if (base > 0 && best_t > 0.99 * base) { best = StepTuning(); best_t = base; }
+  s->tune = best;
   s->args_gen++;
-  if (rc) return rc;
-  if (best_t < 0) return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts");
   g_err.clear();
   if (best_ms) *best_ms = best_t;
+  return 0;
+}
+}  // namespace
+
+// The size-class table's tuning for a run in plane-wave mode `mode` (fdtd_run, the batched
+// engine): applied when the simulation's tuning is automatic and not chosen by fdtd_set_tuning;
+// with policy 2 a class not in the table is autotuned first (at time index n) and added.
+static int apply_tuning_class(fdtd_sim *s, int n, int mode) {
+  if (s->tune_policy == 0 || s->tune_user || !s->init_ready) return 0;
+  const TuneClass c = tune_class(s, mode);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  auto it = g_tune.find(c);
+  if (it != g_tune.end()) {
+    const StepTuning &t = it->second;
+    if (t.tile_rows != s->tune.tile_rows || t.z_chunk != s->tune.z_chunk || t.cpl != s->tune.cpl ||
+        t.no_split != s->tune.no_split) {
+      s->tune = t;
+      s->args_gen++;
+    }
+    return 0;
+  }
+  if (s->tune_policy < 2) return 0;
+  const PWDev *pw = nullptr;
+  if (s->check_pw(n, mode, &pw)) return 0;  // the run reports a bad time index itself
+  double ms = 0;
+  if (int rc = autotune_sim(s, n, mode, 10, &ms)) return rc == FDTD_ERR_DEVICE ? rc : 0;
+  g_tune[c] = s->tune;
+  tune_save_locked();
+  return 0;
+}
+
+int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms) {
+  NEED_READY(s);
+  if (reps < 1) return fail(FDTD_ERR_INVALID, "reps must be >= 1");
+  if (int rc = autotune_sim(s, n, mode, reps, best_ms)) return rc;
+  s->tune_user = true;  // the caller asked for this simulation's own choice
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  g_tune[tune_class(s, mode)] = s->tune;
+  tune_save_locked();
+  return 0;
+}
+
+int fdtd_set_autotune(fdtd_sim *s, int32_t policy) {
+  NEED(s);
+  if (policy < 0 || policy > 2) return fail(FDTD_ERR_INVALID, "autotune policy must be 0, 1 or 2");
+  s->tune_policy = policy;
   return 0;
 }
 
@@ -1870,6 +2049,8 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
   if (count < 0) return fail(FDTD_ERR_INVALID, "negative step count");
   if (snap_every > 0 && (snap_comp < 1 || snap_comp > 6)) return fail(FDTD_ERR_INVALID, "snapshot component must be 1..6");
   if (int rc = s->ensure_second()) return rc;
+  if (count > 0)
+    if (int rc = apply_tuning_class(s, n0, mode)) return rc;
   // snapshot buffers are allocated before the loop so the loop itself allocates nothing
   s->nsnap = 0;
   if (snap_every > 0) {
@@ -2138,6 +2319,7 @@ int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_
     return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2, 4 or -1");
   s->tune.tile_rows = rows_per_group;
   s->tune.z_chunk = z_chunk;
+  s->tune_user = rows_per_group != 0 || z_chunk != 0 || cols_per_lane != 0;
   s->tune.cpl = cols_per_lane < 0 ? 1 : cols_per_lane;
   s->tune.no_split = cols_per_lane < 0;
   s->args_gen++;

@@ -32,7 +32,7 @@ EXPORTS = [
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
-    "fdtd_stream_ceiling", "fdtd_get_tuning",
+    "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune",
 ]
 
 
@@ -88,6 +88,7 @@ def lib():
             "fdtd_run_noalloc": [vp, i32, i32, i32],
             "fdtd_stream_ceiling": [vp, i32, dp],
             "fdtd_get_tuning": [vp, ip],
+            "fdtd_set_autotune": [vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -267,6 +268,10 @@ class Simulation:
         b = C.c_uint64()
         self._chk(lib().fdtd_device_bytes(self._h, C.byref(b)))
         return b.value
+
+    def set_autotune(self, policy):
+        """Size-class tuning table: 0 off, 1 apply a class's entry (default), 2 also autotune a new class."""
+        self._chk(lib().fdtd_set_autotune(self._h, int(policy)))
 
     def get_tuning(self):
         """(rows per group, z chunk, columns per lane) in force; 0 = automatic."""
