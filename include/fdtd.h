@@ -257,7 +257,8 @@ int fdtd_server_stats(fdtd_server *srv, uint64_t *bytes_in_use, uint64_t *bytes_
 int fdtd_autotune(fdtd_sim *s, int32_t n, int32_t mode, int32_t reps, double *best_ms);
 /* The size-class tuning table: the autotuner's choices per class of grid (I; floor(log2 J);
  * floor(log2 K); P; index-form materials; plane wave; per-cell mu), shared by the simulations of
- * the process and read from / written to the JSON file FDTD_TUNE_CACHE names (if any).  fdtd_run
+ * the process and read from / written to the JSON file FDTD_TUNE_CACHE names; without it the
+ * table shipped beside the library (tuning_gfx950.json, written by tune_table.py) is read.  fdtd_run
  * and the batched engine consult it for a simulation whose tuning is automatic: policy 1 (the
  * default) applies the class's entry when there is one; policy 2 also autotunes a class that has
  * none (at the run's first time index) and adds it; 0 keeps the automatic choice. */

@@ -4,6 +4,7 @@
 // set-up (CPML grading, coefficients, plane-wave tables) runs in fdtd_prepare without touching the
 // device; ReadySimulation then allocates and uploads.  The fields and CPML psi arrays live in HBM
 // for the life of the simulation; the fused step ping-pongs between two copies of them.
+#include <dlfcn.h>
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -1893,10 +1894,21 @@ std::string tune_path() {
   return p ? std::string(p) : std::string();
 }
 
+// The table shipped beside the library (tuning_gfx950.json in its directory), read when
+// FDTD_TUNE_CACHE names no file of the caller's; never written.
+std::string packaged_tune_path() {
+  Dl_info info;
+  if (!dladdr((const void *)&packaged_tune_path, &info) || !info.dli_fname) return std::string();
+  std::string lib = info.dli_fname;
+  const size_t slash = lib.find_last_of('/');
+  return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/tuning_gfx950.json";
+}
+
 // The file: {"classes": [{"I": 256, "log2J": 8, "log2K": 8, "P": 10, "index": 1, "plane_wave": 1,
 // "mu": 0, "rows": 0, "z_chunk": 8, "cols_per_lane": 1}, ...]} (integers only).
 void tune_load_locked() {
-  const std::string path = tune_path();
+  std::string path = tune_path();
+  if (path.empty()) path = packaged_tune_path();
   if (path == g_tune_loaded) return;  // read once per path (its entries join the table)
   g_tune_loaded = path;
   if (path.empty()) return;
