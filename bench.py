@@ -64,9 +64,10 @@ def copy_calibration(nbytes=1 << 30, reps=10):
 def run_batch(args):
     """Many-grid workload (96 independent grids, 32..268 cells per edge, 300..1500 steps each):
     every grid run alone one after another with fdtd_run; all grids queued at once on their own
-    streams; all grids advanced together by the batched engine.  The three rates, the batched /
-    sequential ratio, and whether every grid's probe record of the batched run equals that of its
-    run alone."""
+    streams; all grids advanced together by the batched engine (merged launches, the device work
+    queue, the engine's automatic choice).  --batch-legs picks the legs; reported are each leg's
+    rate, the batched / sequential ratio, and whether every grid's probe record of each batched run
+    equals that of its run alone (when the sequential leg ran)."""
     import numpy as np
 
     from maxwell_solver_amd import Batch, Simulation
@@ -88,43 +89,52 @@ def run_batch(args):
     b.clear()
     g0.reset()
     g0.synchronize()
-    # one grid after another (each run completes before the next starts)
-    t0 = time.perf_counter()
-    for g, sc, steps in sims:
-        g.run(0, steps, sc.mode)
-        g.synchronize()
-    t_seq = time.perf_counter() - t0
-    rec = [g.probes().copy() for g, _, _ in sims]
-    for g, _, _ in sims:
-        g.reset()
-        g.synchronize()
-    # every grid queued at once on its own stream (the GPU overlaps them as its queues allow)
-    t0 = time.perf_counter()
-    for g, sc, steps in sims:
-        g.run(0, steps, sc.mode)
-    for g, _, _ in sims:
-        g.synchronize()
-    t_str = time.perf_counter() - t0
-    for g, _, _ in sims:
-        g.reset()
-        g.synchronize()
-    # merged launches (each kernel of the step once for all grids, four interleaved sub-batches)
-    for g, sc, steps in sims:
-        b.add(g, 0, steps, sc.mode)
-    b.set_strategy(1)
-    t0 = time.perf_counter()
-    b.run()
-    t_mrg = time.perf_counter() - t0
-    same = all(np.array_equal(g.probes(), r) for (g, _, _), r in zip(sims, rec))
-    for g, _, _ in sims:
-        g.reset()
-        g.synchronize()
-    # the engine's automatic choice (the measured value)
-    b.set_strategy(0)
-    t0 = time.perf_counter()
-    b.run()
-    t_bat = time.perf_counter() - t0
-    same = same and all(np.array_equal(g.probes(), r) for (g, _, _), r in zip(sims, rec))
+    legs = [x for x in args.batch_legs.split(",") if x]
+    bad = set(legs) - {"sequential", "streams", "merged", "queue", "auto"}
+    if bad or not legs:
+        raise SystemExit(f"--batch-legs: unknown leg(s) {sorted(bad)} (sequential, streams, merged, queue, auto)")
+    times, rec, same = {}, None, True
+
+    def rewind():
+        for g, _, _ in sims:
+            g.reset()
+            g.synchronize()
+
+    if "sequential" in legs:
+        # one grid after another (each run completes before the next starts)
+        t0 = time.perf_counter()
+        for g, sc, steps in sims:
+            g.run(0, steps, sc.mode)
+            g.synchronize()
+        times["sequential"] = time.perf_counter() - t0
+        rec = [g.probes().copy() for g, _, _ in sims]
+        rewind()
+    if "streams" in legs:
+        # every grid queued at once on its own stream (the GPU overlaps them as its queues allow)
+        t0 = time.perf_counter()
+        for g, sc, steps in sims:
+            g.run(0, steps, sc.mode)
+        for g, _, _ in sims:
+            g.synchronize()
+        times["streams"] = time.perf_counter() - t0
+        rewind()
+    # merged launches (each kernel of the step once for all grids, four interleaved sub-batches),
+    # the device work queue (persistent launches pulling tiles from per-XCD counters), and the
+    # engine's automatic choice (the measured value)
+    for leg, strategy in (("merged", 1), ("queue", 3), ("auto", 0)):
+        if leg not in legs:
+            continue
+        b.clear()
+        for g, sc, steps in sims:
+            b.add(g, 0, steps, sc.mode)
+        b.set_strategy(strategy)
+        t0 = time.perf_counter()
+        b.run()
+        times[leg] = time.perf_counter() - t0
+        if rec is not None:
+            same = same and all(np.array_equal(g.probes(), r) for (g, _, _), r in zip(sims, rec))
+        rewind()
+    t_bat = times.get("auto", next(iter(times.values())))
     grid_steps = sum(steps for _, steps in work)
     out = {
         "metric": "Gcell-updates/s",
@@ -144,17 +154,16 @@ def run_batch(args):
                         "300..1500 steps, 8-cell CPML, oblique TF/SF, 0..20 spheres), batched engine",
             "grids": len(work),
             "cell_updates": total,
-            "sequential_value": round(total / t_seq / 1e9, 3),
-            "sequential_s": round(t_seq, 3),
-            "streams_value": round(total / t_str / 1e9, 3),
-            "streams_s": round(t_str, 3),
-            "merged_value": round(total / t_mrg / 1e9, 3),
-            "merged_s": round(t_mrg, 3),
+            "legs": legs,
             "batched_s": round(t_bat, 3),
-            "speedup_vs_sequential": round(t_seq / t_bat, 3),
-            "batched_equals_sequential": bool(same),
         },
     }
+    for leg, t in times.items():
+        out["config"][f"{leg}_value"] = round(total / t / 1e9, 3)
+        out["config"][f"{leg}_s"] = round(t, 3)
+    if "sequential" in times:
+        out["config"]["speedup_vs_sequential"] = round(times["sequential"] / t_bat, 3)
+        out["config"]["batched_equals_sequential"] = bool(same)
     b.close()
     for g, _, _ in sims:
         g.close()
@@ -487,6 +496,9 @@ def main():
     ap.add_argument("--grids", type=int, default=96)
     ap.add_argument("--min-edge", type=float, default=32)
     ap.add_argument("--max-edge", type=float, default=192)
+    ap.add_argument("--batch-legs", default="sequential,streams,merged,auto",
+                    help="legs of the batched workload, in order: sequential, streams, merged, queue, auto "
+                         "(value = auto, else the first leg run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU z-slab "

@@ -207,9 +207,9 @@ int fdtd_batch_run(fdtd_batch *b);
  * for all grids, in up to four interleaved sub-batches); 2 = one stream per grid (each grid's
  * steps queued on its own stream); 3 = device work queue (as 1, but each row-kernel launch is a
  * persistent launch of as many workgroups as fit at once, which pull (grid, z-chunk tile) work
- * items from an atomic counter in device memory until none is left); 0 (default) = merged launches
- * for the grids under 2^18 cells and one stream each for the larger ones, both at once.  Results
- * are the same either way. */
+ * items from an atomic counter in device memory until none is left); 0 (default) = automatic, at
+ * present merged launches (the fastest of the forms on a mix of 32..268-cell grids).  Results are
+ * the same either way. */
 int fdtd_batch_set_strategy(fdtd_batch *b, int32_t strategy);
 /* Request server (no counterpart in the reference; the served form of the solver).  Solves arrive
  * one at a time and are advanced together with continuous batching on one GPU.

@@ -992,17 +992,13 @@ struct fdtd_sim {
 // chip together and the launch count per step does not grow with the number of grids.  Grids
 // whose layout the fused step does not cover run on their own afterwards.
 // ------------------------------------------------------------------------------------------
-// Grid size (cells) below which the automatic batch strategy merges a grid's launches with the
-// other small grids'; larger grids fill the GPU well enough on their own stream.
-constexpr size_t kBatchMergeCells = (size_t)1 << 18;
-
 struct fdtd_batch {
   struct Item {
     fdtd_sim *s;
     int n0, count, mode;
   };
   std::vector<Item> items;
-  int strategy = 0;  // 0 automatic, 1 batched launches, 2 one stream per grid, 3 device work queue
+  int strategy = 0;  // 0 automatic (merged), 1 merged launches, 2 one stream per grid, 3 device work queue
   int *dqueue = nullptr;  // per sub-batch stream: two slots of 8 (one per XCD) ticket counters
   // sub-batches run on their own streams (the GPU has four hardware queues per process), so the
   // kernels of one sub-batch overlap those of another
@@ -1063,13 +1059,13 @@ static int batch_queue_streams(const std::vector<fdtd_batch::Item> &items) {
 static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &items);
 
 static int batch_run(fdtd_batch *b) {
-  // grids large enough to fill the GPU on their own run on their own streams; the others go
-  // through merged launches; both at once
+  // strategy 2 runs every grid on its own stream; 0 and 1 merge the launches of every grid the
+  // fused step covers.  The automatic choice is the merged form: on a mix of 32..268-cell grids it
+  // beat both one stream per grid and a mix (grids of 2^18 cells and more on their own streams
+  // beside merged small grids, which was slower than either, the own streams contending with
+  // the sub-batch streams for the four hardware queues)
   std::vector<fdtd_batch::Item> big, small;
-  for (const auto &it : b->items) {
-    const bool own = b->strategy == 2 || (b->strategy == 0 && it.s->IJK >= kBatchMergeCells);
-    (own ? big : small).push_back(it);
-  }
+  for (const auto &it : b->items) (b->strategy == 2 ? big : small).push_back(it);
   if (int rc = batch_queue_streams(big)) return rc;
   const int rc = small.empty() ? 0 : batch_run_merged(b, small);
   for (const auto &it : big) HIPCHK(hipStreamSynchronize(it.s->stream));

@@ -376,7 +376,7 @@ class Batch:
         Simulation._chk(lib().fdtd_batch_run(self._h))
 
     def set_strategy(self, strategy):
-        """0 automatic, 1 merged launches, 2 one stream per grid, 3 device work queue."""
+        """0 automatic (merged launches), 1 merged launches, 2 one stream per grid, 3 device work queue."""
         Simulation._chk(lib().fdtd_batch_set_strategy(self._h, int(strategy)))
 
     def close(self):
