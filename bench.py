@@ -476,6 +476,7 @@ def main():
     ap.add_argument("--rows-per-group", type=int, default=0)
     ap.add_argument("--cols-per-lane", type=int, default=0)
     ap.add_argument("--z-chunk", type=int, default=0)
+    ap.add_argument("--slab-chunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -531,6 +532,8 @@ def main():
     g = Simulation()
     S.setup(g, sc)
     g.set_tuning(args.rows_per_group, args.z_chunk, args.cols_per_lane)
+    if args.slab_chunk:
+        g.set_slab_chunk(args.slab_chunk)
 
     # warmup (untimed), then exactly --steps timed steps
     if args.warmup:

@@ -188,8 +188,13 @@ int fdtd_read_setup(fdtd_sim *s, int32_t what, float *out, uint64_t *count);
  * the next step fail with FDTD_ERR_INVALID. */
 int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_t cols_per_lane);
 /* The launch tuning in force: out = [rows per group, z chunk, columns per lane (-1: one column
- * without the split edge segment), 0]; 0 means automatic. */
+ * without the split edge segment), slab chunk]; 0 means automatic. */
 int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]);
+/* Planes per z chunk of the row kernel's slab part (its y-band rows over the interior planes, and
+ * the z slabs, at most the whole slab); 0 (default) = the interior part's chunk depth for the
+ * y-band rows and the whole z slab where it is at most two chunks deep.  Results do not depend on
+ * it.  (No counterpart in the reference.) */
+int fdtd_set_slab_chunk(fdtd_sim *s, int32_t planes);
 /* Batched engine (no counterpart in the reference; for many independent grids, each much smaller
  * than the GPU).  fdtd_batch_add queues `count` steps of a ready simulation from time index n0
  * with plane-wave mode `mode`; fdtd_batch_run advances every queued simulation to the end of its

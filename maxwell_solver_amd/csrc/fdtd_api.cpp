@@ -2017,7 +2017,7 @@ static int apply_tuning_class(fdtd_sim *s, int n, int mode) {
   if (it != g_tune.end()) {
     const StepTuning &t = it->second;
     if (t.tile_rows != s->tune.tile_rows || t.z_chunk != s->tune.z_chunk || t.cpl != s->tune.cpl ||
-        t.no_split != s->tune.no_split) {
+        t.no_split != s->tune.no_split || t.slab_chunk != s->tune.slab_chunk) {
       s->tune = t;
       s->args_gen++;
     }
@@ -2327,7 +2327,7 @@ int fdtd_set_tuning(fdtd_sim *s, int32_t rows_per_group, int32_t z_chunk, int32_
     return fail(FDTD_ERR_INVALID, "cols_per_lane must be 0, 1, 2, 4 or -1");
   s->tune.tile_rows = rows_per_group;
   s->tune.z_chunk = z_chunk;
-  s->tune_user = rows_per_group != 0 || z_chunk != 0 || cols_per_lane != 0;
+  s->tune_user = rows_per_group != 0 || z_chunk != 0 || cols_per_lane != 0 || s->tune.slab_chunk != 0;
   s->tune.cpl = cols_per_lane < 0 ? 1 : cols_per_lane;
   s->tune.no_split = cols_per_lane < 0;
   s->args_gen++;
@@ -2340,7 +2340,16 @@ int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]) {
   out[0] = s->tune.tile_rows;
   out[1] = s->tune.z_chunk;
   out[2] = s->tune.no_split ? -1 : s->tune.cpl;
-  out[3] = 0;
+  out[3] = s->tune.slab_chunk;
+  return 0;
+}
+
+int fdtd_set_slab_chunk(fdtd_sim *s, int32_t planes) {
+  NEED(s);
+  if (planes < 0) return fail(FDTD_ERR_INVALID, "slab chunk must be >= 0");
+  s->tune.slab_chunk = planes;
+  if (planes) s->tune_user = true;
+  s->args_gen++;
   return 0;
 }
 

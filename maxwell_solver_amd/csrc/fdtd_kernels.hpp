@@ -67,6 +67,7 @@ struct StepTuning {
   int z_chunk = 0;    // planes per chunk (0: automatic)
   int cpl = 0;        // columns per lane (1, 2, 4; 0: automatic)
   bool no_split = false;  // 1 column per lane without the split edge segment
+  int slab_chunk = 0;  // planes per chunk of the slab part (0: as the interior part; the z slabs whole when thin)
 };
 struct FusedDev {
   GridDev g;

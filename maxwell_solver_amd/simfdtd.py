@@ -32,7 +32,7 @@ EXPORTS = [
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
-    "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune",
+    "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune", "fdtd_set_slab_chunk",
 ]
 
 
@@ -89,6 +89,7 @@ def lib():
             "fdtd_stream_ceiling": [vp, i32, dp],
             "fdtd_get_tuning": [vp, ip],
             "fdtd_set_autotune": [vp, i32],
+            "fdtd_set_slab_chunk": [vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -281,6 +282,15 @@ class Simulation:
 
     def set_tuning(self, rows_per_group=0, z_chunk=0, cols_per_lane=0):
         self._chk(lib().fdtd_set_tuning(self._h, int(rows_per_group), int(z_chunk), int(cols_per_lane)))
+
+    def slab_chunk(self):
+        """Planes per z chunk of the row kernel's slab part in force (0 = automatic)."""
+        out = np.zeros(4, np.int32)
+        self._chk(lib().fdtd_get_tuning(self._h, _p(out, C.c_int32)))
+        return int(out[3])
+
+    def set_slab_chunk(self, planes=0):
+        self._chk(lib().fdtd_set_slab_chunk(self._h, int(planes)))
 
     def autotune(self, n=0, mode=0, reps=10):
         """Pick the fastest row layout and z-chunk for this grid by timing dry steps (state

@@ -262,6 +262,28 @@ def test_fused_tilings_oblique(I, rows, z_chunk, cpl, inset):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+@pytest.mark.parametrize("I,slab,z_chunk,inset", [
+    (64, 1, 0, 15), (64, 3, 0, 13), (128, 4, 6, 15), (256, 5, 0, 40), (52, 7, 3, 15), (136, 16, 0, 15),
+    (256, 2, 9, 15)])
+def test_slab_chunk_depths(I, slab, z_chunk, inset):
+    """The slab part's own z-chunk depth (y-band rows over the interior planes, z slabs cut into
+    chunks of at most that many planes): every depth equals the oracle, with the wave crossing the
+    TF/SF faces in the slabs' chunks."""
+    from maxwell_solver_amd import scenarios as S
+    J, K = (37, 45) if inset < 20 else (90, 88)
+    sc = _scenario(I=I, J=J, K=K, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7), inset=inset)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_tuning(0, z_chunk, 0)
+    g.set_slab_chunk(slab)
+    assert g.slab_chunk() == slab
+    g.run(0, sc.T, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def test_runs_and_steps_interleaved():
     """A run fills the next step's face sheets and gathers the previous step's probes inside the
     row kernel's slab-part launch: runs of one and two steps, single steps between runs, and a run
