@@ -448,6 +448,7 @@ struct StepArgs {
   int nwg0, nwg1;           // workgroups of the two parts (host information)
   int xni, xnj, xnk;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
   int spl;            // split edge segment (1 column per lane, I a multiple of 64; see rows_body)
+  int ntf;            // no TF/SF work in the slab part (the box lies clear of its rows and planes)
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -733,7 +734,9 @@ constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
 // MU: per-cell H coefficients (CreateMu): H = Chh H + C(i,j,k) curl E, which is K2's H + C curl E
 // bit for bit since Chh is exactly 1.
 // WB: write-back stores (a state that fits in the Infinity Cache), else non-temporal.
-template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false>
+// NTF: the slab part of a grid whose TF/SF box lies clear of the slab rows and planes (StepArgs
+// ntf): the march carries no TF/SF code (256^3 slab part 74.1 -> 70.0 us).
+template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false, bool NTF = false>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
   // it is loaded with scalar loads into scalar registers (also where the block's address was
@@ -744,7 +747,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   const int I = pa->I, J = pa->J, K = pa->K, P = pa->P;
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
   const int S = pa->S, R = PART == 0 ? pa->R0 : pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
-  const int act_h = uni(pa->act_h), act_e = uni(pa->act_e);
+  const int act_h = PART == 1 && NTF ? 0 : uni(pa->act_h), act_e = PART == 1 && NTF ? 0 : uni(pa->act_e);
   const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J;
   const float Chh = pa->Chh, Che = pa->Che, dx = pa->dx;
   const int P2 = 2 * P;
@@ -773,8 +776,12 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
   // the wave index is uniform across the wave; readfirstlane lets the compiler keep the row, its
   // offsets and flags in scalar registers (and branch on them) instead of per-lane values
-  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int seg = wv % S, r = wv / S;
+  // Waves of a workgroup take their segments in an order rotated per row and per workgroup (a hash
+  // of L), so that the edge segments of the workgroups resident on a CU do not all fall on the same
+  // SIMD (256^3 interior part 175.0 -> 173.7 us).
+  const int lane = threadIdx.x & 63, wp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = wp / S, rot = (int)(((uint32_t)L * 0x9E3779B1u) >> 30);
+  const int seg = uni((wp % S + r + rot) % S), wv = uni(r * S + seg);
   // Part 0: the interior band of rows [P+1, J-P) over the interior chunks of planes [P, K-P).
   // Part 1: the rest -- the bands [0, P] and [J-P, J) over every chunk, and the interior band over
   // the z-slab chunks.  The two parts read only the old state and write disjoint cells.
@@ -863,17 +870,20 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
   // y-slab rows: slab index and 1/kappa_y of rows j and j-1 (1 off the slabs)
   const AS4 float *prmc = (const AS4 float *)pa->prm;
-  const bool ys = j < P || j >= J - P, ysm = has_jm && (j - 1 < P || j - 1 >= J - P);
+  const bool ys0 = j < P || j >= J - P, ysm0 = has_jm && (j - 1 < P || j - 1 >= J - P);
   const int sy = j < P ? j : j - (J - P2), sym = j - 1 < P ? j - 1 : j - 1 - (J - P2);
   const int ikE0 = 4 * P2, ikH0 = 4 * P2 + I + J + K;  // 1/kappa tables of E and H: x, y, z
-  // Waves run one of three marches: 0 plain, 1 edge (x-slab columns, TF/SF faces, partial
-  // segments), 2 slab (y-slab rows and the rows above them, z-slab chunks; also the edge work).
+  // Waves run one of four marches: 0 plain, 1 edge (x-slab columns, TF/SF faces, partial
+  // segments), 2 slab (y-slab rows and the rows above them, z-slab chunks; also the edge work),
+  // 3 slab without the y CPML (the z-slab chunks of the rows between the y slabs: 1/kappa_y is
+  // exactly 1 there and its factors are omitted).
   auto march = [&](auto kind_tag) {
     constexpr int KIND = decltype(kind_tag)::value;
-    constexpr bool EDGE = KIND >= 1, SLAB = KIND == 2;
+    constexpr bool EDGE = KIND >= 1, SLAB = KIND >= 2, YC = KIND != 3;
+    const bool ys = YC && ys0, ysm = YC && ysm0;
     const bool lon = !EDGE || on;
-    const float kyH = SLAB ? prmc[ikH0 + I + j] : 1.0f, kyE = SLAB ? prmc[ikE0 + I + j] : 1.0f;
-    const float kyHm = SLAB && has_jm ? prmc[ikH0 + I + j - 1] : 1.0f;
+    const float kyH = SLAB && YC ? prmc[ikH0 + I + j] : 1.0f, kyE = SLAB && YC ? prmc[ikE0 + I + j] : 1.0f;
+    const float kyHm = SLAB && YC && has_jm ? prmc[ikH0 + I + j - 1] : 1.0f;
     const int syc = ys ? sy : 0, symc = ysm ? sym : 0;
     const float bHy = sSlab[2][syc], cHy = sSlab[3][syc], bEy = sSlab[0][syc], cEy = sSlab[1][syc];
     const float bHym = sSlab[2][symc], cHym = sSlab[3][symc];
@@ -953,7 +963,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       const O pn1 = SLAB ? uni(k + 1 < K ? pz : (O)0) : pz;
       unsigned f = lf;
       int ah = act_h, ae = act_e, jf = jfl;
-      if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
+      if constexpr (PART == 1 && NTF) asm volatile("" : "+v"(f), "+s"(jf));
+      else if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
       const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4;
       if (!EDGE || bit(f, F_ON)) {
         Fc.ld(o1, pn1, Lp.nx); Fc.ld(o1, c1 + pn1, Lp.ny); Fc.ld(o1, c2 + pn1, Lp.nz);
@@ -1038,7 +1049,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       const bool xk0 = k >= bk0 && k <= bk1 - 1, xk1 = k >= bk0 && k <= bk1;
       unsigned f = lf;
       int ah = act_h, ae = act_e, jf = jfl;
-      if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
+      if constexpr (PART == 1 && NTF) asm volatile("" : "+v"(f), "+s"(jf));
+      else if (EDGE) asm volatile("" : "+v"(f), "+s"(ah), "+s"(ae), "+s"(jf));
       const bool xj0 = jf & 1, xj1 = jf & 2, xjm1 = jf & 4, zrow = jf & 8, zrow_m = jf & 16;
       const bool lon = !EDGE || bit(f, F_ON);
 
@@ -1438,7 +1450,10 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
             for (int c = 0; c < CPL; ++c)
               if (slab_col(f, c)) {
                 const float v[4] = {Lp.pEy[c], Lp.pEz[c], Lp.pHy[c], Lp.pHz[c]};
-                Pn.template st<4, !WB>(so[c], pq, v);
+                // write-back: a row's block of 2P elements (320 bytes at P = 10) is not a whole
+                // number of 128-byte lines; the neighbouring rows' stores complete its lines in
+                // the L2 (256^3 interior part -0.5 us against non-temporal)
+                Pn.template st<4, false>(so[c], pq, v);
               }
           }
           if (SLAB && ys) {
@@ -1507,7 +1522,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     }
   } else {
     (void)slabw;
-    march(std::integral_constant<int, 2>{});
+    if (!ys0 && !ysm0) march(std::integral_constant<int, 3>{});
+    else march(std::integral_constant<int, 2>{});
   }
 }
 
@@ -1578,7 +1594,7 @@ __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
 }
 
 // The row kernel's workgroups [0, nwg); the slab part's launch may carry tail workgroups after them.
-template <bool IDX, bool BUF, int CPL, int PART, bool MU, bool WB = false>
+template <bool IDX, bool BUF, int CPL, int PART, bool MU, bool WB = false, bool NTF = false>
 __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *__restrict__ pa, int n, int nwg,
                                                                const TailArgs ta) {
   const int b = blockIdx.x;
@@ -1586,7 +1602,7 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *_
     tail_body<IDX>(ta, b - nwg);
     return;
   }
-  rows_body<IDX, BUF, CPL, PART, MU, WB>(pa, xcd_remap(b, nwg));
+  rows_body<IDX, BUF, CPL, PART, MU, WB, NTF>(pa, xcd_remap(b, nwg));
 }
 // The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
 // (measured 4% faster at 256^3 than the 68 the compiler picks freely).
@@ -1899,13 +1915,19 @@ static unsigned rows_lds(const StepArgs &a, int part) {
 
 template <bool IDX, bool BUF, int PART, bool MU>
 static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta, bool wb) {
+                          int n, const TailArgs &ta, bool wb, bool ntf) {
   const dim3 grid(nwg + ntail);
   if (grid.x == 0) return;
   if constexpr (BUF && !MU) {  // write-back variants: one column per lane (grids that fit the cache)
     if (wb && cpl == 1) {
       if constexpr (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF, true>), grid, blk, lds, st, d, n);
+      else if (ntf) hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART, false, true, true>), grid, blk, lds, st, d, n, nwg, ta);
       else hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART, false, true>), grid, blk, lds, st, d, n, nwg, ta);
+      return;
+    }
+    // the slab part without TF/SF code (one column per lane, buffer addressing)
+    if (PART == 1 && ntf && cpl == 1) {
+      hipLaunchKernelGGL((k_rows<IDX, BUF, 1, PART, false, false, true>), grid, blk, lds, st, d, n, nwg, ta);
       return;
     }
   }
@@ -1920,9 +1942,9 @@ static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3
 // mu: the per-cell H coefficients of CreateMu are in use
 template <bool IDX, bool BUF, int PART>
 static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta, bool mu, bool wb) {
-  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb);
-  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb);
+                          int n, const TailArgs &ta, bool mu, bool wb, bool ntf) {
+  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf);
+  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf);
 }
 
 // part: 0 the interior part, 1 the slab part (with the tail workgroups of ta, if any), -1 both
@@ -1938,6 +1960,7 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // the state (two field sets, two psi sets, materials) fits in the 256 MB Infinity Cache
   const size_t state = 2 * 24 * a.cs + 2 * 16 * (a.sx + a.sy + a.sz) + (a.mat ? a.IJK : 24 * a.IJK);
   const bool wb = state <= ((size_t)256 << 20);
+  const bool ntf = a.ntf != 0;
   const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
   TailArgs ta{};
   if (tail && part != 0) ta = *tail;
@@ -1945,11 +1968,11 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
   }
   return hipGetLastError();
 }
@@ -2004,6 +2027,11 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.S = S; a.R = R; a.CPL = cpl; a.nTy = nTy;
   a.spl = (cpl == 1 && g.I % 64 == 0 && !tune.no_split) ? 1 : 0;
   a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
+  // The slab part meets the TF/SF corrections only where the box reaches its rows (j <= P,
+  // j >= J-P; a row's H of row j-1 included) or planes (k < P, k >= K-P, and the plane K-P-1 below
+  // the upper z-slab chunks): faces on the rows j0-1..j1+1 and the planes k0-1..k1+1.
+  a.ntf = !(f.act_h | f.act_e) ||
+          (g.j0 >= g.P + 2 && g.j1 <= g.J - g.P - 2 && g.k0 >= g.P + 2 && g.k1 <= g.K - g.P - 2);
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
   a.Wrow = 64 * cpl * S;
   a.nTlo = nTlo; a.nThi = nThi; a.nTmid = nTmid;
