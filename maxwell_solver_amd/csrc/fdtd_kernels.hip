@@ -1627,12 +1627,12 @@ __device__ __forceinline__ int find_grid(const BatchGrid *__restrict__ tab, int 
 
 // Blocks of one grid whose local index agrees mod 8 run on the same XCD (blocks are dealt to the
 // XCDs round-robin), so the per-grid remap is xcd_remap of the local index.
-template <bool IDX, bool BUF, int CPL, int PART>
+template <bool IDX, bool BUF, int CPL, int PART, bool NTF = false>
 __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows_batch(const BatchGrid *__restrict__ tab, int ng, int step) {
   const int b = blockIdx.x, g = find_grid(tab, ng, b);
   const int par = (tab[g].cur0 + step) & 1;
   const int off = tab[g].off, nb = tab[g + 1].off - off;
-  rows_body<IDX, BUF, CPL, PART, false>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
+  rows_body<IDX, BUF, CPL, PART, false, false, NTF>((const StepArgs *)tab[g].arg[par], xcd_remap(b - off, nb));
 }
 
 
@@ -2116,6 +2116,7 @@ RowsKey rows_key(const void *hargs) {
   k.threads0 = 64 * a.S * a.R0;
   k.lds = rows_lds(a, 1);
   k.lds0 = rows_lds(a, 0);
+  k.ntf = a.ntf != 0;
   return k;
 }
 
@@ -2131,7 +2132,13 @@ void make_slab_args(const FusedDev &f, void *out, unsigned *nb_sheets) {
 
 template <bool IDX, bool BUF, int PART>
 static void launch_rows_batch_w(hipStream_t st, int cpl, dim3 grid, dim3 blk, unsigned lds, const BatchGrid *tab, int ng,
-                                int step) {
+                                int step, bool ntf) {
+  if constexpr (BUF && PART == 1) {  // every grid of the launch has its TF/SF box clear of the slab part
+    if (ntf && cpl == 1) {
+      hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1, PART, true>), grid, blk, lds, st, tab, ng, step);
+      return;
+    }
+  }
   if (cpl == 4) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 4, PART>), grid, blk, lds, st, tab, ng, step);
   else if (cpl == 2) hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 2, PART>), grid, blk, lds, st, tab, ng, step);
   else hipLaunchKernelGGL((k_rows_batch<IDX, BUF, 1, PART>), grid, blk, lds, st, tab, ng, step);
@@ -2141,11 +2148,11 @@ template <int PART>
 static void launch_rows_batch_p(hipStream_t st, const RowsKey &key, dim3 grid, dim3 blk, const BatchGrid *tab, int ng,
                                 int step) {
   if (key.idx) {
-    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
-    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<true, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step, key.ntf);
+    else launch_rows_batch_w<true, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step, key.ntf);
   } else {
-    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
-    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step);
+    if (key.buf) launch_rows_batch_w<false, true, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step, key.ntf);
+    else launch_rows_batch_w<false, false, PART>(st, key.cpl, grid, blk, PART == 0 ? key.lds0 : key.lds, tab, ng, step, key.ntf);
   }
 }
 

@@ -159,7 +159,9 @@ struct RowsKey {
   bool idx = false, buf = false;
   int cpl = 1, threads = 256, threads0 = 256;  // workgroup sizes of the slab part and the interior part
   unsigned lds = 0, lds0 = 0;                    // their dynamic LDS bytes (row exchange)
+  bool ntf = false;                              // the slab part needs no TF/SF code
   bool operator<(const RowsKey &o) const {
+    if (ntf != o.ntf) return ntf < o.ntf;
     if (idx != o.idx) return idx < o.idx;
     if (buf != o.buf) return buf < o.buf;
     if (cpl != o.cpl) return cpl < o.cpl;
