@@ -195,6 +195,11 @@ int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]);
  * y-band rows and the whole z slab where it is at most two chunks deep.  Results do not depend on
  * it.  (No counterpart in the reference.) */
 int fdtd_set_slab_chunk(fdtd_sim *s, int32_t planes);
+/* Addressing of the fused step's row kernel: 0 (default) = buffer resources with 32-bit offsets
+ * where every array it touches is under 4 GiB, 64-bit global addresses otherwise; 1 = the 64-bit
+ * global path on any grid (it serves grids of more than about 178 M cells; forcing it lets a
+ * small grid test that path).  Results do not depend on it.  (No counterpart in the reference.) */
+int fdtd_set_addressing(fdtd_sim *s, int32_t mode);
 /* Batched engine (no counterpart in the reference; for many independent grids, each much smaller
  * than the GPU).  fdtd_batch_add queues `count` steps of a ready simulation from time index n0
  * with plane-wave mode `mode`; fdtd_batch_run advances every queued simulation to the end of its

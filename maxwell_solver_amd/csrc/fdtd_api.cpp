@@ -126,6 +126,7 @@ struct fdtd_sim {
   int nsnap = 0;
   PWDev pwdev[8];
   StepTuning tune;
+  bool addr_global = false;  // fdtd_set_addressing(1): the row kernel's 64-bit global path
   bool tune_user = false;  // fdtd_set_tuning chose the tuning: the size-class table leaves it
   int tune_policy = 1;     // fdtd_set_autotune: 0 heuristics, 1 size-class table, 2 table + autotune on a miss
   // device argument blocks of the fused kernel per (state parity, plane-wave mode); rebuilt when
@@ -738,6 +739,7 @@ struct fdtd_sim {
   void fill_fused(int which, int mode, const PWDev *pw, float *xs, FusedDev &f) const {
     f = FusedDev();
     f.g = g;
+    f.force_global = addr_global;
     f.Fc = dFb[which]; f.Fn = dFb[1 - which]; f.Pc = dPsiB[which]; f.Pn = dPsiB[1 - which];
     f.cs = cs;
     f.mat = index_mode ? dMat : nullptr;
@@ -2341,6 +2343,14 @@ int fdtd_get_tuning(fdtd_sim *s, int32_t out[4]) {
   out[1] = s->tune.z_chunk;
   out[2] = s->tune.no_split ? -1 : s->tune.cpl;
   out[3] = s->tune.slab_chunk;
+  return 0;
+}
+
+int fdtd_set_addressing(fdtd_sim *s, int32_t mode) {
+  NEED(s);
+  if (mode != 0 && mode != 1) return fail(FDTD_ERR_INVALID, "addressing mode must be 0 (automatic) or 1 (global)");
+  s->addr_global = mode == 1;
+  s->args_gen++;
   return 0;
 }
 

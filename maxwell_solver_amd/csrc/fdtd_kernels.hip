@@ -449,6 +449,7 @@ struct StepArgs {
   int xni, xnj, xnk;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
   int spl;            // split edge segment (1 column per lane, I a multiple of 64; see rows_body)
   int ntf;            // no TF/SF work in the slab part (the box lies clear of its rows and planes)
+  int glob;           // 64-bit global addressing although every array fits a buffer resource
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -1954,7 +1955,7 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // buffer addressing when every array the kernel touches (a state set, a psi set, the material
   // bytes or the per-cell coefficients) is below 4 GiB
   const size_t lim = ((size_t)1 << 32) - 1;
-  const bool buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim &&
+  const bool buf = !a.glob && 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim &&
                    (!a.mu || 12 * a.IJK < lim);
   const bool mu = a.mu != nullptr;
   // the state (two field sets, two psi sets, materials) fits in the 256 MB Infinity Cache
@@ -2030,6 +2031,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   // The slab part meets the TF/SF corrections only where the box reaches its rows (j <= P,
   // j >= J-P; a row's H of row j-1 included) or planes (k < P, k >= K-P, and the plane K-P-1 below
   // the upper z-slab chunks): faces on the rows j0-1..j1+1 and the planes k0-1..k1+1.
+  a.glob = f.force_global ? 1 : 0;
   a.ntf = !(f.act_h | f.act_e) ||
           (g.j0 >= g.P + 2 && g.j1 <= g.J - g.P - 2 && g.k0 >= g.P + 2 && g.k1 <= g.K - g.P - 2);
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
@@ -2110,7 +2112,7 @@ RowsKey rows_key(const void *hargs) {
   const size_t lim = ((size_t)1 << 32) - 1;
   RowsKey k;
   k.idx = a.mat != nullptr;
-  k.buf = 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
+  k.buf = !a.glob && 24 * a.cs < lim && 16 * (a.sx + a.sy + a.sz) < lim && (a.mat ? a.IJK : 24 * a.IJK) < lim;
   k.cpl = a.CPL;
   k.threads = 64 * a.S * a.R;
   k.threads0 = 64 * a.S * a.R0;

@@ -71,6 +71,7 @@ struct StepTuning {
 };
 struct FusedDev {
   GridDev g;
+  bool force_global = false;  // the row kernel's 64-bit global addressing even where buffers fit
   const float *Fc = nullptr;  // six fields, component c at c*cs
   size_t cs = 0;
   float *Fn = nullptr;

@@ -32,7 +32,7 @@ EXPORTS = [
     "fdtd_server_create", "fdtd_server_destroy", "fdtd_server_now", "fdtd_server_submit", "fdtd_server_round",
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
-    "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune", "fdtd_set_slab_chunk",
+    "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune", "fdtd_set_slab_chunk", "fdtd_set_addressing",
 ]
 
 
@@ -89,7 +89,7 @@ def lib():
             "fdtd_stream_ceiling": [vp, i32, dp],
             "fdtd_get_tuning": [vp, ip],
             "fdtd_set_autotune": [vp, i32],
-            "fdtd_set_slab_chunk": [vp, i32],
+            "fdtd_set_slab_chunk": [vp, i32], "fdtd_set_addressing": [vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -291,6 +291,10 @@ class Simulation:
 
     def set_slab_chunk(self, planes=0):
         self._chk(lib().fdtd_set_slab_chunk(self._h, int(planes)))
+
+    def set_addressing(self, mode=0):
+        """0: buffer addressing where arrays fit (default); 1: the 64-bit global path."""
+        self._chk(lib().fdtd_set_addressing(self._h, int(mode)))
 
     def autotune(self, n=0, mode=0, reps=10):
         """Pick the fastest row layout and z-chunk for this grid by timing dry steps (state

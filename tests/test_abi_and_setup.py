@@ -202,4 +202,8 @@ def test_new_entry_points_validate_before_the_device():
         s.set_plane_wave(mode, [0.0])
     with pytest.raises(SimFDTDError):
         s.set_plane_wave(8, [0.0])
+    s.set_addressing(1)  # the row kernel's 64-bit global path on any grid
+    s.set_addressing(0)
+    with pytest.raises(SimFDTDError):
+        s.set_addressing(2)
     s.close()

@@ -262,6 +262,85 @@ def test_fused_tilings_oblique(I, rows, z_chunk, cpl, inset):
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+# The row kernel's 64-bit global path (grids whose arrays exceed 4 GiB take it) forced on small
+# grids: the row-layout families (1, 2 and 4 columns per lane, split and unsplit edge segments,
+# several rows per tile), per-cell mu, a TF/SF box inside the CPML (the slab march with its TF/SF
+# code) and a merged batch.  The buffer path reads 0 and drops stores out of range, the global
+# path would not, so this also shows that no access of the row kernel leaves its array.
+GLOBAL_CASES = [(52, 0, 0, 0, 15), (52, 2, 13, 4, 15), (136, 0, 0, 1, 15), (200, 0, 7, 2, 15), (264, 1, 4, 2, 15),
+                (64, 4, 6, 1, 13), (128, 2, 5, 1, 40), (256, 0, 0, 1, 15), (256, 0, 0, -1, 15), (300, 0, 6, 0, 15)]
+
+
+def _global_case(I, rows, z_chunk, cpl, inset):
+    from maxwell_solver_amd import scenarios as S
+    J, K = (37, 45) if inset < 20 else (90, 88)
+    sc = _scenario(I=I, J=J, K=K, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7), inset=inset)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_addressing(1)
+    g.set_tuning(rows, z_chunk, cpl)
+    g.run(0, sc.T, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def _global_mu_and_inset():
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=64, J=37, K=45, T=40, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7))
+    mu = _mu(sc, 13)
+    g, o = _backends()
+    S.setup(g, sc, mu=mu)
+    S.setup(o, sc, mu=mu)
+    g.set_addressing(1)
+    g.run(0, sc.T, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    sc = _scenario(I=64, J=44, K=48, mode=5, cpml=CFG_CPML, inset=6, spheres=False, oblique=(0.8, 2.0, 0.5))
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    g.set_addressing(1)
+    g.run(0, sc.T, 5)
+    o.run(0, sc.T, 5)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
+def _global_batch():
+    from maxwell_solver_amd import Batch
+    from maxwell_solver_amd import scenarios as S
+    b = Batch()
+    pairs = []
+    for c in (dict(I=40, J=44, K=48, mode=3, cpml=REF_CPML, count=37),
+              dict(I=52, J=37, K=45, mode=5, cpml=CFG_CPML, oblique=(0.9, 2.3, 0.7), count=60)):
+        count = c.pop("count")
+        sc = _scenario(T=60, **c)
+        g, o = _backends()
+        S.setup(g, sc)
+        S.setup(o, sc)
+        g.set_addressing(1)
+        b.add(g, 0, count, sc.mode)
+        pairs.append((g, o, sc, count))
+    b.run()
+    for g, o, sc, count in pairs:
+        o.run(0, count, sc.mode)
+        assert_close(compare_state(g, o), exact=True)
+        np.testing.assert_array_equal(g.probes(), o.probes())
+    b.close()
+
+
+@pytest.mark.parametrize("I,rows,z_chunk,cpl,inset", GLOBAL_CASES)
+def test_forced_global_addressing(I, rows, z_chunk, cpl, inset):
+    _global_case(I, rows, z_chunk, cpl, inset)
+
+
+def test_forced_global_addressing_mu_inset_batch():
+    _global_mu_and_inset()
+    _global_batch()
+
+
 @pytest.mark.parametrize("I,slab,z_chunk,inset", [
     (64, 1, 0, 15), (64, 3, 0, 13), (128, 4, 6, 15), (256, 5, 0, 40), (52, 7, 3, 15), (136, 16, 0, 15),
     (256, 2, 9, 15)])
