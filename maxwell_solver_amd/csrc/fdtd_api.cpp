@@ -1903,7 +1903,8 @@ std::string packaged_tune_path() {
 }
 
 // The file: {"classes": [{"I": 256, "log2J": 8, "log2K": 8, "P": 10, "index": 1, "plane_wave": 1,
-// "mu": 0, "rows": 0, "z_chunk": 8, "cols_per_lane": 1}, ...]} (integers only).
+// "mu": 0, "rows": 0, "z_chunk": 8, "cols_per_lane": 1, "slab_chunk": 0}, ...]} (integers only;
+// an entry without "slab_chunk" keeps the slab part's automatic chunks).
 void tune_load_locked() {
   std::string path = tune_path();
   if (path.empty()) path = packaged_tune_path();
@@ -1937,7 +1938,7 @@ void tune_load_locked() {
     c.I = kv["I"]; c.Jl = kv["log2J"]; c.Kl = kv["log2K"]; c.P = kv["P"];
     c.idx = kv["index"]; c.pw = kv["plane_wave"]; c.mu = kv["mu"];
     StepTuning t;
-    t.tile_rows = kv["rows"]; t.z_chunk = kv["z_chunk"];
+    t.tile_rows = kv["rows"]; t.z_chunk = kv["z_chunk"]; t.slab_chunk = kv["slab_chunk"];
     const int cpl = kv["cols_per_lane"];
     t.cpl = cpl < 0 ? 1 : cpl;
     t.no_split = cpl < 0;
@@ -1958,9 +1959,9 @@ void tune_save_locked() {
     const TuneClass &c = kv.first;
     const StepTuning &t = kv.second;
     std::fprintf(f, "%s\n  {\"I\": %d, \"log2J\": %d, \"log2K\": %d, \"P\": %d, \"index\": %d, \"plane_wave\": %d, "
-                 "\"mu\": %d, \"rows\": %d, \"z_chunk\": %d, \"cols_per_lane\": %d}",
+                 "\"mu\": %d, \"rows\": %d, \"z_chunk\": %d, \"cols_per_lane\": %d, \"slab_chunk\": %d}",
                  first ? "" : ",", c.I, c.Jl, c.Kl, c.P, c.idx, c.pw, c.mu, t.tile_rows, t.z_chunk,
-                 t.no_split ? -1 : t.cpl);
+                 t.no_split ? -1 : t.cpl, t.slab_chunk);
     first = false;
   }
   std::fprintf(f, "\n]}\n");
@@ -1969,8 +1970,9 @@ void tune_save_locked() {
 }
 
 // The autotuner: every columns-per-lane and z-chunk choice and two tile heights, each timed over
-// `reps` dry steps in a run's launch form; a choice replaces the automatic one only when it is
-// faster by more than 1% (timing noise would otherwise pick between equals).
+// `reps` dry steps in a run's launch form, then the slab part's chunk depth with the best of
+// those; a choice replaces the automatic one only when it is faster by more than 1% (timing noise
+// would otherwise pick between equals).
 int autotune_sim(fdtd_sim *s, int n, int mode, int reps, double *best_ms) {
   const StepTuning saved = s->tune;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1994,11 +1996,27 @@ int autotune_sim(fdtd_sim *s, int n, int mode, int reps, double *best_ms) {
         if (rows == 0 && cpl == 0 && kc == 0) base = ms;
         if (best_t < 0 || ms < best_t) { best_t = ms; best = s->tune; }
       }
+  if (base > 0 && best_t > 0.99 * base) { best = StepTuning(); best_t = base; }
+  // the slab part's chunk depth (y-band chunks of the interior planes, z slabs cut to at most as
+  // many planes) with the layout chosen above
+  if (!rc && best_t > 0) {
+    const StepTuning lay = best;
+    const double lay_t = best_t;
+    for (int sc : {3, 4, 6, 8, 12, 16}) {
+      s->tune = lay;
+      s->tune.slab_chunk = sc;
+      s->args_gen++;
+      double ms = 0;
+      const int r = s->time_variant(n, mode, reps, e0, e1, &ms);
+      if (r == FDTD_ERR_DEVICE) { rc = r; break; }
+      if (r) continue;
+      if (ms < 0.99 * lay_t && ms < best_t) { best_t = ms; best = s->tune; }
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc) { s->tune = saved; s->args_gen++; return rc; }
   if (best_t < 0) { s->tune = saved; s->args_gen++; return fail(FDTD_ERR_INVALID, "grid outside the fused step's layouts"); }
-  if (base > 0 && best_t > 0.99 * base) { best = StepTuning(); best_t = base; }
   s->tune = best;
   s->args_gen++;
   g_err.clear();

@@ -443,6 +443,7 @@ struct StepArgs {
   int R0, nTmid0;      // rows per workgroup and row tiles of the interior part
   int KC, ntab, act_h, act_e;
   int KCS, nz1, nz2;  // z chunks: nz1 of KCS planes over [0, P), nz2 of KC over [P, K-P), nz1 over [K-P, K)
+  int KCY, nzy;       // the slab part's y-band rows over [P, K-P): nzy chunks of KCY planes
   int Wrow;           // LDS row stride of the row exchange: 64 CPL S columns
   int nTlo, nThi, nTmid;    // row tiles of the bands [0, P], [J-P, J) and [P+1, J-P)
   int nwg0, nwg1;           // workgroups of the two parts (host information)
@@ -793,7 +794,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       const int nT0 = pa->nTmid0;
       row0 = P + 1 + R * (L % nT0); row_end = J - P; tz = n1 + L / nT0;
     } else {
-      const int nb = nTlo + nThi, nA = nb * (2 * n1 + n2);
+      const int ny = pa->nzy, nb = nTlo + nThi, nA = nb * (2 * n1 + ny);
       if (L < nA) {
         const int t = L % nb;
         tz = L / nb;
@@ -801,18 +802,20 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         else { row0 = J - P + R * (t - nTlo); row_end = J; }
       } else {
         const int q = L - nA, zz = q / nTmid;
-        row0 = P + 1 + R * (q % nTmid); row_end = J - P; tz = zz < n1 ? zz : n1 + n2 + (zz - n1);
+        row0 = P + 1 + R * (q % nTmid); row_end = J - P; tz = zz < n1 ? zz : n1 + ny + (zz - n1);
       }
+      (void)n2;
     }
     (void)nTy;
   }
   const int j = uni(row0 + r);
-  // z chunk of the wave; chunks do not straddle the z-slab boundaries P and K-P
+  // z chunk of the wave; chunks do not straddle the z-slab boundaries P and K-P (the interior
+  // planes [P, K-P) in chunks of KC planes in the interior part, of KCY in the slab part)
   int kb, ke;
   {
-    const int n1 = pa->nz1, n2 = pa->nz2, KCS = pa->KCS;
+    const int n1 = pa->nz1, n2 = PART == 0 ? pa->nz2 : pa->nzy, KCS = pa->KCS, KCm = PART == 0 ? KC : pa->KCY;
     if (tz < n1) { kb = tz * KCS; ke = min(kb + KCS, P); }
-    else if (tz < n1 + n2) { kb = P + (tz - n1) * KC; ke = min(kb + KC, K - P); }
+    else if (tz < n1 + n2) { kb = P + (tz - n1) * KCm; ke = min(kb + KCm, K - P); }
     else { kb = K - P + (tz - n1 - n2) * KCS; ke = min(kb + KCS, K); }
     kb = uni(kb); ke = uni(ke);
   }
@@ -2018,9 +2021,17 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
     while (kc > 2 && layer * ((cK + kc - 1) / kc) < 16384) kc /= 2;
     if (kc > cK) kc = cK;
   }
-  const int kcs = g.P <= 2 * kc ? g.P : kc;  // z-slab chunks: the whole slab when it is thin
-  const int nz1 = (g.P + kcs - 1) / kcs, nz2 = (cK + kc - 1) / kc;
-  const int nKc = 2 * nz1 + nz2;
+  // The slab part: its y-band rows take the interior planes in chunks of kcy planes and every row
+  // its z slabs in chunks of kcs (the whole slab when it is thin); tune.slab_chunk sets both (the
+  // z slabs cut into chunks of at most that many planes).  By default a grid with full-depth
+  // interior chunks gives its y-band rows chunks as deep as a whole z slab: the slab part's waves
+  // are then all of one length, fewer and with less recompute (256^3, P = 10: slab part 68.0 ->
+  // 65.5 us with 10-plane y-band chunks against 8).
+  const int sc = tune.slab_chunk;
+  const int kcs = sc > 0 ? std::min(sc, g.P) : (g.P <= 2 * kc ? g.P : kc);
+  const int kcy = sc > 0 ? std::min(sc, cK) : (kc >= 8 && kcs == g.P ? std::min(std::max(kc, g.P), cK) : kc);
+  const int nz1 = (g.P + kcs - 1) / kcs, nz2 = (cK + kc - 1) / kc, nzy = (cK + kcy - 1) / kcy;
+  const int nKc = 2 * nz1 + nzy;
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
@@ -2035,6 +2046,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.ntf = !(f.act_h | f.act_e) ||
           (g.j0 >= g.P + 2 && g.j1 <= g.J - g.P - 2 && g.k0 >= g.P + 2 && g.k1 <= g.K - g.P - 2);
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;
+  a.KCY = kcy; a.nzy = nzy;
   a.Wrow = 64 * cpl * S;
   a.nTlo = nTlo; a.nThi = nThi; a.nTmid = nTmid;
   a.R0 = R0; a.nTmid0 = nTmid0;

@@ -1,6 +1,7 @@
 """Writes the size-class tuning table shipped beside the library (tuning_gfx950.json): the autotuner
-(fdtd_autotune: every columns-per-lane and z-chunk choice and two tile heights, timed in a run's
-launch form, the automatic choice kept unless another is faster by more than 1%) run on one
+(fdtd_autotune: every columns-per-lane and z-chunk choice and two tile heights, then the slab part's
+chunk depth, timed in a run's launch form, the automatic choice kept unless another is faster by
+more than 1%) run on one
 simulation of each size class of the workloads -- the 256^3 sphere grid, the 384x384x256 layered
 ground, the GIC grid, the 96 grids of the batched workload and a sample of the served requests.
 Run on the GPU the table is for:  python -m maxwell_solver_amd.tune_table [out.json]
@@ -38,7 +39,7 @@ def main(out: str) -> None:
     with open(tmp) as f:
         table = json.load(f)
     table["note"] = ("autotuned on one MI355X (gfx950) by maxwell_solver_amd/tune_table.py; rows / z_chunk / "
-                     "cols_per_lane 0 mean the automatic choice")
+                     "cols_per_lane / slab_chunk 0 mean the automatic choice")
     with open(out, "w") as f:
         json.dump(table, f, indent=1)
     print(f"{len(table['classes'])} classes -> {out}")

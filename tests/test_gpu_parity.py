@@ -119,6 +119,31 @@ def test_fused_run_with_mu():
     np.testing.assert_array_equal(g.probes(), o.probes())
 
 
+# TF/SF box faces inside the 10-cell CPML (inset 6: the faces lie in slab rows, slab columns and
+# slab planes; inset 10: the faces sit on the slabs' last cells), every plane-wave mode, per-op
+# sequence and fused run + step
+@pytest.mark.parametrize("inset", [6, 10])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5, 6, 7])
+def test_tfsf_box_inside_cpml(mode, inset):
+    from maxwell_solver_amd import scenarios as S
+    sc = _scenario(I=44, J=40, K=46, T=40, mode=mode, cpml=CFG_CPML, inset=inset,
+                   oblique=(0.9, 2.3, 0.7) if mode == 5 else None)
+    g, o = _backends()
+    S.setup(g, sc)
+    S.setup(o, sc)
+    if mode != 5:  # oblique mode 5 has no per-op form (the reference dispatches modes 1-4)
+        _per_op_steps(g, 20, mode)
+        _per_op_steps(o, 20, mode)
+        assert_close(compare_state(g, o), exact=True)
+        g.reset()
+        o.reset()
+    g.run(0, sc.T - 1, mode)
+    g.step(sc.T - 1, mode)
+    o.run(0, sc.T, mode)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+
+
 def _gic_oracle_steps(o, n0, count, box):
     for n in range(n0, n0 + count):
         o.update_h_all()
