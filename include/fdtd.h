@@ -100,6 +100,20 @@ int fdtd_output_vis(fdtd_sim *s, float *out, uint32_t size, int32_t flag);
 int fdtd_set_cpml(fdtd_sim *s, const double params[6]);
 /* TF/SF box inset from the grid faces (reference 15, src/simulation.h:102); before ReadySimulation */
 int fdtd_set_tfsf(fdtd_sim *s, int32_t inset);
+/* Graded (tensor-product) mesh, between SetConst and ReadySimulation: dx[I], dy[J], dz[K] are the
+ * distances between consecutive grid planes along x, y, z (entry n: planes n and n+1); a null
+ * pointer keeps the uniform dx of SetConst along that axis.  Replaces the reference's
+ * non-uniform-mesh scaffolding (FDTD_UNIFORM_MESH and den_ey/den_ez/den_hy/den_hz,
+ * src/simulation.h:2, 679-685, 718-724, allocated but never read): the H update differences E
+ * over the cell n (spacing d[n]), the E update differences H over the dual cell n
+ * ((d[n-1] + d[n]) / 2), both as factors dx/spacing on the per-axis den tables (den_h* and den_e*
+ * of :861-912).  The CPML cells (P+1 at each end of an axis, the last entry included) must keep
+ * dx; every spacing must be positive, and SetConst's dt within the Courant limit of the finest
+ * cells, dt <= 1/(c sqrt(1/min dx^2 + 1/min dy^2 + 1/min dz^2)) (checked at ReadySimulation).  A graded mesh takes no plane wave (modes 1..7 are refused:
+ * the 1D incident line is uniform); AddCurrents and step currents keep the reference's source
+ * scaling (src/gpu.h:55-82).  Fused steps of a graded grid run the row kernel's full-slab march
+ * (one column per lane; with per-cell mu, the per-op operations). */
+int fdtd_set_mesh(fdtd_sim *s, const float *dx, const float *dy, const float *dz);
 /* Plane-wave set-up used by Init1D.  mode 1..4, 6, 7: params[0] != 0 makes the 1D line carry the
  * grid's centre-column material profile (layered media).  mode 5: oblique incidence,
  * params = [theta, phi, psi] (radians), dispersion-matched 1D line. */

@@ -77,6 +77,9 @@ struct fdtd_sim {
   CpmlProfiles prof;
   std::vector<float> ikE[3], ikH[3];
   float Chh = 1, Che = 0, den = 0;
+  // graded mesh (fdtd_set_mesh): cell sizes along x, y, z (empty: uniform dx along that axis)
+  std::vector<float> mesh[3];
+  bool graded() const { return !mesh[0].empty() || !mesh[1].empty() || !mesh[2].empty(); }
   GridDev g{};
   bool index_mode = true;
   std::vector<uint8_t> cell_mat;   // index mode: material id per cell
@@ -388,6 +391,34 @@ struct fdtd_sim {
     prof = compute_cpml(cp, delX, delT);
     ikE[0] = inv_kappa_E(prof, P, I); ikE[1] = inv_kappa_E(prof, P, J); ikE[2] = inv_kappa_E(prof, P, K);
     ikH[0] = inv_kappa_H(prof, P, I); ikH[1] = inv_kappa_H(prof, P, J); ikH[2] = inv_kappa_H(prof, P, K);
+    // graded mesh: the H update differences E across the primal cell n (spacing d[n]), the E
+    // update H across the dual cell n (spacing (d[n-1] + d[n]) / 2); both enter the per-axis tables
+    // as dx / spacing (the reference's den_h* and den_e*, src/simulation.h:861-912, 679-685), the
+    // CPML cells keeping dx so that the psi updates and the CPML tables are the uniform grid's
+    if (graded()) {
+      const size_t Q[3] = {(size_t)I, (size_t)J, (size_t)K};
+      const double dx = (double)delX;
+      // the time step of SetConst must stay within the Courant limit of the finest cells,
+      // dt <= 1 / (c sqrt(sum over the axes of 1 / min spacing^2))
+      double inv2 = 0;
+      for (int a = 0; a < 3; ++a) {
+        double m = dx;
+        for (float v : mesh[a]) m = std::min(m, (double)v);
+        inv2 += 1.0 / (m * m);
+      }
+      if ((double)delT > 1.0 / (kC0 * std::sqrt(inv2)))
+        return fail(FDTD_ERR_INVALID, "SetMesh: dt exceeds the Courant limit of the finest cells");
+      for (int a = 0; a < 3; ++a) {
+        const std::vector<float> &d = mesh[a];
+        if (d.empty()) continue;
+        for (size_t n = 0; n < Q[a]; ++n)
+          if ((n <= (size_t)P || n + (size_t)P + 2 >= Q[a]) && (double)d[n] != dx)
+            return fail(FDTD_ERR_INVALID, "SetMesh: the CPML cells (P+1 at each end) must keep the spacing dx");
+        for (size_t n = 0; n + 1 < Q[a]; ++n) ikH[a][n] = (float)((double)ikH[a][n] * (dx / (double)d[n]));
+        for (size_t n = 1; n < Q[a]; ++n)
+          ikE[a][n] = (float)((double)ikE[a][n] * (dx / (0.5 * ((double)d[n - 1] + (double)d[n]))));
+      }
+    }
     // src/simulation.h:914-922 with sigma_m = 0; den_ex = 1/dx (kappa is in the 1/kappa tables)
     const float sigma_m = 0;
     Chh = (float)((1 - sigma_m * delT / 2 / kMu0) / (1 + sigma_m * delT / 2 / kMu0));
@@ -661,6 +692,7 @@ struct fdtd_sim {
   int check_pw(int n, int mode, const PWDev **out) {
     *out = nullptr;
     if (mode < 1 || mode > 7) return 0;  // the reference ignores other modes
+    if (graded()) return fail(FDTD_ERR_INVALID, "a graded mesh takes no plane wave (the 1D incident line is uniform)");
     if (!have_1d) return fail(FDTD_ERR_STATE, "plane wave before Init1D");
     if (n < 0 || n >= T) return fail(FDTD_ERR_INVALID, "time index outside [0, T)");
     if (!pwdev[mode].active_h) return fail(FDTD_ERR_STATE, "the 1D line for this plane-wave mode was not initialised");
@@ -753,6 +785,7 @@ struct fdtd_sim {
     }
     f.Chh = Chh; f.Che = Che; f.dx = delX;
     f.Chmu = dChmu[0];  // the three arrays are one block
+    f.gr = graded();
   }
 
   // State carried from step to step by a run of consecutive steps (fdtd_run): which face sheets
@@ -1105,7 +1138,7 @@ static int batch_run_merged(fdtd_batch *b, const std::vector<fdtd_batch::Item> &
       FusedDev f;
       fdtd_sim::ArgSlot *slot = nullptr;
       if (int rc = s->fused_slot(w, it.mode, pw, f, &slot)) return rc;
-      if (slot->nwg < 0 || s->step_pol || s->dChmu[0]) {  // per-op layouts, step currents, mu run alone
+      if (slot->nwg < 0 || s->step_pol || s->dChmu[0] || s->graded()) {  // per-op layouts, step currents, mu, graded meshes run alone
         ok = false;
         break;
       }
@@ -1479,6 +1512,24 @@ int fdtd_set_cpml(fdtd_sim *s, const double p[6]) {
   s->cp.alpha_max = (float)p[3];
   s->cp.alpha_m = (float)p[4];
   s->cp.sig_factor = p[5];
+  return 0;
+}
+
+int fdtd_set_mesh(fdtd_sim *s, const float *dx, const float *dy, const float *dz) {
+  NEED(s);
+  if (!s->init_const) return fail(FDTD_ERR_STATE, "SetMesh before SetConst");
+  if (s->init_ready) return fail(FDTD_ERR_STATE, "SetMesh after ReadySimulation");
+  const float *d[3] = {dx, dy, dz};
+  const int Q[3] = {s->I, s->J, s->K};
+  for (int a = 0; a < 3; ++a)
+    if (d[a])
+      for (int n = 0; n < Q[a]; ++n)
+        if (!std::isfinite(d[a][n]) || !(d[a][n] > 0)) return fail(FDTD_ERR_INVALID, "SetMesh: spacings must be positive");
+  for (int a = 0; a < 3; ++a) {
+    if (d[a]) s->mesh[a].assign(d[a], d[a] + Q[a]);
+    else s->mesh[a].clear();
+  }
+  s->args_gen++;
   return 0;
 }
 

@@ -451,6 +451,7 @@ struct StepArgs {
   int spl;            // split edge segment (1 column per lane, I a multiple of 64; see rows_body)
   int ntf;            // no TF/SF work in the slab part (the box lies clear of its rows and planes)
   int glob;           // 64-bit global addressing although every array fits a buffer resource
+  int gr;             // graded mesh: every row and plane applies the per-axis table factors
   size_t IJK, sx, sy, sz;
   size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
   const float *Fc;
@@ -738,7 +739,10 @@ constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
 // WB: write-back stores (a state that fits in the Infinity Cache), else non-temporal.
 // NTF: the slab part of a grid whose TF/SF box lies clear of the slab rows and planes (StepArgs
 // ntf): the march carries no TF/SF code (256^3 slab part 74.1 -> 70.0 us).
-template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false, bool NTF = false>
+// GR: a graded mesh.  The per-axis tables carry dx / spacing on every index (1 only in the CPML
+// cells' neighbourhood where the spacing is dx), so every wave runs the slab march, which applies
+// the y and z factors per row and plane, and every lane holds its column's x factors.
+template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false, bool NTF = false, bool GR = false>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
   // it is loaded with scalar loads into scalar registers (also where the block's address was
@@ -910,8 +914,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       sl[c] = EDGE && on ? (i < P ? i : (i >= I - P ? i - (I - P2) : -1)) : -1;
       const int s = sl[c] < 0 ? 0 : sl[c];
       bE[c] = sSlab[0][s]; cE[c] = sSlab[1][s]; bH[c] = sSlab[2][s]; cH[c] = sSlab[3][s];
-      kE[c] = sl[c] >= 0 ? sSlab[4][s] : 1.0f;
-      kH[c] = sl[c] >= 0 ? sSlab[5][s] : 1.0f;
+      kE[c] = sl[c] >= 0 ? sSlab[4][s] : (GR && on ? prmc[ikE0 + i] : 1.0f);
+      kH[c] = sl[c] >= 0 ? sSlab[5][s] : (GR && on ? prmc[ikH0 + i] : 1.0f);
       const unsigned b = (unsigned)FS * c;
       if (sl[c] >= 0) lf |= 1u << F_SLAB;
       if (sl[c] >= 0 && sl[c] < P2 - 1) lf |= 1u << (b + F_SLH);
@@ -1515,7 +1519,11 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     }
   };
 
-  if constexpr (PART == 0) {
+  if constexpr (GR) {
+    (void)plain;
+    (void)slabw;
+    march(std::integral_constant<int, 2>{});
+  } else if constexpr (PART == 0) {
     if (uni((int)plain)) {
       march(std::integral_constant<int, 0>{});
     } else {
@@ -1607,6 +1615,18 @@ __global__ void __launch_bounds__(kRowsMaxThreads[CPL]) k_rows(const StepArgs *_
     return;
   }
   rows_body<IDX, BUF, CPL, PART, MU, WB, NTF>(pa, xcd_remap(b, nwg));
+}
+// Graded mesh: both parts with one column per lane, non-temporal stores, no TF/SF code (a graded
+// mesh takes no plane wave); the slab part's launch carries the tail workgroups as k_rows's does.
+template <bool IDX, bool BUF, int PART>
+__global__ void __launch_bounds__(kRowsMaxThreads[1]) k_rows_gr(const StepArgs *__restrict__ pa, int n, int nwg,
+                                                                const TailArgs ta) {
+  const int b = blockIdx.x;
+  if (PART == 1 && b >= nwg) {
+    tail_body<IDX>(ta, b - nwg);
+    return;
+  }
+  rows_body<IDX, BUF, 1, PART, false, false, true, true>(pa, xcd_remap(b, nwg));
 }
 // The interior part with one column per lane, held to 64 registers so that 8 waves share each SIMD
 // (measured 4% faster at 256^3 than the 68 the compiler picks freely).
@@ -1919,9 +1939,15 @@ static unsigned rows_lds(const StepArgs &a, int part) {
 
 template <bool IDX, bool BUF, int PART, bool MU>
 static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta, bool wb, bool ntf) {
+                          int n, const TailArgs &ta, bool wb, bool ntf, bool gr) {
   const dim3 grid(nwg + ntail);
   if (grid.x == 0) return;
+  if constexpr (!MU) {
+    if (gr) {  // make_step_args gives a graded mesh one column per lane and no per-cell mu
+      hipLaunchKernelGGL((k_rows_gr<IDX, BUF, PART>), grid, blk, lds, st, d, n, nwg, ta);
+      return;
+    }
+  }
   if constexpr (BUF && !MU) {  // write-back variants: one column per lane (grids that fit the cache)
     if (wb && cpl == 1) {
       if constexpr (PART == 0) hipLaunchKernelGGL((k_rows_c1<IDX, BUF, true>), grid, blk, lds, st, d, n);
@@ -1946,9 +1972,9 @@ static void launch_rows_m(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3
 // mu: the per-cell H coefficients of CreateMu are in use
 template <bool IDX, bool BUF, int PART>
 static void launch_rows_w(hipStream_t st, int cpl, int nwg, unsigned ntail, dim3 blk, unsigned lds, const StepArgs *d,
-                          int n, const TailArgs &ta, bool mu, bool wb, bool ntf) {
-  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf);
-  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf);
+                          int n, const TailArgs &ta, bool mu, bool wb, bool ntf, bool gr) {
+  if (mu) launch_rows_m<IDX, BUF, PART, true>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf, false);
+  else launch_rows_m<IDX, BUF, PART, false>(st, cpl, nwg, ntail, blk, lds, d, n, ta, wb, ntf, gr);
 }
 
 // part: 0 the interior part, 1 the slab part (with the tail workgroups of ta, if any), -1 both
@@ -1964,7 +1990,7 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   // the state (two field sets, two psi sets, materials) fits in the 256 MB Infinity Cache
   const size_t state = 2 * 24 * a.cs + 2 * 16 * (a.sx + a.sy + a.sz) + (a.mat ? a.IJK : 24 * a.IJK);
   const bool wb = state <= ((size_t)256 << 20);
-  const bool ntf = a.ntf != 0;
+  const bool ntf = a.ntf != 0, gr = a.gr != 0;
   const int n0 = part == 1 ? 0 : a.nwg0, n1 = part == 0 ? 0 : a.nwg1;
   TailArgs ta{};
   if (tail && part != 0) ta = *tail;
@@ -1972,11 +1998,11 @@ static hipError_t launch_rows(hipStream_t st, const StepArgs &a, const StepArgs 
   const dim3 b0(64 * a.S * a.R0), b1(64 * a.S * a.R);
   const unsigned L0 = rows_lds(a, 0), L1 = rows_lds(a, 1);
   if (a.mat) {
-    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
-    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
+    if (buf) { launch_rows_w<true, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf, gr); launch_rows_w<true, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf, gr); }
+    else { launch_rows_w<true, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf, gr); launch_rows_w<true, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf, gr); }
   } else {
-    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
-    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf); }
+    if (buf) { launch_rows_w<false, true, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf, gr); launch_rows_w<false, true, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf, gr); }
+    else { launch_rows_w<false, false, 0>(st, a.CPL, n0, 0, b0, L0, dargs, n, ta, mu, wb, ntf, gr); launch_rows_w<false, false, 1>(st, a.CPL, n1, nt, b1, L1, dargs, n, ta, mu, wb, ntf, gr); }
   }
   return hipGetLastError();
 }
@@ -1991,7 +2017,10 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   if (!fused_supported(g)) return -1;
   if (f.mat && f.ntab > 256) return -1;
   int cpl = 1, S = 1;
-  if (!row_layout(g.I, tune.cpl, cpl, S)) return -1;
+  // a graded mesh: one column per lane (its x factors are per lane) and no per-cell mu (the
+  // per-op operations step such a grid)
+  if (f.gr && f.Chmu) return -1;
+  if (!row_layout(g.I, f.gr ? 1 : tune.cpl, cpl, S)) return -1;
   // the two x faces of the TF/SF box must not share a lane (at most one face column per lane)
   if ((f.act_h | f.act_e) && g.i1 - g.i0 < cpl) {
     if (tune.cpl || !row_layout(g.I, 1, cpl, S)) return -1;
@@ -2043,6 +2072,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   // j >= J-P; a row's H of row j-1 included) or planes (k < P, k >= K-P, and the plane K-P-1 below
   // the upper z-slab chunks): faces on the rows j0-1..j1+1 and the planes k0-1..k1+1.
   a.glob = f.force_global ? 1 : 0;
+  a.gr = f.gr ? 1 : 0;
   a.ntf = !(f.act_h | f.act_e) ||
           (g.j0 >= g.P + 2 && g.j1 <= g.J - g.P - 2 && g.k0 >= g.P + 2 && g.k1 <= g.K - g.P - 2);
   a.KCS = kcs; a.nz1 = nz1; a.nz2 = nz2;

@@ -88,6 +88,7 @@ struct FusedDev {
   float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
   float Chh = 1, Che = 0, dx = 0;
   const float *Chmu = nullptr;  // per-cell H coefficients of CreateMu [3][I*J*K] (K2), or null
+  bool gr = false;              // graded mesh: the per-axis tables carry dx / spacing in every cell
 };
 // Argument block of the fused kernel: built on the host (make_step_args returns the workgroup
 // count, -1 if unsupported), uploaded once per (state parity, plane-wave mode), then launched.

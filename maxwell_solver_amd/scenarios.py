@@ -102,6 +102,7 @@ class Scenario:
     snap_comp: int = 3
     seed: int = 0
     currents: tuple | None = None        # step currents (pol, box[6]): AddCurrents after every step
+    mesh: tuple | None = None            # graded mesh: spacings along x, y, z (None: uniform dx)
 
     @property
     def cells(self) -> int:
@@ -189,6 +190,34 @@ def config2(T: int = 2000, N: int = 256, n_spheres: int = 40, n_probes: int = 64
                     gauss_sine(T, dt, f0), mat_index=idx, mat_table=tab,
                     probes=_probes(rng, n_probes, N, N, N, P), probe_comps=(3, 5), snap_every=250,
                     snap_comp=3, seed=seed)
+
+
+def graded_spacings(Q: int, P: int, dx: float, amp: float, phase: float = 0.0) -> np.ndarray:
+    """Spacings of an axis of Q grid planes for fdtd_set_mesh: dx in the CPML cells (n <= P and
+    n >= Q-P-2), between them a smooth grading from dx up to (1 + amp) dx (no cell finer than dx,
+    so that the time step chosen for dx stays within the Courant limit)."""
+    d = np.full(Q, np.float32(dx), np.float32)
+    n = np.arange(P + 1, Q - P - 2)
+    t = (n - (P + 1)) / max(1, Q - 2 * P - 4)
+    w = np.abs(np.sin(math.pi * t + phase) * np.sin(3.1 * math.pi * t))
+    d[P + 1:Q - P - 2] = (dx * (1 + amp * w)).astype(np.float32)
+    return d
+
+
+def config_graded(T: int = 2000, N: int = 256, graded: bool = True) -> Scenario:
+    """The 256^3 sphere grid of config2 on a graded mesh (spacings from dx to 1.3 dx along every
+    axis, dx in the CPML), driven by an x-directed line current through the centre (AddCurrents
+    after every step, a Gaussian-modulated 3 GHz pulse) instead of a plane wave, which a graded
+    mesh does not take; 64 probes (Ez, Hy).  graded=False: the same grid and source, uniform."""
+    base = config2(T, N)
+    P = base.cpml[0]
+    mesh = ((graded_spacings(N, P, base.dx, 0.3), graded_spacings(N, P, base.dx, 0.25, 0.5),
+             graded_spacings(N, P, base.dx, 0.2, 1.0)) if graded else None)
+    c = N // 2
+    return Scenario("graded" if graded else "graded-uniform", N, N, N, T, base.dx, base.dt, base.fctr, base.cpml,
+                    base.inset, 0, (0.0,), gauss_sine(T, base.dt, base.fctr), mat_index=base.mat_index,
+                    mat_table=base.mat_table, probes=base.probes, probe_comps=(3, 5), seed=base.seed,
+                    currents=(1, (P + 2, N - P - 3, c, c, c, c)), mesh=mesh)
 
 
 def config3(T: int = 4000, I: int = 384, J: int = 384, K: int = 256, lattice: int = 16) -> Scenario:
@@ -434,6 +463,8 @@ def setup_host(backend, sc: Scenario, use_arrays: bool = False):
                 backend.update_material(f + 1, arrs[f])
         else:
             backend.set_material_index(sc.mat_index, sc.mat_table)
+    if sc.mesh is not None:
+        backend.set_mesh(*sc.mesh)
     if sc.mode == 5 or (sc.mode in (1, 2, 3, 4, 6, 7) and sc.pw_params and sc.pw_params[0]):
         backend.set_plane_wave(sc.mode, sc.pw_params)
 
@@ -457,6 +488,8 @@ def setup(backend, sc: Scenario, use_arrays: bool = False, init_line: bool = Tru
                 backend.update_material(f + 1, arrs[f])
         else:
             backend.set_material_index(sc.mat_index, sc.mat_table)
+    if sc.mesh is not None:
+        backend.set_mesh(*sc.mesh)
     if sc.mode == 5 or (sc.mode in (1, 2, 3, 4, 6, 7) and sc.pw_params and sc.pw_params[0]):
         backend.set_plane_wave(sc.mode, sc.pw_params)
     if mu is not None:

@@ -33,6 +33,7 @@ EXPORTS = [
     "fdtd_server_collect", "fdtd_server_release", "fdtd_server_request_bytes", "fdtd_server_stats",
     "fdtd_server_request_error", "fdtd_device_bytes", "fdtd_run_noalloc",
     "fdtd_stream_ceiling", "fdtd_get_tuning", "fdtd_set_autotune", "fdtd_set_slab_chunk", "fdtd_set_addressing",
+    "fdtd_set_mesh",
 ]
 
 
@@ -90,6 +91,7 @@ def lib():
             "fdtd_get_tuning": [vp, ip],
             "fdtd_set_autotune": [vp, i32],
             "fdtd_set_slab_chunk": [vp, i32], "fdtd_set_addressing": [vp, i32],
+            "fdtd_set_mesh": [vp, fp, fp, fp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -161,6 +163,21 @@ class Simulation:
     def set_cpml(self, params):
         p = _f64(params)
         self._chk(lib().fdtd_set_cpml(self._h, _p(p, C.c_double)))
+
+    def set_mesh(self, dx=None, dy=None, dz=None):
+        """Graded mesh: distances between consecutive grid planes along x, y, z (None: uniform)."""
+        ptrs = []
+        self._mesh = []
+        for q, d in enumerate((dx, dy, dz)):
+            if d is None:
+                ptrs.append(None)
+                continue
+            a = np.ascontiguousarray(d, dtype=np.float32)
+            if self.dims is None or a.size != self.dims[q]:
+                raise ValueError("set_mesh: one spacing per grid plane of the axis (after set_const)")
+            self._mesh.append(a)
+            ptrs.append(_p(a, C.c_float))
+        self._chk(lib().fdtd_set_mesh(self._h, *ptrs))
 
     def set_tfsf(self, inset):
         self._chk(lib().fdtd_set_tfsf(self._h, int(inset)))

@@ -14,7 +14,7 @@
  *
  * Every function cites the reference file:line it restates.  Extensions that the reference lacks
  * (runtime CPML parameters with per-axis 1/kappa, a TF/SF inset, oblique and layered plane waves,
- * probes, snapshots) reduce to the reference arithmetic when set to the reference preset
+ * probes, snapshots, a graded mesh) reduce to the reference arithmetic when set to the reference preset
  * (P=10, m=3, kappa_max=1, alpha_max=0.08, alpha_m=1, inset 15, vacuum 1D line).
  *
  * Parity pins (checked in tests/test_oracle_*.py): closed-form CPML profiles and coefficients,
@@ -75,7 +75,9 @@ struct Sim : Base {
   // ---- readySimulation (src/simulation.h:640-979) ----
   std::vector<R> bE, cE, bH, cH;          // [2P], shared by all axes and the 1D line
   std::vector<R> kapE, kapH;              // kappa profiles [P] (profile index)
-  std::vector<R> ikE[3], ikH[3];          // per-axis 1/kappa along x, y, z
+  std::vector<R> ikE[3], ikH[3];          // per-axis 1/kappa along x, y, z (times dx / spacing on a graded mesh)
+  // graded mesh (extension): cell sizes along x, y, z as given (empty: uniform dx along that axis)
+  std::vector<double> mesh[3];
   std::vector<R> Ce[3], Ch[3];            // Cexe/Ceye/Ceze, Cexh/Ceyh/Cezh per cell
   std::vector<R> den_ex;                  // [I] (den_ex depends on i only)
   R Chh = 1, Che = 0;
@@ -106,6 +108,59 @@ struct Sim : Base {
   R polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
 
   // =====================================================================================
+  // Graded (tensor-product) mesh, the build's use of the reference's non-uniform-mesh
+  // scaffolding (FDTD_UNIFORM_MESH and den_ey/den_ez/den_hy/den_hz, src/simulation.h:2, 679-685,
+  // 718-724, allocated and zeroed but never read there): d[a][n] is the distance between the
+  // grid planes n and n+1 along axis a (the last entry, a CPML cell, equals dx).  The H update differences
+  // E across the primal cell n (spacing d[n]) and the E update differences H across the dual
+  // cell n (spacing (d[n-1] + d[n]) / 2); both enter as factors dx / spacing on the per-axis
+  // tables (the reference's den_h* = 1/(dx kappa_H), den_e* = 1/(dx kappa_E), :861-912), so
+  // Che and Ch keep their 1/dx.
+  int set_mesh(const float *const d[3]) {
+    if (!init_const) { g_err = "simFDTD:InvalidInput|SetMesh before SetConst"; return 2; }
+    if (init_ready) { g_err = "simFDTD:InvalidInput|SetMesh after ReadySimulation"; return 2; }
+    const size_t Q[3] = {I, J, K};
+    for (int a = 0; a < 3; ++a) {
+      mesh[a].clear();
+      if (!d[a]) continue;
+      for (size_t n = 0; n < Q[a]; ++n) {
+        if (!std::isfinite(d[a][n]) || !(d[a][n] > 0)) { g_err = "simFDTD:InvalidInput|SetMesh: spacings must be positive"; return 1; }
+        mesh[a].push_back((double)d[a][n]);
+      }
+    }
+    return 0;
+  }
+  bool graded() const { return !mesh[0].empty() || !mesh[1].empty() || !mesh[2].empty(); }
+  // The CPML layers keep the uniform spacing dx (cells n <= P and n >= Q-P-2), so that the psi
+  // updates, which difference over dx, and every CPML table are those of the uniform grid.
+  int grade_tables() {
+    const size_t Q[3] = {I, J, K};
+    const double dx = (double)delX;
+    // Courant limit of the finest cells: dt <= 1 / (c sqrt(sum_a 1 / min_n d[a][n]^2))
+    double inv2 = 0;
+    for (int a = 0; a < 3; ++a) {
+      double m = dx;
+      for (double v : mesh[a]) m = std::min(m, v);
+      inv2 += 1.0 / (m * m);
+    }
+    if ((double)delT > 1.0 / (kC0 * std::sqrt(inv2))) {
+      g_err = "simFDTD:InvalidInput|SetMesh: dt exceeds the Courant limit of the finest cells";
+      return 1;
+    }
+    for (int a = 0; a < 3; ++a) {
+      const std::vector<double> &d = mesh[a];
+      if (d.empty()) continue;
+      for (size_t n = 0; n < Q[a]; ++n)
+        if ((n <= P || n + P + 2 >= Q[a]) && d[n] != dx) {
+          g_err = "simFDTD:InvalidInput|SetMesh: the CPML cells (P+1 at each end) must keep the spacing dx";
+          return 1;
+        }
+      for (size_t n = 0; n + 1 < Q[a]; ++n) ikH[a][n] = (R)((double)ikH[a][n] * (dx / d[n]));
+      for (size_t n = 1; n < Q[a]; ++n) ikE[a][n] = (R)((double)ikE[a][n] * (dx / (0.5 * (d[n - 1] + d[n]))));
+    }
+    return 0;
+  }
+
   int set_const(const float *freqRes, const int32_t *timeDim) {
     if (init_const) { g_err = "simFDTD:AlreadyInitialized|SetConst: already called."; return 3; }
     fctr = freqRes[0];
@@ -328,6 +383,8 @@ struct Sim : Base {
     for (size_t i = 0; i + 1 < I; i++) den_ex[i] = (R)(1.0 / (delX * (R)1.0));
     ikE[0] = inv_kappa_E(I); ikE[1] = inv_kappa_E(J); ikE[2] = inv_kappa_E(K);
     ikH[0] = inv_kappa_H(I); ikH[1] = inv_kappa_H(J); ikH[2] = inv_kappa_H(K);
+    if (graded())
+      if (int rc = grade_tables()) return rc;
     // src/simulation.h:914-922 (sigma_m = 0)
     R sigma_m = 0;
     Chh = (1 - sigma_m * delT / 2 / kMu0) / (1 + sigma_m * delT / 2 / kMu0);
@@ -726,6 +783,7 @@ struct Sim : Base {
 
   int update_h_pw(int n, int mode) {
     if (mode < 1 || mode > 7) return 0;
+    if (graded()) { g_err = "simFDTD:InvalidInput|a graded mesh takes no plane wave (the 1D incident line is uniform)"; return 1; }
     if (!have_1d) { g_err = "simFDTD:InvalidInput|plane wave before Init1D"; return 2; }
     if (n < 0 || (size_t)n >= T) { g_err = "simFDTD:InvalidInput|time index outside [0,T)"; return 1; }
     if (mode == 5) return oblique_h((size_t)n);
@@ -825,6 +883,7 @@ struct Sim : Base {
 
   int update_e_pw(int n, int mode) {
     if (mode < 1 || mode > 7) return 0;
+    if (graded()) { g_err = "simFDTD:InvalidInput|a graded mesh takes no plane wave (the 1D incident line is uniform)"; return 1; }
     if (!have_1d) { g_err = "simFDTD:InvalidInput|plane wave before Init1D"; return 2; }
     if (n < 0 || (size_t)n >= T) { g_err = "simFDTD:InvalidInput|time index outside [0,T)"; return 1; }
     if (mode == 5) return oblique_e((size_t)n);
@@ -1141,6 +1200,10 @@ int orc_set_tfsf(void *h, int32_t width) {
   DISPATCH(h, (s->init_ready ? 2 : (s->width = (size_t)width, 0)));
 }
 int orc_set_source(void *h, const float *src, int update) { DISPATCH(h, s->set_source(src, update)); }
+int orc_set_mesh(void *h, const float *dx, const float *dy, const float *dz) {
+  const float *const d[3] = {dx, dy, dz};
+  DISPATCH(h, s->set_mesh(d));
+}
 int orc_set_antenna(void *h, const int32_t *loc) { DISPATCH(h, s->set_antenna(loc)); }
 int orc_set_probe_comps(void *h, const int32_t *c, int n) { DISPATCH(h, s->set_probe_comps(c, n)); }
 int orc_update_material(void *h, int32_t flag, const float *v) { DISPATCH(h, s->update_material(flag, v)); }

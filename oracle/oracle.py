@@ -79,6 +79,7 @@ def lib():
             "orc_set_cpml": [vp, f64p],
             "orc_set_tfsf": [vp, i32],
             "orc_set_source": [vp, f32p, C.c_int],
+            "orc_set_mesh": [vp, f32p, f32p, f32p],
             "orc_set_antenna": [vp, i32p],
             "orc_set_probe_comps": [vp, i32p, C.c_int],
             "orc_update_material": [vp, i32, f32p],
@@ -173,6 +174,21 @@ class OracleSim:
 
     def set_tfsf(self, inset):
         self._chk(lib().orc_set_tfsf(self._h, int(inset)))
+
+    def set_mesh(self, dx=None, dy=None, dz=None):
+        """Graded mesh: cell sizes along x, y, z (None: uniform dx along that axis)."""
+        ptrs = []
+        self._mesh = []
+        for q, d in enumerate((dx, dy, dz)):
+            if d is None:
+                ptrs.append(None)
+            else:
+                a = np.ascontiguousarray(d, dtype=np.float32)
+                if self.dims is None or a.size != self.dims[q]:
+                    raise ValueError("set_mesh: one spacing per grid plane of the axis (after set_const)")
+                self._mesh.append(a)
+                ptrs.append(_p(a, C.c_float))
+        self._chk(lib().orc_set_mesh(self._h, *ptrs))
 
     def set_source(self, src, update=False):
         s = _f32(src)
