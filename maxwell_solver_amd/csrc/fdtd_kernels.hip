@@ -739,9 +739,9 @@ constexpr int kRowsMaxThreads[5] = {0, 1024, 512, 0, 256};
 // WB: write-back stores (a state that fits in the Infinity Cache), else non-temporal.
 // NTF: the slab part of a grid whose TF/SF box lies clear of the slab rows and planes (StepArgs
 // ntf): the march carries no TF/SF code (256^3 slab part 74.1 -> 70.0 us).
-// GR: a graded mesh.  The per-axis tables carry dx / spacing on every index (1 only in the CPML
-// cells' neighbourhood where the spacing is dx), so every wave runs the slab march, which applies
-// the y and z factors per row and plane, and every lane holds its column's x factors.
+// GR: a graded mesh.  The per-axis tables carry dx / spacing on every index (1/kappa in the CPML
+// cells, where the spacing is dx), so every march applies the y and z factors per row and plane
+// and the x factor of each lane's column, which the uniform grid's marches omit where they are 1.
 template <bool IDX, bool BUF, int CPL, int PART, bool MU = false, bool WB = false, bool NTF = false, bool GR = false>
 __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, int L) {
   // The argument block does not change during a launch: read through the constant address space
@@ -888,10 +888,13 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   auto march = [&](auto kind_tag) {
     constexpr int KIND = decltype(kind_tag)::value;
     constexpr bool EDGE = KIND >= 1, SLAB = KIND >= 2, YC = KIND != 3;
+    // factors of the per-axis tables: applied where 1/kappa may differ from 1 (x per lane in edge
+    // waves, y per row and z per plane in slab waves) and, on a graded mesh, everywhere
+    constexpr bool XF = EDGE || GR, YF = (SLAB && YC) || GR;
     const bool ys = YC && ys0, ysm = YC && ysm0;
     const bool lon = !EDGE || on;
-    const float kyH = SLAB && YC ? prmc[ikH0 + I + j] : 1.0f, kyE = SLAB && YC ? prmc[ikE0 + I + j] : 1.0f;
-    const float kyHm = SLAB && YC && has_jm ? prmc[ikH0 + I + j - 1] : 1.0f;
+    const float kyH = YF ? prmc[ikH0 + I + j] : 1.0f, kyE = YF ? prmc[ikE0 + I + j] : 1.0f;
+    const float kyHm = YF && has_jm ? prmc[ikH0 + I + j - 1] : 1.0f;
     const int syc = ys ? sy : 0, symc = ysm ? sym : 0;
     const float bHy = sSlab[2][syc], cHy = sSlab[3][syc], bEy = sSlab[0][syc], cEy = sSlab[1][syc];
     const float bHym = sSlab[2][symc], cHym = sSlab[3][symc];
@@ -1084,7 +1087,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
       constexpr bool ZC = SLAB || !own;
       const bool zs = ZC && (k < P || k >= K - P);
       const int sz = zs ? (k < P ? k : k - (K - P2)) : 0;
-      const float kzH = ZC ? prmc[ikH0 + I + J + k] : 1.0f, kzE = SLAB ? prmc[ikE0 + I + J + k] : 1.0f;
+      const float kzH = ZC || GR ? prmc[ikH0 + I + J + k] : 1.0f, kzE = SLAB || GR ? prmc[ikE0 + I + J + k] : 1.0f;
       const float bHz = sSlab[2][sz], cHz = sSlab[3][sz], bEz = sSlab[0][sz], cEz = sSlab[1][sz];
       // uniform parts of the update masks (slab waves reach the grid's first and last rows / planes)
       const bool h_jk = !SLAB || (j < J - 1 && k < K - 1);
@@ -1097,8 +1100,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         const float ez1 = c + 1 < CPL ? ez[c + 1 < CPL ? c + 1 : 0] : ez_n;
         hx[c] = Lp.hx[c]; hy[c] = Lp.hy[c]; hz[c] = Lp.hz[c];
         if (h_jk && (!EDGE || bit(f, FS * c + F_H))) {
-          const float dyz = EDGE ? (ez1 - ez[c]) * kH[c] : ez1 - ez[c];
-          const float dzy = EDGE ? (ey1 - ey[c]) * kH[c] : ey1 - ey[c];
+          const float dyz = XF ? (ez1 - ez[c]) * kH[c] : ez1 - ez[c];
+          const float dzy = XF ? (ey1 - ey[c]) * kH[c] : ey1 - ey[c];
           float cx = Che, cy = Che, cz = Che;
           if constexpr (MU) { cx = Lp.mux[c]; cy = Lp.muy[c]; cz = Lp.muz[c]; }
           hx[c] = Chh * hx[c] + cx * (-(Lp.ezq[c] - ez[c]) * kyH + (Lp.ny[c] - ey[c]) * kzH);
@@ -1211,7 +1214,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
         for (int c = 0; c < CPL; ++c) {
           const float eym1 = c + 1 < CPL ? eym[c + 1 < CPL ? c + 1 : 0] : eym_n;
           if (hm_k && (!EDGE || bit(f, FS * c + F_H))) {
-            const float dzy = EDGE ? (eym1 - eym[c]) * kH[c] : eym1 - eym[c];
+            const float dzy = XF ? (eym1 - eym[c]) * kH[c] : eym1 - eym[c];
             float cx = Che, cz = Che;
             if constexpr (MU) { cx = Lp.muxm[c]; cz = Lp.muzm[c]; }
             hxm[c] = Chh * hxm[c] + cx * (-(ez[c] - Lp.ezm[c]) * kyHm + (Lp.nym[c] - eym[c]) * kzH);
@@ -1347,8 +1350,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
           if (!SLAB || (j != 0 && k != 0))
             ux[c] = ce[0][c] * ex[c] + ch[0][c] * ((hz[c] - hzm[c]) * kyE - (hy[c] - hpy[c]) * kzE);
           if (!EDGE || bit(f, FS * c + F_E)) {
-            const float dzx = EDGE ? (hz[c] - hzm1) * kE[c] : hz[c] - hzm1;
-            const float dyx = EDGE ? (hy[c] - hym1) * kE[c] : hy[c] - hym1;
+            const float dzx = XF ? (hz[c] - hzm1) * kE[c] : hz[c] - hzm1;
+            const float dyx = XF ? (hy[c] - hym1) * kE[c] : hy[c] - hym1;
             if (!SLAB || k != 0) uy[c] = ce[1][c] * ey[c] + ch[1][c] * ((hx[c] - hpx[c]) * kzE - dzx);
             if (!SLAB || j != 0) uz[c] = ce[2][c] * ez[c] + ch[2][c] * (dyx - (hx[c] - hxm[c]) * kyE);
           }
@@ -1519,11 +1522,7 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
     }
   };
 
-  if constexpr (GR) {
-    (void)plain;
-    (void)slabw;
-    march(std::integral_constant<int, 2>{});
-  } else if constexpr (PART == 0) {
+  if constexpr (PART == 0) {
     if (uni((int)plain)) {
       march(std::integral_constant<int, 0>{});
     } else {
