@@ -6,7 +6,7 @@
  * the same error identifiers.  Build against MATLAB's mex.h:
  *     mex -I../../include simFDTD_gateway.c -L.. -lfdtd_hip
  * Extension ops (no counterpart in the reference): SetCPML, SetTFSF, SetPlaneWave,
- * SetMaterialIndex, SetProbeComponents, Step, Run, OutputSnapshots.
+ * SetMaterialIndex, SetProbeComponents, SetMesh, Step, Run, OutputSnapshots.
  */
 #include <ctype.h>
 #include <stdint.h>
@@ -116,6 +116,12 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     need(nrhs, 4, "index and table required.");
     check(fdtd_set_material_index(s, (const uint8_t *)mxGetData(prhs[2]), (const float *)mxGetData(prhs[3]),
                                   (int32_t)(mxGetNumberOfElements(prhs[3]) / 6)));
+  } else if (strcmp(op, "setmesh") == 0) {
+    /* single spacings along x, y, z (I, J, K entries), an empty array for a uniform axis */
+    need(nrhs, 5, "spacings along x, y and z required (empty: uniform).");
+    const float *d[3];
+    for (int a = 0; a < 3; ++a) d[a] = mxGetNumberOfElements(prhs[2 + a]) ? (const float *)mxGetData(prhs[2 + a]) : NULL;
+    check(fdtd_set_mesh(s, d[0], d[1], d[2]));
   } else if (strcmp(op, "setprobecomponents") == 0) {
     need(nrhs, 3, "components required.");
     check(fdtd_set_probe_components(s, (const int32_t *)mxGetData(prhs[2]), (int32_t)mxGetNumberOfElements(prhs[2])));

@@ -71,6 +71,24 @@ int main(int argc, char **argv) {
     bad |= expect_err("setconst twice", op("setconst", h, 2, a, NULL), "simFDTD:AlreadyInitialized");
     bad |= expect_err("update before ready", op("UpdateE_All", h, 0, NULL, NULL), "simFDTD:InvalidInput");
     {
+      /* SetMesh (graded mesh extension): arity, a non-positive spacing, then a valid mesh and a
+       * uniform one (empty arrays) */
+      bad |= expect_err("setmesh arity", op("SetMesh", h, 0, NULL, NULL), "simFDTD:InvalidInput");
+      float d[40];
+      for (int q = 0; q < 40; ++q) d[q] = 0.01f;
+      mxArray *dx = f32s(40, d), *dy = f32s(40, d), *none = vec(mxSINGLE_CLASS, 0);
+      d[20] = 0.0f;
+      mxArray *dz_bad = f32s(40, d);
+      mxArray *m_bad[3] = {dx, dy, dz_bad};
+      bad |= expect_err("setmesh zero spacing", op("SetMesh", h, 3, m_bad, NULL), "simFDTD:InvalidInput");
+      mxArray *m_ok[3] = {dx, none, dy};
+      if (op("SetMesh", h, 3, m_ok, NULL)) { printf("FAIL setmesh %s\n", g_err_msg); bad = 1; }
+      else printf("ok setmesh\n");
+      mxArray *m_uni[3] = {none, none, none};
+      if (op("SetMesh", h, 3, m_uni, NULL)) { printf("FAIL setmesh uniform %s\n", g_err_msg); bad = 1; }
+      else printf("ok setmesh uniform\n");
+    }
+    {
       /* OutputAnt without antennas: the reference prints a message and returns the zero-filled
        * array (src/simulation.h:634-637) */
       mxArray *sz = vec(mxUINT32_CLASS, 1), *out = NULL;
