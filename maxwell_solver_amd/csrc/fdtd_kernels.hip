@@ -1123,15 +1123,19 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const int i = ib + c;
-          if (bit(f, F_ON) && i >= bi0 && i <= bi1) {
-            if ((ah & (1 << 3)) && k <= bk1 - 1) {
-              const float v = Che * ldo(yr, (uint32_t)i * 4u);
-              hx[c] = lo ? hx[c] + v : hx[c] - v;
-            }
-            if ((ah & (1 << 5)) && i <= bi1 - 1) {
-              const float v = Che * ldo(yr + yS, (uint32_t)i * 4u);
-              hz[c] = lo ? hz[c] - v : hz[c] + v;
-            }
+          // both sheet values are loaded before either is used: one wait for the pair
+          const bool in = bit(f, F_ON) && i >= bi0 && i <= bi1;
+          const bool ua = in && (ah & (1 << 3)) && k <= bk1 - 1, ub = in && (ah & (1 << 5)) && i <= bi1 - 1;
+          float va = 0.0f, vb = 0.0f;
+          if (ua) va = ldo(yr, (uint32_t)i * 4u);
+          if (ub) vb = ldo(yr + yS, (uint32_t)i * 4u);
+          if (ua) {
+            const float v = Che * va;
+            hx[c] = lo ? hx[c] + v : hx[c] - v;
+          }
+          if (ub) {
+            const float v = Che * vb;
+            hz[c] = lo ? hz[c] - v : hz[c] + v;
           }
         }
       }
@@ -1147,15 +1151,18 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
           const int i = ib + c;
-          if (lon && i >= bi0 && i <= bi1) {
-            if ((ah & (1 << 6)) && j <= bj1 - 1) {
-              const float v = Che * ldo(zr, (uint32_t)i * 4u);
-              hx[c] = lo ? hx[c] - v : hx[c] + v;
-            }
-            if ((ah & (1 << 7)) && i <= bi1 - 1) {
-              const float v = Che * ldo(zr + zS, (uint32_t)i * 4u);
-              hy[c] = lo ? hy[c] + v : hy[c] - v;
-            }
+          const bool in = lon && i >= bi0 && i <= bi1;
+          const bool ua = in && (ah & (1 << 6)) && j <= bj1 - 1, ub = in && (ah & (1 << 7)) && i <= bi1 - 1;
+          float va = 0.0f, vb = 0.0f;
+          if (ua) va = ldo(zr, (uint32_t)i * 4u);
+          if (ub) vb = ldo(zr + zS, (uint32_t)i * 4u);
+          if (ua) {
+            const float v = Che * va;
+            hx[c] = lo ? hx[c] - v : hx[c] + v;
+          }
+          if (ub) {
+            const float v = Che * vb;
+            hy[c] = lo ? hy[c] + v : hy[c] - v;
           }
         }
       }
@@ -1231,15 +1238,18 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             const int i = ib + c;
-            if (bit(f, F_ON) && i >= bi0 && i <= bi1) {
-              if ((ah & (1 << 3)) && k <= bk1 - 1) {
-                const float v = Che * ldo(yr, (uint32_t)i * 4u);
-                hxm[c] = lo ? hxm[c] + v : hxm[c] - v;
-              }
-              if ((ah & (1 << 5)) && i <= bi1 - 1) {
-                const float v = Che * ldo(yr + yS, (uint32_t)i * 4u);
-                hzm[c] = lo ? hzm[c] - v : hzm[c] + v;
-              }
+            const bool in = bit(f, F_ON) && i >= bi0 && i <= bi1;
+            const bool ua = in && (ah & (1 << 3)) && k <= bk1 - 1, ub = in && (ah & (1 << 5)) && i <= bi1 - 1;
+            float va = 0.0f, vb = 0.0f;
+            if (ua) va = ldo(yr, (uint32_t)i * 4u);
+            if (ub) vb = ldo(yr + yS, (uint32_t)i * 4u);
+            if (ua) {
+              const float v = Che * va;
+              hxm[c] = lo ? hxm[c] + v : hxm[c] - v;
+            }
+            if (ub) {
+              const float v = Che * vb;
+              hzm[c] = lo ? hzm[c] - v : hzm[c] + v;
             }
           }
         }
@@ -1370,15 +1380,18 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             const int i = ib + c;
-            if (i >= bi0 && i <= bi1) {
-              if ((ae & (1 << 3)) && i <= bi1 - 1) {
-                const float v = ch[0][c] * ldo(yr, (uint32_t)i * 4u);
-                ux[c] = lo ? ux[c] - v : ux[c] + v;
-              }
-              if ((ae & (1 << 5)) && k <= bk1 - 1) {
-                const float v = ch[2][c] * ldo(yr + yS, (uint32_t)i * 4u);
-                uz[c] = lo ? uz[c] + v : uz[c] - v;
-              }
+            const bool in = i >= bi0 && i <= bi1;
+            const bool ua = in && (ae & (1 << 3)) && i <= bi1 - 1, ub = in && (ae & (1 << 5)) && k <= bk1 - 1;
+            float va = 0.0f, vb = 0.0f;
+            if (ua) va = ldo(yr, (uint32_t)i * 4u);
+            if (ub) vb = ldo(yr + yS, (uint32_t)i * 4u);
+            if (ua) {
+              const float v = ch[0][c] * va;
+              ux[c] = lo ? ux[c] - v : ux[c] + v;
+            }
+            if (ub) {
+              const float v = ch[2][c] * vb;
+              uz[c] = lo ? uz[c] + v : uz[c] - v;
             }
           }
         }
@@ -1393,15 +1406,18 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 #pragma unroll
           for (int c = 0; c < CPL; ++c) {
             const int i = ib + c;
-            if (i >= bi0 && i <= bi1) {
-              if ((ae & (1 << 6)) && i <= bi1 - 1) {
-                const float v = ch[0][c] * ldo(zr, (uint32_t)i * 4u);
-                ux[c] = lo ? ux[c] + v : ux[c] - v;
-              }
-              if ((ae & (1 << 7)) && j <= bj1 - 1) {
-                const float v = ch[1][c] * ldo(zr + zS, (uint32_t)i * 4u);
-                uy[c] = lo ? uy[c] - v : uy[c] + v;
-              }
+            const bool in = i >= bi0 && i <= bi1;
+            const bool ua = in && (ae & (1 << 6)) && i <= bi1 - 1, ub = in && (ae & (1 << 7)) && j <= bj1 - 1;
+            float va = 0.0f, vb = 0.0f;
+            if (ua) va = ldo(zr, (uint32_t)i * 4u);
+            if (ub) vb = ldo(zr + zS, (uint32_t)i * 4u);
+            if (ua) {
+              const float v = ch[0][c] * va;
+              ux[c] = lo ? ux[c] + v : ux[c] - v;
+            }
+            if (ub) {
+              const float v = ch[1][c] * vb;
+              uy[c] = lo ? uy[c] - v : uy[c] + v;
             }
           }
         }
