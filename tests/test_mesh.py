@@ -391,3 +391,39 @@ def test_gpu_graded_refuses_plane_wave():
     g, _ = _gpu_pair(I, J, K, P, _mesh3(I, J, K, P), probes=False)
     with pytest.raises(SimFDTDError, match="plane wave"):
         g.run(0, 3, 3)
+
+
+@pytest.mark.gpu
+def test_gpu_graded_request_served_matches_oracle():
+    """A graded-mesh request (the graded sphere workload at 48^3, a line current after every
+    step) served by the request server beside a uniform one: fields and probes equal the oracle's
+    op sequence, bit for bit."""
+    from maxwell_solver_amd import Server, Simulation
+    from maxwell_solver_amd import scenarios as S
+    from tests.fdtd_compare import assert_close, compare_state
+    srv = Server(hbm_budget=64 << 30, max_active=2, steps_per_round=16)
+    cases = {}
+    for graded in (True, False):
+        sc = S.config_graded(T=50, N=48, graded=graded)
+        sim = Simulation()
+        S.setup_host(sim, sc)
+        sim.set_step_currents(sc.currents[0], list(sc.currents[1]))
+        cases[srv.submit(sim, 50, 0, source=sc.source)] = sc
+    done = []
+    while len(done) < len(cases):
+        srv.round()
+        done += srv.collect()
+    for rid, *_ in done:
+        sc = cases[rid]
+        o = OracleSim(np.float32)
+        S.setup(o, sc, init_line=False)
+        pol, box = sc.currents[0], list(sc.currents[1])
+        for n in range(50):
+            o.update_h_all(); o.update_h_pml(); o.update_e_all(); o.update_e_pml()
+            o.add_currents(n, pol, box)
+        g = srv.simulation(rid)
+        assert_close(compare_state(g, o), exact=True)
+        np.testing.assert_array_equal(g.probes(), o.probes())
+        assert np.abs(g.field("Ex")).max() > 0
+        srv.release(rid)
+    srv.close()
