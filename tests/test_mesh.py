@@ -427,3 +427,22 @@ def test_gpu_graded_request_served_matches_oracle():
         assert np.abs(g.field("Ex")).max() > 0
         srv.release(rid)
     srv.close()
+
+
+@pytest.mark.gpu
+def test_gpu_graded_graph_replay_with_step_currents():
+    """Runs of a graded grid replayed from recorded HIP graphs (8 steps each), with eager steps
+    between the replays, equal to the oracle's op sequence."""
+    from tests.fdtd_compare import assert_close, compare_state
+    I, J, K, P = 48, 40, 44, 6
+    g, o = _gpu_pair(I, J, K, P, _mesh3(I, J, K, P), probes=False)
+    box = [20, 26, 18, 18, 21, 21]
+    g.set_step_currents(1, box)
+    g.set_graph_steps(8)
+    g.run(0, 21, 0)
+    g.step(21, 0)
+    g.run(22, 18, 0)
+    for n in range(40):
+        o.update_h_all(); o.update_h_pml(); o.update_e_all(); o.update_e_pml()
+        o.add_currents(n, 1, box)
+    assert_close(compare_state(g, o), exact=True)
