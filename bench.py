@@ -8,8 +8,10 @@ fill, the row kernel's interior part (rows P+1 .. J-P-1 of the planes P .. K-P-1
 dominant launch), its slab part (the other rows and planes, y/z CPML inline) and the probe gather.
 
 Prints one JSON line: Gcell-updates/s over exactly --steps timed steps (inputs resident in HBM),
-the interior part's HBM roofline fraction from HIP-event times around its launches, and the CPU
-oracle timed on a bounded sample of the same workload on this host's cores.
+the interior part's HBM roofline fraction from HIP-event times around its launches, the CPU
+oracle timed on a bounded sample of the same workload on this host's cores, and the GPU's probe
+record and fields after the oracle's timed steps compared with the oracle's (float32, and a
+shorter float64 run).
 
 --copy-calibration runs a plain device copy of known size (the scale of the HBM byte counters).
 --workload batch runs the 96-grid workload through the batched engine and, for comparison, each
@@ -31,6 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # part): each event pair costs the stream about 4 us, so timing every step would slow the run
 # that `value` measures by ~7%.
 TIMING_EVERY = 16
+# parity bar of a float32 GPU result against the float32 CPU oracle (relative L2)
+REL_L2_TOL = 1e-5
+FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
 def bytes_per_cell(sc, material_bytes=1):
@@ -67,7 +72,9 @@ def run_batch(args):
     streams; all grids advanced together by the batched engine (merged launches, the device work
     queue, the engine's automatic choice).  --batch-legs picks the legs; reported are each leg's
     rate, the batched / sequential ratio, and whether every grid's probe record of each batched run
-    equals that of its run alone (when the sequential leg ran)."""
+    equals that of its run alone (when the sequential leg ran).  Unless --no-cpu, the probe records
+    of the least-work grids (--oracle-cells cell-updates in all) from the leg that gives `value` are
+    compared with the CPU oracle run in float32 and float64 (`parity`)."""
     import numpy as np
 
     from maxwell_solver_amd import Batch, Simulation
@@ -94,8 +101,16 @@ def run_batch(args):
     if bad or not legs:
         raise SystemExit(f"--batch-legs: unknown leg(s) {sorted(bad)} (sequential, streams, merged, queue, auto)")
     times, rec, same = {}, None, True
+    # the leg that gives `value`, and the grids whose probe records from it go to the CPU oracle
+    order = [x for x in ("sequential", "streams", "merged", "queue", "auto") if x in legs]
+    val_leg = "auto" if "auto" in legs else order[0]
+    sample = sample_by_work([sc.cells * steps for sc, steps in work], args.oracle_cells)
+    kept = {}
 
-    def rewind():
+    def rewind(leg):
+        if leg == val_leg:
+            for q in sample:
+                kept[q] = sims[q][0].probes().copy()
         for g, _, _ in sims:
             g.reset()
             g.synchronize()
@@ -108,7 +123,7 @@ def run_batch(args):
             g.synchronize()
         times["sequential"] = time.perf_counter() - t0
         rec = [g.probes().copy() for g, _, _ in sims]
-        rewind()
+        rewind("sequential")
     if "streams" in legs:
         # every grid queued at once on its own stream (the GPU overlaps them as its queues allow)
         t0 = time.perf_counter()
@@ -117,7 +132,7 @@ def run_batch(args):
         for g, _, _ in sims:
             g.synchronize()
         times["streams"] = time.perf_counter() - t0
-        rewind()
+        rewind("streams")
     # merged launches (each kernel of the step once for all grids, four interleaved sub-batches),
     # the device work queue (persistent launches pulling tiles from per-XCD counters), and the
     # engine's automatic choice (the measured value)
@@ -133,7 +148,7 @@ def run_batch(args):
         times[leg] = time.perf_counter() - t0
         if rec is not None:
             same = same and all(np.array_equal(g.probes(), r) for (g, _, _), r in zip(sims, rec))
-        rewind()
+        rewind(leg)
     t_bat = times.get("auto", next(iter(times.values())))
     grid_steps = sum(steps for _, steps in work)
     out = {
@@ -167,6 +182,8 @@ def run_batch(args):
     b.close()
     for g, _, _ in sims:
         g.close()
+    if not args.no_cpu:
+        out["parity"] = oracle_check([(f"grid {q} ({val_leg} leg)", work[q][0], work[q][1], kept[q]) for q in sample])
     print(json.dumps(out), flush=True)
 
 
@@ -176,7 +193,9 @@ def run_serve(args):
     requests run one after another through the plain path.  The arrival rate is sized from the
     plain path's rate to keep the GPU near saturation (0.9 of it).  Reports aggregate
     Gcell-updates/s, p50 / p99 time to completion, and whether every served probe record equals the
-    plain run's.
+    plain run's.  Unless --no-cpu, the probe records of the least-work requests (--oracle-cells
+    cell-updates in all) from the leg that gives `value` are compared with the CPU oracle run in
+    float32 and float64 (`parity`: per-request relative L2).
 
     --serve-legs picks the legs (sequential, open, closed); a run with more requests than one
     process should take at once runs the sequential leg alone with --serve-digests FILE (the plain
@@ -192,6 +211,10 @@ def run_serve(args):
     legs = set(args.serve_legs.split(","))
     reqs = S.config5(args.requests)
     total = sum(sc.cells * steps for sc, steps, _ in reqs)
+    # the leg that gives `value`, and the requests whose probe records from it go to the CPU oracle
+    val_leg = "open" if "open" in legs else "closed" if "closed" in legs else "sequential"
+    sample = set(sample_by_work([sc.cells * steps for sc, steps, _ in reqs], args.oracle_cells))
+    kept = {}
 
     def digest(a):
         return hashlib.sha1(np.ascontiguousarray(a).tobytes()).hexdigest()
@@ -217,6 +240,8 @@ def run_serve(args):
             t_seq += dt
             lat_seq.append(t_seq)
             rec.append(digest(g.probes()))
+            if val_leg == "sequential" and q in sample:
+                kept[q] = g.probes().copy()
             g.close()
             if q % 100 == 99:
                 print(f"sequential: {q + 1} requests, {t_seq:.1f} s", file=sys.stderr, flush=True)
@@ -260,7 +285,10 @@ def run_serve(args):
             active, queued = srv.round()
             for rid, ta, tadm, tdone in srv.collect():
                 q = ids[rid]
-                same = same and digest(srv.simulation(rid).probes()) == seq["digests"][q]
+                pr = srv.simulation(rid).probes()
+                same = same and digest(pr) == seq["digests"][q]
+                if label == val_leg and q in sample:
+                    kept[q] = pr.copy()
                 lat.append(tdone - ta)
                 t_end = max(t_end, tdone)
                 srv.release(rid)
@@ -307,6 +335,9 @@ def run_serve(args):
     if value is None:
         value = rate_plain / 1e9
     cfg["served_equals_sequential"] = bool(same) if ("open" in legs or "closed" in legs) else None
+    parity = None
+    if not args.no_cpu:
+        parity = oracle_check([(f"request {q} ({val_leg} leg)", reqs[q][0], reqs[q][1], kept[q]) for q in sorted(sample)])
     out = {
         "metric": "Gcell-updates/s",
         "value": round(value, 3),
@@ -322,6 +353,8 @@ def run_serve(args):
         "data": "synthetic (seeded request stream: sizes, materials, plane waves, Poisson arrivals)",
         "config": cfg,
     }
+    if parity is not None:
+        out["parity"] = parity
     print(json.dumps(out), flush=True)
 
 
@@ -349,11 +382,106 @@ def host_cpu():
     return model or "unknown", len(cores) or None, usable
 
 
-def cpu_baseline(sc, budget_s=15.0):
+def oracle_threads():
+    """The threads the CPU oracle may use here: the usable logical CPUs, at most OMP_NUM_THREADS."""
+    usable = host_cpu()[2]
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return min(usable, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else usable
+
+
+def _diff(a, b):
+    """(relative L2, max-abs) of a against b, in float64."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    nb = float(np.linalg.norm(b))
+    d = float(np.linalg.norm(a - b))
+    return (d / nb if nb > 0 else d), (float(np.max(np.abs(a - b))) if a.size else 0.0)
+
+
+def gpu_state(sc, steps):
+    """Probe record and the six field components of `sc` after `steps` steps from step 0 on the
+    GPU (a fresh simulation, no snapshots)."""
+    from maxwell_solver_amd import Simulation
+    from maxwell_solver_amd import scenarios as S
+    g = Simulation()
+    S.setup(g, sc)
+    g.run(0, steps, sc.mode)
+    g.synchronize()
+    out = g.probes().copy(), {c: g.field(c) for c in FIELDS}
+    g.close()
+    return out
+
+
+def state_diff(gpu, o):
+    """Probe and field differences of a GPU state (gpu_state) against an oracle simulation."""
+    probes, fields = gpu
+    pr, pa = _diff(probes, o.probes())
+    fr = [_diff(fields[c], o.field(c)) for c in FIELDS]
+    return {"probe_rel_l2": pr, "probe_max_abs": pa,
+            "field_rel_l2_max": max(r for r, _ in fr), "field_max_abs": max(a for _, a in fr)}
+
+
+def sample_by_work(works, budget_cells):
+    """Indices of the cases with the least work (cell-updates) first, as many as fit in
+    budget_cells in all, at least one."""
+    pick, tot = [], 0
+    for q in sorted(range(len(works)), key=lambda q: works[q]):
+        if pick and tot + works[q] > budget_cells:
+            break
+        pick.append(q)
+        tot += works[q]
+    return pick
+
+
+def oracle_check(cases):
+    """Probe records of GPU runs against the CPU oracle run on this host from step 0, in float32
+    (the parity bar: relative L2 at most 1e-5) and in float64 (reported beside it).  `cases` holds
+    (label, scenario, steps, gpu probe record) tuples."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from maxwell_solver_amd import scenarios as S
+    threads = oracle_threads()
+    orc.set_threads(threads)
+    t0 = time.perf_counter()
+    rows = []
+    for label, sc, steps, rec in cases:
+        row = {"case": label, "grid": [sc.I, sc.J, sc.K], "steps": steps}
+        for dt, key in ((np.float32, "f32"), (np.float64, "f64")):
+            o = orc.OracleSim(dt)
+            S.setup(o, sc)
+            o.run(0, steps, sc.mode)
+            r, a = _diff(rec, o.probes())
+            o.close()
+            row[f"probe_rel_l2_{key}"] = r
+            row[f"probe_max_abs_{key}"] = a
+        rows.append(row)
+    worst = max(r["probe_rel_l2_f32"] for r in rows)
+    return {
+        "oracle": "CPU oracle on this host, float32 and float64, same scenario from step 0",
+        "oracle_threads": threads,
+        "cases": len(rows),
+        "cell_updates": int(sum(sc.cells * steps for _, sc, steps, _ in cases)),
+        "probe_rel_l2_f32_max": worst,
+        "probe_rel_l2_f64_max": max(r["probe_rel_l2_f64"] for r in rows),
+        "tol": REL_L2_TOL,
+        "within_tol": bool(worst <= REL_L2_TOL),
+        "seconds": round(time.perf_counter() - t0, 2),
+        "per_case": rows,
+    }
+
+
+def cpu_baseline(sc, budget_s=15.0, parity=True):
     """The CPU oracle (float32, OpenMP over k planes) on the same workload, on this host's cores:
     built for this CPU (-march=native) when g++ can, timed with 1 thread and with all the threads
     the process may use (at most OMP_NUM_THREADS when set), each the median of 3 timed samples of a
-    few steps; about budget_s seconds in all."""
+    few steps; about budget_s seconds in all.
+
+    The all-thread samples run contiguously from step 0, so with `parity` the oracle's state after
+    them is compared with the GPU's after the same steps (probe record and fields), and a short
+    float64 oracle run (about budget_s / 6 seconds) with the GPU's state after its steps; returned
+    as the second value (None without `parity`)."""
     import statistics
     import tempfile
 
@@ -368,8 +496,7 @@ def cpu_baseline(sc, budget_s=15.0):
         orc.use_library(orc.build_native(tmp))
     except Exception as e:  # no compiler or no -march=native: the in-tree x86-64-v3 build
         build = f"-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (in-tree build; native build failed: {type(e).__name__})"
-    omp = os.environ.get("OMP_NUM_THREADS")
-    threads = min(usable, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else usable
+    threads = oracle_threads()
     o = orc.OracleSim(np.float32)
     S.setup(o, sc)
     n = 0
@@ -386,14 +513,32 @@ def cpu_baseline(sc, budget_s=15.0):
         return sc.cells * steps / dt / 1e9, dt
 
     per = budget_s / 6.0  # three samples each of two thread counts
-    out = {}
+    out, par = {}, None
     for label, nt in (("all", threads), ("one", 1)):
         r0, dt0 = rate(nt, 1)  # calibration step (also warms the caches)
-        steps = max(1, min(sc.T, int(per / dt0)))
+        # the all-thread samples stay within the source's T samples: steps 0 .. n-1 in order
+        steps = max(1, min((sc.T - 1) // 3, int(per / dt0)))
         runs = [rate(nt, steps)[0] for _ in range(3)]
         out[label] = (statistics.median(runs), runs, steps)
+        if label == "all" and parity:
+            par = {"steps": n, "oracle": "float32 CPU oracle (the all-thread samples, steps 0 .. n-1 in order)"}
+            par.update(state_diff(gpu_state(sc, n), o))
+            par["tol"] = REL_L2_TOL
+            par["within_tol"] = bool(par["probe_rel_l2"] <= REL_L2_TOL and par["field_rel_l2_max"] <= REL_L2_TOL)
     o.close()
-    return {
+    if par is not None:
+        # float64 oracle: about per seconds at half the float32 rate
+        n64 = max(4, min(par["steps"], int(per * out["all"][0] * 1e9 / 2 / sc.cells)))
+        orc.set_threads(threads)
+        o64 = orc.OracleSim(np.float64)
+        S.setup(o64, sc)
+        t0 = time.perf_counter()
+        o64.run(0, n64, sc.mode)
+        f64 = state_diff(gpu_state(sc, n64), o64)
+        o64.close()
+        par.update({"f64_steps": n64, "f64_seconds": round(time.perf_counter() - t0, 2)})
+        par.update({f"{k}_f64": v for k, v in f64.items()})
+    base = {
         "value": round(out["all"][0], 4),
         "unit": "Gcell-updates/s",
         "cores": threads,
@@ -408,6 +553,7 @@ def cpu_baseline(sc, budget_s=15.0):
         "logical_cpus_usable": usable,
         "build": build,
     }
+    return base, par
 
 
 def run_gic(args):
@@ -539,6 +685,9 @@ def main():
     ap.add_argument("--slab-chunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--oracle-cells", type=float, default=4e9,
+                    help="batch / serve: cell-updates of the least-work grids or requests whose probe "
+                         "records are checked against the CPU oracle (float32 and float64)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["grid", "layered", "batch", "serve", "gic", "graded"], default="grid",
                     help="grid: the 256^3 workload (default); layered: the 384x384x256 lossy layered-ground "
@@ -683,7 +832,7 @@ def main():
     if not args.no_cpu:
         base = (S.config3(T=600) if args.workload == "layered" else
                 S.config2(T=2000) if args.size == 256 else sc)
-        rec["cpu_baseline"] = cpu_baseline(base, budget_s=args.cpu_budget)
+        rec["cpu_baseline"], rec["parity"] = cpu_baseline(base, budget_s=args.cpu_budget)
     print(json.dumps(rec), flush=True)
 
 

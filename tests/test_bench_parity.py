@@ -1,0 +1,60 @@
+"""The CPU-oracle checks that bench.py attaches to its lines (runs without a GPU).
+
+The checker itself is checked here: fed the oracle's own float32 record it reports zero error and
+passes; fed a record with one planted mistake (a probe sample off by 1e-3 relative) it fails; its
+float64 figure is the float32 / float64 distance of the oracle itself, small but non-zero.  The
+least-work sample takes cases in ascending work up to its budget, at least one.
+"""
+import dataclasses
+
+import numpy as np
+
+import bench
+
+
+def _case(T=60):
+    from maxwell_solver_amd import scenarios as S
+    sc = S.small_variant(S.config2(T=T), 40, T)
+    return dataclasses.replace(sc, snap_every=0)
+
+
+def _oracle_record(sc, steps, dtype=np.float32):
+    from maxwell_solver_amd import scenarios as S
+    from oracle.oracle import OracleSim
+    o = OracleSim(dtype)
+    S.setup(o, sc)
+    o.run(0, steps, sc.mode)
+    rec = o.probes().copy()
+    o.close()
+    return rec
+
+
+def test_sample_by_work_takes_least_work_first_within_budget():
+    assert bench.sample_by_work([5, 1, 3, 2], 6) == [1, 3, 2]
+    assert bench.sample_by_work([5, 1, 3, 2], 0) == [1]  # at least one
+    assert bench.sample_by_work([7, 9], 100) == [0, 1]
+
+
+def test_oracle_check_passes_the_oracle_and_fails_a_planted_mistake():
+    sc, steps = _case(), 50
+    rec = _oracle_record(sc, steps)
+    assert np.abs(rec).max() > 0  # the plane wave has reached the probes
+    good = bench.oracle_check([("good", sc, steps, rec)])
+    assert good["cases"] == 1 and good["within_tol"]
+    assert good["probe_rel_l2_f32_max"] == 0.0
+    assert 0.0 < good["probe_rel_l2_f64_max"] < 1e-4
+    bad = rec.copy()
+    q = np.unravel_index(np.argmax(np.abs(bad)), bad.shape)
+    bad[q] *= 1.0 + 1e-3 * np.sqrt(bad.size)  # relative L2 of about 1e-3 if all samples were alike
+    out = bench.oracle_check([("good", sc, steps, rec), ("planted", sc, steps, bad)])
+    assert not out["within_tol"]
+    assert out["per_case"][0]["probe_rel_l2_f32"] == 0.0
+    assert out["per_case"][1]["probe_rel_l2_f32"] > bench.REL_L2_TOL
+
+
+def test_diff_is_relative_l2_and_max_abs():
+    a = np.array([1.0, 2.0, 2.0], dtype=np.float32)
+    b = np.array([1.0, 2.0, 1.0], dtype=np.float32)
+    r, m = bench._diff(a, b)
+    assert m == 1.0 and abs(r - 1.0 / np.sqrt(6.0)) < 1e-15
+    assert bench._diff(np.zeros(3), np.zeros(3)) == (0.0, 0.0)
