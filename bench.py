@@ -336,8 +336,8 @@ def run_serve(args):
         value = rate_plain / 1e9
     cfg["served_equals_sequential"] = bool(same) if ("open" in legs or "closed" in legs) else None
     parity = None
-    if not args.no_cpu:
-        parity = oracle_check([(f"request {q} ({val_leg} leg)", reqs[q][0], reqs[q][1], kept[q]) for q in sorted(sample)])
+    if not args.no_cpu and kept:  # the sample's records from the leg that gives `value`
+        parity = oracle_check([(f"request {q} ({val_leg} leg)", reqs[q][0], reqs[q][1], kept[q]) for q in sorted(kept)])
     out = {
         "metric": "Gcell-updates/s",
         "value": round(value, 3),
@@ -434,10 +434,17 @@ def sample_by_work(works, budget_cells):
     return pick
 
 
+# probe records whose float64 RMS is below this (the incident plane wave has unit amplitude) hold
+# rounding residue only -- e.g. a low-frequency pulse that has not started within the run's steps
+# -- and their relative L2 against float64 measures nothing; they are listed, not averaged in
+F64_SIGNAL_FLOOR = 1e-9
+
+
 def oracle_check(cases):
     """Probe records of GPU runs against the CPU oracle run on this host from step 0, in float32
-    (the parity bar: relative L2 at most 1e-5) and in float64 (reported beside it).  `cases` holds
-    (label, scenario, steps, gpu probe record) tuples."""
+    (the parity bar: relative L2 at most 1e-5) and in float64 (reported beside it; the maximum over
+    the cases whose float64 record is above F64_SIGNAL_FLOOR in RMS).  `cases` holds (label,
+    scenario, steps, gpu probe record) tuples."""
     import numpy as np
 
     from oracle import oracle as orc
@@ -452,19 +459,26 @@ def oracle_check(cases):
             o = orc.OracleSim(dt)
             S.setup(o, sc)
             o.run(0, steps, sc.mode)
-            r, a = _diff(rec, o.probes())
+            pr = o.probes()
+            r, a = _diff(rec, pr)
             o.close()
             row[f"probe_rel_l2_{key}"] = r
             row[f"probe_max_abs_{key}"] = a
+            if key == "f64":
+                row["probe_rms_f64"] = float(np.sqrt(np.mean(np.square(pr, dtype=np.float64)))) if pr.size else 0.0
         rows.append(row)
     worst = max(r["probe_rel_l2_f32"] for r in rows)
+    sig = [r for r in rows if r["probe_rms_f64"] >= F64_SIGNAL_FLOOR]
     return {
         "oracle": "CPU oracle on this host, float32 and float64, same scenario from step 0",
         "oracle_threads": threads,
         "cases": len(rows),
         "cell_updates": int(sum(sc.cells * steps for _, sc, steps, _ in cases)),
         "probe_rel_l2_f32_max": worst,
-        "probe_rel_l2_f64_max": max(r["probe_rel_l2_f64"] for r in rows),
+        "probe_rel_l2_f64_max": max((r["probe_rel_l2_f64"] for r in sig), default=None),
+        "probe_max_abs_f64_max": max(r["probe_max_abs_f64"] for r in rows),
+        "f64_signal_floor": F64_SIGNAL_FLOOR,
+        "cases_below_f64_floor": [r["case"] for r in rows if r["probe_rms_f64"] < F64_SIGNAL_FLOOR],
         "tol": REL_L2_TOL,
         "within_tol": bool(worst <= REL_L2_TOL),
         "seconds": round(time.perf_counter() - t0, 2),

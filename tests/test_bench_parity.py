@@ -52,6 +52,22 @@ def test_oracle_check_passes_the_oracle_and_fails_a_planted_mistake():
     assert out["per_case"][1]["probe_rel_l2_f32"] > bench.REL_L2_TOL
 
 
+def test_records_without_a_field_are_listed_below_the_float64_floor():
+    """One step: no field has reached the probes, the float64 record is zero, and its relative L2
+    measures nothing -- the case is listed and left out of the float64 maximum."""
+    sc = _case()
+    empty = _oracle_record(sc, 1)
+    assert not np.any(empty)
+    alone = bench.oracle_check([("empty", sc, 1, empty)])
+    assert alone["within_tol"] and alone["cases_below_f64_floor"] == ["empty"]
+    assert alone["probe_rel_l2_f64_max"] is None
+    rec = _oracle_record(sc, 50)
+    both = bench.oracle_check([("empty", sc, 1, empty), ("field", sc, 50, rec)])
+    assert both["cases_below_f64_floor"] == ["empty"]
+    assert both["probe_rel_l2_f64_max"] == both["per_case"][1]["probe_rel_l2_f64"] > 0.0
+    assert both["per_case"][1]["probe_rms_f64"] >= bench.F64_SIGNAL_FLOOR
+
+
 def test_diff_is_relative_l2_and_max_abs():
     a = np.array([1.0, 2.0, 2.0], dtype=np.float32)
     b = np.array([1.0, 2.0, 1.0], dtype=np.float32)
