@@ -36,6 +36,11 @@ TIMING_EVERY = 16
 # parity bar of a float32 GPU result against the float32 CPU oracle (relative L2)
 REL_L2_TOL = 1e-5
 FIELDS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+# states (probe records, field components) whose float64 RMS is below this (the incident plane
+# wave has unit amplitude) hold rounding residue only -- e.g. a low-frequency pulse that has not
+# started within the run's steps, or a component the polarisation never excites -- and a relative
+# L2 against them measures nothing; they are listed, not counted in the relative maxima
+F64_SIGNAL_FLOOR = 1e-9
 
 
 def bytes_per_cell(sc, material_bytes=1):
@@ -415,11 +420,27 @@ def gpu_state(sc, steps):
 
 def state_diff(gpu, o):
     """Probe and field differences of a GPU state (gpu_state) against an oracle simulation."""
+    import numpy as np
     probes, fields = gpu
-    pr, pa = _diff(probes, o.probes())
-    fr = [_diff(fields[c], o.field(c)) for c in FIELDS]
-    return {"probe_rel_l2": pr, "probe_max_abs": pa,
-            "field_rel_l2_max": max(r for r, _ in fr), "field_max_abs": max(a for _, a in fr)}
+    op = o.probes()
+    pr, pa = _diff(probes, op)
+    of = {c: o.field(c) for c in FIELDS}
+    fr = {c: _diff(fields[c], of[c]) for c in FIELDS}
+    # RMS of the oracle's state: the magnitude the differences are relative to.  A component whose
+    # RMS is below F64_SIGNAL_FLOOR is zero in exact arithmetic here (e.g. a component the
+    # polarisation never excites) and holds rounding residue only: it is listed, and left out of
+    # the per-component maximum; the whole-state figure covers all six components.
+    ss = {c: float(np.sum(np.square(of[c], dtype=np.float64))) for c in FIELDS}
+    n = sum(of[c].size for c in FIELDS)
+    dd = sum(float(np.sum(np.square(fields[c].astype(np.float64) - of[c]))) for c in FIELDS)
+    tot = sum(ss.values())
+    quiet = [c for c in FIELDS if np.sqrt(ss[c] / max(of[c].size, 1)) < F64_SIGNAL_FLOOR]
+    rms_p = float(np.sqrt(np.mean(np.square(op, dtype=np.float64)))) if op.size else 0.0
+    return {"probe_rel_l2": pr, "probe_max_abs": pa, "probe_rms": rms_p,
+            "field_rel_l2": float(np.sqrt(dd / tot)) if tot > 0 else float(np.sqrt(dd)),
+            "field_rel_l2_max": max((fr[c][0] for c in FIELDS if c not in quiet), default=0.0),
+            "field_max_abs": max(a for _, a in fr.values()),
+            "field_rms": float(np.sqrt(tot / n)), "field_components_below_floor": quiet}
 
 
 def sample_by_work(works, budget_cells):
@@ -432,12 +453,6 @@ def sample_by_work(works, budget_cells):
         pick.append(q)
         tot += works[q]
     return pick
-
-
-# probe records whose float64 RMS is below this (the incident plane wave has unit amplitude) hold
-# rounding residue only -- e.g. a low-frequency pulse that has not started within the run's steps
-# -- and their relative L2 against float64 measures nothing; they are listed, not averaged in
-F64_SIGNAL_FLOOR = 1e-9
 
 
 def oracle_check(cases):
@@ -493,9 +508,9 @@ def cpu_baseline(sc, budget_s=15.0, parity=True):
     few steps; about budget_s seconds in all.
 
     The all-thread samples run contiguously from step 0, so with `parity` the oracle's state after
-    them is compared with the GPU's after the same steps (probe record and fields), and a short
-    float64 oracle run (about budget_s / 6 seconds) with the GPU's state after its steps; returned
-    as the second value (None without `parity`)."""
+    them is compared with the GPU's after the same steps (probe record and fields), and a float64
+    oracle run over the same steps (about budget_s more seconds) likewise; returned as the second
+    value (None without `parity`)."""
     import statistics
     import tempfile
 
@@ -538,11 +553,12 @@ def cpu_baseline(sc, budget_s=15.0, parity=True):
             par = {"steps": n, "oracle": "float32 CPU oracle (the all-thread samples, steps 0 .. n-1 in order)"}
             par.update(state_diff(gpu_state(sc, n), o))
             par["tol"] = REL_L2_TOL
-            par["within_tol"] = bool(par["probe_rel_l2"] <= REL_L2_TOL and par["field_rel_l2_max"] <= REL_L2_TOL)
+            par["within_tol"] = bool(par["probe_rel_l2"] <= REL_L2_TOL and par["field_rel_l2"] <= REL_L2_TOL
+                                     and par["field_rel_l2_max"] <= REL_L2_TOL)
     o.close()
     if par is not None:
-        # float64 oracle: about per seconds at half the float32 rate
-        n64 = max(4, min(par["steps"], int(per * out["all"][0] * 1e9 / 2 / sc.cells)))
+        # float64 oracle over the same steps (about half the float32 rate: ~budget_s seconds)
+        n64 = par["steps"]
         orc.set_threads(threads)
         o64 = orc.OracleSim(np.float64)
         S.setup(o64, sc)
@@ -552,6 +568,9 @@ def cpu_baseline(sc, budget_s=15.0, parity=True):
         o64.close()
         par.update({"f64_steps": n64, "f64_seconds": round(time.perf_counter() - t0, 2)})
         par.update({f"{k}_f64": v for k, v in f64.items()})
+        # relative figures against a float64 state that holds rounding residue only measure nothing
+        par["f64_below_floor"] = {"probes": f64["probe_rms"] < F64_SIGNAL_FLOOR,
+                                  "fields": f64["field_rms"] < F64_SIGNAL_FLOOR}
     base = {
         "value": round(out["all"][0], 4),
         "unit": "Gcell-updates/s",

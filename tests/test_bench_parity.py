@@ -2,8 +2,10 @@
 
 The checker itself is checked here: fed the oracle's own float32 record it reports zero error and
 passes; fed a record with one planted mistake (a probe sample off by 1e-3 relative) it fails; its
-float64 figure is the float32 / float64 distance of the oracle itself, small but non-zero.  The
-least-work sample takes cases in ascending work up to its budget, at least one.
+float64 figure is the float32 / float64 distance of the oracle itself, small but non-zero; a
+record or field component at rounding level (below the float64 floor) is listed and kept out of
+the relative maxima.  The state comparison finds one planted field error.  The least-work sample
+takes cases in ascending work up to its budget, at least one.
 """
 import dataclasses
 
@@ -66,6 +68,37 @@ def test_records_without_a_field_are_listed_below_the_float64_floor():
     assert both["cases_below_f64_floor"] == ["empty"]
     assert both["probe_rel_l2_f64_max"] == both["per_case"][1]["probe_rel_l2_f64"] > 0.0
     assert both["per_case"][1]["probe_rms_f64"] >= bench.F64_SIGNAL_FLOOR
+
+
+def test_state_diff_of_the_oracle_with_itself_and_a_planted_field_error():
+    from maxwell_solver_amd import scenarios as S
+    from oracle.oracle import OracleSim
+    sc = _case()
+    o = OracleSim(np.float32)
+    S.setup(o, sc)
+    o.run(0, 50, sc.mode)
+    state = (o.probes().copy(), {c: o.field(c).copy() for c in bench.FIELDS})
+    same = bench.state_diff(state, o)
+    assert same["probe_rel_l2"] == same["field_rel_l2_max"] == 0.0
+    assert same["probe_max_abs"] == same["field_max_abs"] == 0.0
+    assert same["field_rms"] > bench.F64_SIGNAL_FLOOR and same["probe_rms"] > 0.0
+    hz = state[1]["Hz"]
+    q = np.unravel_index(np.argmax(np.abs(hz)), hz.shape)
+    hz[q] += 0.5 * abs(hz[q])  # one cell of one component off by half its value
+    off = bench.state_diff(state, o)
+    assert off["probe_rel_l2"] == 0.0 and off["field_rel_l2_max"] > bench.REL_L2_TOL
+    assert off["field_max_abs"] == np.float32(0.5 * abs(o.field("Hz")[q]))
+    # a component that is zero in the reference (as one the polarisation never excites) against
+    # rounding-sized residue in every cell: listed below the floor, out of the per-component
+    # maximum, and counted in the whole-state figure
+    o.set_field("Ez", np.zeros_like(o.field("Ez")))
+    state[1]["Hz"][q] = o.field("Hz")[q]
+    state[1]["Ez"] = np.full_like(state[1]["Ez"], 1e-7)
+    quiet = bench.state_diff(state, o)
+    assert quiet["field_components_below_floor"] == ["Ez"]
+    assert quiet["field_rel_l2_max"] == 0.0
+    assert 0.0 < quiet["field_rel_l2"] < 1e-3 and quiet["field_max_abs"] == np.float32(1e-7)
+    o.close()
 
 
 def test_diff_is_relative_l2_and_max_abs():
