@@ -78,8 +78,9 @@ def run_batch(args):
     queue, the engine's automatic choice).  --batch-legs picks the legs; reported are each leg's
     rate, the batched / sequential ratio, and whether every grid's probe record of each batched run
     equals that of its run alone (when the sequential leg ran).  Unless --no-cpu, the probe records
-    of the least-work grids (--oracle-cells cell-updates in all) from the leg that gives `value` are
-    compared with the CPU oracle run in float32 and float64 (`parity`)."""
+    of a sample of the grids (the least-work grid of each class of largest edge, then the least-work
+    others up to --oracle-cells cell-updates) from the leg that gives `value` are compared with the
+    CPU oracle run in float32 and float64 (`parity`)."""
     import numpy as np
 
     from maxwell_solver_amd import Batch, Simulation
@@ -109,7 +110,9 @@ def run_batch(args):
     # the leg that gives `value`, and the grids whose probe records from it go to the CPU oracle
     order = [x for x in ("sequential", "streams", "merged", "queue", "auto") if x in legs]
     val_leg = "auto" if "auto" in legs else order[0]
-    sample = sample_by_work([sc.cells * steps for sc, steps in work], args.oracle_cells)
+    # one grid per class of largest edge (32, 64, 96, 128, 192 and up), then the least work
+    sample = sample_by_work([sc.cells * steps for sc, steps in work], args.oracle_cells,
+                            [size_class(sc, (0, 64, 96, 128, 192)) for sc, _ in work])
     kept = {}
 
     def rewind(leg):
@@ -198,9 +201,10 @@ def run_serve(args):
     requests run one after another through the plain path.  The arrival rate is sized from the
     plain path's rate to keep the GPU near saturation (0.9 of it).  Reports aggregate
     Gcell-updates/s, p50 / p99 time to completion, and whether every served probe record equals the
-    plain run's.  Unless --no-cpu, the probe records of the least-work requests (--oracle-cells
-    cell-updates in all) from the leg that gives `value` are compared with the CPU oracle run in
-    float32 and float64 (`parity`: per-request relative L2).
+    plain run's.  Unless --no-cpu, the probe records of a sample of the requests (the least-work
+    request of each class of largest edge and material kind, then the least-work others up to
+    --oracle-cells cell-updates) from the leg that gives `value` are compared with the CPU oracle
+    run in float32 and float64 (`parity`: per-request relative L2).
 
     --serve-legs picks the legs (sequential, open, closed); a run with more requests than one
     process should take at once runs the sequential leg alone with --serve-digests FILE (the plain
@@ -218,7 +222,11 @@ def run_serve(args):
     total = sum(sc.cells * steps for sc, steps, _ in reqs)
     # the leg that gives `value`, and the requests whose probe records from it go to the CPU oracle
     val_leg = "open" if "open" in legs else "closed" if "closed" in legs else "sequential"
-    sample = set(sample_by_work([sc.cells * steps for sc, steps, _ in reqs], args.oracle_cells))
+    # one request per class of largest edge (48, 80, 128, 200 and up) and material kind (spheres
+    # with an oblique plane wave, mode 5; layered ground with a vertical pulse, mode 2), then the
+    # least work
+    sample = set(sample_by_work([sc.cells * steps for sc, steps, _ in reqs], args.oracle_cells,
+                                [(size_class(sc, (0, 80, 128, 200)), sc.mode) for sc, _, _ in reqs]))
     kept = {}
 
     def digest(a):
@@ -443,16 +451,30 @@ def state_diff(gpu, o):
             "field_rms": float(np.sqrt(tot / n)), "field_components_below_floor": quiet}
 
 
-def sample_by_work(works, budget_cells):
-    """Indices of the cases with the least work (cell-updates) first, as many as fit in
-    budget_cells in all, at least one."""
+def sample_by_work(works, budget_cells, groups=None):
+    """Indices of a sample of the cases: with `groups` (a class label per case) first the case with
+    the least work (cell-updates) of every class, whatever its work; then the remaining cases with
+    the least work first, as many as fit in budget_cells in all; at least one case."""
     pick, tot = [], 0
+    if groups is not None:
+        for g in sorted(set(groups), key=str):
+            q = min((q for q in range(len(works)) if groups[q] == g), key=lambda q: works[q])
+            pick.append(q)
+            tot += works[q]
     for q in sorted(range(len(works)), key=lambda q: works[q]):
+        if q in pick:
+            continue
         if pick and tot + works[q] > budget_cells:
             break
         pick.append(q)
         tot += works[q]
-    return pick
+    return sorted(pick, key=lambda q: works[q])
+
+
+def size_class(sc, bounds):
+    """Index of the class [bounds[c], bounds[c+1]) that holds the grid's largest edge."""
+    e = max(sc.I, sc.J, sc.K)
+    return max(c for c in range(len(bounds)) if bounds[c] <= e)
 
 
 def oracle_check(cases):
@@ -719,8 +741,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--oracle-cells", type=float, default=4e9,
-                    help="batch / serve: cell-updates of the least-work grids or requests whose probe "
-                         "records are checked against the CPU oracle (float32 and float64)")
+                    help="batch / serve: the probe records of the least-work grid or request of each size "
+                         "class (and material kind), then of the least-work others up to this many cell-updates "
+                         "in all, are checked against the CPU oracle (float32 and float64)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["grid", "layered", "batch", "serve", "gic", "graded"], default="grid",
                     help="grid: the 256^3 workload (default); layered: the 384x384x256 lossy layered-ground "

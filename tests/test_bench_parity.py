@@ -35,6 +35,20 @@ def test_sample_by_work_takes_least_work_first_within_budget():
     assert bench.sample_by_work([5, 1, 3, 2], 6) == [1, 3, 2]
     assert bench.sample_by_work([5, 1, 3, 2], 0) == [1]  # at least one
     assert bench.sample_by_work([7, 9], 100) == [0, 1]
+    # one case per class first (whatever its work), then the least work within the budget
+    works, groups = [5, 1, 3, 2, 40, 30], ["a", "a", "a", "a", "b", "b"]
+    assert bench.sample_by_work(works, 0, groups) == [1, 5]
+    assert bench.sample_by_work(works, 34, groups) == [1, 3, 5]
+    assert bench.sample_by_work(works, 100, groups) == [1, 3, 2, 0, 5, 4]
+
+
+def test_size_class_of_the_largest_edge():
+    from types import SimpleNamespace as G
+    b = (0, 64, 96, 128, 192)
+    assert bench.size_class(G(I=32, J=63, K=40), b) == 0
+    assert bench.size_class(G(I=64, J=32, K=32), b) == 1
+    assert bench.size_class(G(I=32, J=32, K=191), b) == 3
+    assert bench.size_class(G(I=268, J=140, K=134), b) == 4
 
 
 def test_oracle_check_passes_the_oracle_and_fails_a_planted_mistake():
