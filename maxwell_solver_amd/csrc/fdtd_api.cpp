@@ -53,6 +53,26 @@ int fail(int code, const std::string &msg) {
 
 }  // namespace
 
+// Row pitch of the per-cell device arrays.  A row of I floats that is not a whole number of
+// 128-byte lines starts off a line boundary in every row after the first, so each wave's load and
+// store of 64 consecutive columns straddles three lines, two of them partly written by another
+// wave.  Where the state is larger than the 256 MB Infinity Cache the step's stores stream to HBM
+// and such lines cost a read-modify-write there (252 x 256 x 256: 60.3 Gcell/s, 68.7 with rows
+// padded to 256; 256^3: 74.4); where it fits, the partial lines meet in the cache and padding
+// gains nothing (250 x 280 x 50: 54.2 against 53.5 padded) while adding bytes.  So rows are padded
+// to a multiple of 32 floats when the state streams; a multiple of 64 columns is kept as it is
+// (the split row layout needs I itself).  FDTD_ROW_ALIGN = 1 (no padding), 16, 32 or 64 sets the
+// multiple whatever the state size.
+static int row_pitch(int I, bool streams) {
+  int a = streams ? 32 : 1;
+  if (const char *e = std::getenv("FDTD_ROW_ALIGN")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 16 || v == 32 || v == 64) a = v;
+  }
+  if (a == 1 || I % 64 == 0) return I;
+  return (I + a - 1) / a * a;
+}
+
 struct fdtd_sim {
   // ---- flags (src/simulation.h:1305-1311) ----
   bool init_const = false, init_source = false, init_antenna = false, init_ready = false;
@@ -62,6 +82,10 @@ struct fdtd_sim {
   float fctr = 0, delX = 0, delT = 0;
   int T = 0, I = 0, J = 0, K = 0, nAnt = 0;
   size_t IJ = 0, IJK = 0;
+  // per-cell device arrays (fields, materials, per-cell coefficients): rows of Ip >= I elements
+  // (row_pitch), planes of IJp = Ip*J, IJKp = Ip*J*K elements; host arrays stay I*J*K
+  int Ip = 0;
+  size_t IJp = 0, IJKp = 0;
   CpmlParams cp;
   int width = 15;
 
@@ -220,8 +244,9 @@ struct fdtd_sim {
   size_t device_bytes_bound(int mode) const {
     const size_t P = (size_t)cp.P;
     const size_t sx = 2 * P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
-    size_t b = 2 * ((6 * (IJK + kCompSkew) + kSetSkew) + 4 * (sx + sy + sz)) * sizeof(float);
-    b += have_index ? IJK + 256 * 6 * sizeof(float) : 6 * IJK * sizeof(float);
+    const size_t IJKb = (size_t)((I + 63) / 64 * 64) * J * K;  // per-cell arrays at the widest row pitch
+    size_t b = 2 * ((6 * (IJKb + kCompSkew) + kSetSkew) + 4 * (sx + sy + sz)) * sizeof(float);
+    b += have_index ? IJKb + 256 * 6 * sizeof(float) : 6 * IJKb * sizeof(float);
     b += (8 * P + 2 * ((size_t)I + J + K)) * sizeof(float);
     if (mode) {
       const size_t N = (size_t)I + J + K + 64;                // line nodes, oblique included
@@ -231,7 +256,7 @@ struct fdtd_sim {
       b += 2 * 8 * (nj * nk + ni * nk + ni * nj) * sizeof(float);  // face sheets, two buffers
     }
     if (init_antenna) b += (probe_comps.size() * (size_t)T * nAnt + nAnt + probe_comps.size()) * sizeof(float);
-    if (!mu_in[0].empty()) b += 3 * IJK * sizeof(float);  // CreateMu coefficients
+    if (!mu_in[0].empty()) b += 3 * IJKb * sizeof(float);  // CreateMu coefficients
     return b + ((size_t)1 << 20);  // argument blocks and small tables
   }
   template <typename Tp>
@@ -239,6 +264,28 @@ struct fdtd_sim {
     if (int rc = dalloc(p, v.size())) return rc;
     HIPCHK(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(Tp), hipMemcpyHostToDevice, stream));
     return 0;
+  }
+  // A host array of I*J*K cells to a per-cell device array (rows of Ip) and back.  Padded rows go
+  // through a host copy of the padded layout (zero padding columns) and one linear transfer: a
+  // strided transfer from pageable memory moves a row at a time (a 320^3 material upload took
+  // longer than the solve).  The padded case completes before returning.
+  template <typename Tp>
+  hipError_t put_cells(Tp *dev, const Tp *host) const {
+    if (Ip == I) return hipMemcpyAsync(dev, host, IJK * sizeof(Tp), hipMemcpyHostToDevice, stream);
+    std::vector<Tp> pad(IJKp, Tp(0));
+    for (size_t r = 0; r < (size_t)J * K; ++r) std::memcpy(&pad[r * Ip], host + r * I, (size_t)I * sizeof(Tp));
+    hipError_t e = hipMemcpyAsync(dev, pad.data(), IJKp * sizeof(Tp), hipMemcpyHostToDevice, stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(stream);
+  }
+  template <typename Tp>
+  hipError_t get_cells(Tp *host, const Tp *dev) const {
+    if (Ip == I) return hipMemcpyAsync(host, dev, IJK * sizeof(Tp), hipMemcpyDeviceToHost, stream);
+    std::vector<Tp> pad(IJKp);
+    hipError_t e = hipMemcpyAsync(pad.data(), dev, IJKp * sizeof(Tp), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    for (size_t r = 0; r < (size_t)J * K; ++r) std::memcpy(host + r * I, &pad[r * Ip], (size_t)I * sizeof(Tp));
+    return hipSuccess;
   }
 
   FieldSet fields(int which) const {
@@ -277,8 +324,9 @@ struct fdtd_sim {
     std::vector<float> C(IJK);
     if (!dChmu[0]) {
       float *blk = nullptr;
-      if (int rc = dalloc(&blk, 3 * IJK)) return rc;
-      for (int c = 0; c < 3; ++c) dChmu[c] = blk + c * IJK;
+      if (int rc = dalloc(&blk, 3 * IJKp)) return rc;
+      HIPCHK(hipMemsetAsync(blk, 0, 3 * IJKp * sizeof(float), stream));  // the padding columns
+      for (int c = 0; c < 3; ++c) dChmu[c] = blk + c * IJKp;
       args_gen++;
     }
     for (int c = 0; c < 3; ++c) {
@@ -287,7 +335,8 @@ struct fdtd_sim {
         const float dh = ((int)(id % (size_t)I) == I - 1) ? 0.0f : den;
         C[id] = (float)(delT / kMu0 / mu / (1 + sigma_m * delT / 2 / kMu0 / mu) * dh);
       }
-      HIPCHK(hipMemcpy(dChmu[c], C.data(), IJK * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(put_cells(dChmu[c], C.data()));
+      HIPCHK(hipStreamSynchronize(stream));  // C is rebuilt for the next component
     }
     return 0;
   }
@@ -384,7 +433,15 @@ struct fdtd_sim {
     if (I < 2 * P + 2 || J < 2 * P + 2 || K < 2 * P + 2) return fail(FDTD_ERR_INVALID, "grid smaller than 2P+2 cells along an axis");
     if (width < 1 || 2 * width + 3 > I || 2 * width + 3 > J || 2 * width + 3 > K)
       return fail(FDTD_ERR_INVALID, "TF/SF inset too large for the grid");
+    if (!init_ready) {  // the pitch of the device arrays, fixed once they exist
+      const size_t sx = 2 * (size_t)P * J * K, sy = 2 * (size_t)I * P * K, sz = 2 * (size_t)I * J * P;
+      const size_t state = 2 * 24 * (IJK + kCompSkew) + 2 * 16 * (sx + sy + sz) + IJK;
+      Ip = row_pitch(I, state > ((size_t)256 << 20));
+      IJp = (size_t)Ip * J;
+      IJKp = IJp * K;
+    }
     g.I = I; g.J = J; g.K = K; g.P = P;
+    g.Ip = Ip;
     g.i0 = width; g.i1 = I - width - 1;
     g.j0 = width; g.j1 = J - width - 1;
     g.k0 = width; g.k1 = K - width - 1;
@@ -441,7 +498,7 @@ struct fdtd_sim {
         init_antenna = false;
         return fail(FDTD_ERR_INVALID, "antenna location outside the grid");
       }
-      coax[a] = (int)(x + y * (long)I + z * (long)IJ);
+      coax[a] = (int)(x + y * (long)Ip + z * (long)IJp);
     }
     return 0;
   }
@@ -469,19 +526,25 @@ struct fdtd_sim {
     if (index_mode) {
       dfree(&dCoefFull);
       for (int c = 0; c < 3; ++c) dCe[c] = dCh[c] = nullptr;
-      if (!dMat && dalloc(&dMat, IJK)) return FDTD_ERR_NOMEM;
+      if (!dMat) {
+        if (dalloc(&dMat, IJKp)) return FDTD_ERR_NOMEM;
+        HIPCHK(hipMemsetAsync(dMat, 0, IJKp, stream));  // the padding columns: material 0
+      }
       if (!dTab && dalloc(&dTab, (size_t)256 * 6)) return FDTD_ERR_NOMEM;
-      HIPCHK(hipMemcpyAsync(dMat, cell_mat.data(), IJK, hipMemcpyHostToDevice, stream));
+      HIPCHK(put_cells(dMat, cell_mat.data()));
       HIPCHK(hipMemcpyAsync(dTab, coef_tab.data(), coef_tab.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     } else {
       dfree(&dMat);
       dfree(&dTab);
-      if (!dCoefFull && dalloc(&dCoefFull, 6 * IJK)) return FDTD_ERR_NOMEM;
+      if (!dCoefFull) {
+        if (dalloc(&dCoefFull, 6 * IJKp)) return FDTD_ERR_NOMEM;
+        HIPCHK(hipMemsetAsync(dCoefFull, 0, 6 * IJKp * sizeof(float), stream));  // the padding columns
+      }
       for (int c = 0; c < 3; ++c) {
-        dCe[c] = dCoefFull + c * IJK;
-        dCh[c] = dCoefFull + (3 + c) * IJK;
-        HIPCHK(hipMemcpyAsync(dCe[c], cell_Ce[c].data(), IJK * sizeof(float), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(dCh[c], cell_Ch[c].data(), IJK * sizeof(float), hipMemcpyHostToDevice, stream));
+        dCe[c] = dCoefFull + c * IJKp;
+        dCh[c] = dCoefFull + (3 + c) * IJKp;
+        HIPCHK(put_cells(dCe[c], cell_Ce[c].data()));
+        HIPCHK(put_cells(dCh[c], cell_Ch[c].data()));
       }
     }
     return 0;
@@ -522,7 +585,7 @@ struct fdtd_sim {
     return 0;
   }
 
-  // One field set (six components at stride I*J*K) and one psi set, packed by slab orientation:
+  // One field set (six components at stride Ip*J*K plus a skew) and one psi set, packed by slab orientation:
   // x slabs [Eyx Ezx Hyx Hzx], y slabs [Exy Ezy Hxy Hzy], z slabs [Exz Eyz Hxz Hyz].
   int alloc_set(int which) {
     const size_t P = (size_t)cp.P;
@@ -530,7 +593,7 @@ struct fdtd_sim {
     // components cs = I*J*K + skew apart, and the two sets offset differently, so that the twelve
     // streams a step touches at one cell do not sit at large power-of-two distances (HBM channel
     // interleaving would otherwise send them to the same channels)
-    cs = IJK + kCompSkew;
+    cs = IJKp + kCompSkew;
     float *blk = nullptr;
     if (int rc = dalloc(&blk, 6 * cs + kSetSkew)) return rc;
     dFb[which] = blk + (which ? kSetSkew : 0);
@@ -1498,6 +1561,9 @@ int fdtd_set_const(fdtd_sim *s, const float freqRes[3], const int32_t timeDim[5]
   s->nAnt = timeDim[4];
   s->IJ = (size_t)s->I * s->J;
   s->IJK = s->IJ * s->K;
+  s->Ip = s->I;  // fixed by the set-up before the device arrays are allocated (prepare)
+  s->IJp = s->IJ;
+  s->IJKp = s->IJK;
   s->init_const = true;
   return 0;
 }
@@ -2136,7 +2202,7 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
     const int need = (s->rec_step + count) / snap_every - s->rec_step / snap_every;
     while ((int)s->dSnaps.size() < need) {
       void *p = nullptr;
-      hipError_t e = hipMalloc(&p, s->IJK * sizeof(float));
+      hipError_t e = hipMalloc(&p, s->IJKp * sizeof(float));
       if (e != hipSuccess) return fail(FDTD_ERR_NOMEM, "snapshot buffer allocation failed");
       s->dSnaps.push_back((float *)p);
     }
@@ -2160,7 +2226,7 @@ int fdtd_run(fdtd_sim *s, int32_t n0, int32_t count, int32_t mode, int32_t snap_
       return rc;
     }
     if (snap_every > 0 && s->rec_step % snap_every == 0) {
-      HIPCHK(hipMemcpyAsync(s->dSnaps[s->nsnap], s->dF[s->cur][snap_comp - 1], s->IJK * sizeof(float),
+      HIPCHK(hipMemcpyAsync(s->dSnaps[s->nsnap], s->dF[s->cur][snap_comp - 1], s->IJKp * sizeof(float),
                             hipMemcpyDeviceToDevice, s->stream));
       s->nsnap++;
     }
@@ -2232,7 +2298,7 @@ int fdtd_output_snapshots(fdtd_sim *s, float *out, uint64_t size) {
   NEED_READY(s);
   if (!out || size < (uint64_t)s->nsnap * s->IJK) return fail(FDTD_ERR_INVALID, "snapshot output too small");
   for (int q = 0; q < s->nsnap; ++q)
-    HIPCHK(hipMemcpyAsync(out + (size_t)q * s->IJK, s->dSnaps[q], s->IJK * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s->get_cells(out + (size_t)q * s->IJK, s->dSnaps[q]));
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -2244,10 +2310,10 @@ int fdtd_stream_ceiling(fdtd_sim *s, int32_t reps, double *ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  hipError_t e = launch_stream(s->stream, s->dFb[s->cur], s->dFb[1 - s->cur], s->dMat, s->cs, s->IJK);  // warm
+  hipError_t e = launch_stream(s->stream, s->dFb[s->cur], s->dFb[1 - s->cur], s->dMat, s->cs, s->IJKp);  // warm
   if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
   for (int q = 0; q < reps && e == hipSuccess; ++q)
-    e = launch_stream(s->stream, s->dFb[(s->cur + q) & 1], s->dFb[(s->cur + q + 1) & 1], s->dMat, s->cs, s->IJK);
+    e = launch_stream(s->stream, s->dFb[(s->cur + q) & 1], s->dFb[(s->cur + q + 1) & 1], s->dMat, s->cs, s->IJKp);
   if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0;
@@ -2262,7 +2328,7 @@ int fdtd_stream_ceiling(fdtd_sim *s, int32_t reps, double *ms) {
 int fdtd_device_bytes(fdtd_sim *s, uint64_t *bytes) {
   NEED(s);
   if (!bytes) return fail(FDTD_ERR_INVALID, "null output");
-  *bytes = s->dev_bytes + s->dSnaps.size() * s->IJK * sizeof(float);
+  *bytes = s->dev_bytes + s->dSnaps.size() * s->IJKp * sizeof(float);
   return 0;
 }
 
@@ -2292,7 +2358,7 @@ int fdtd_output_vis(fdtd_sim *s, float *out, uint32_t size, int32_t flag) {
   if (!out) return fail(FDTD_ERR_INVALID, "OutAntSize and flag required.");
   if (size < s->IJK) return fail(FDTD_ERR_INVALID, "OutputVis: size smaller than I*J*K");
   if (flag >= 1 && flag <= 6) {
-    HIPCHK(hipMemcpyAsync(out, s->dF[s->cur][flag - 1], s->IJK * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s->get_cells(out, s->dF[s->cur][flag - 1]));
     HIPCHK(hipStreamSynchronize(s->stream));
     return 0;
   }
@@ -2335,6 +2401,11 @@ int fdtd_read_array(fdtd_sim *s, int32_t what, float *out, uint64_t *count) {
   if (count) *count = n;
   if (!out) return 0;
   if (host) { std::memcpy(out, tmp.data(), n * sizeof(float)); return 0; }
+  if (what >= 0 && what < 6) {  // a field component: rows of Ip on the device
+    HIPCHK(s->get_cells(out, src));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+  }
   if (what >= 20 && what < 32) {  // psi: every 4th float of its interleaved group (one linear copy)
     std::vector<float> grp(4 * n);
     HIPCHK(hipMemcpyAsync(grp.data(), src, (4 * n - 3) * sizeof(float), hipMemcpyDeviceToHost, s->stream));

@@ -134,6 +134,9 @@ __device__ __forceinline__ void pw_e_face(const GridDev &g, const float ch[3], c
 
 // Material coefficients of cell id (column i).  Index mode: table row of the cell's material,
 // with Ch = 0 on the last x plane (den_ex = 0 there, src/simulation.h:887-912).
+// row pitch of the per-cell arrays (GridDev::Ip; 0 means I)
+__host__ __device__ __forceinline__ size_t gpitch(const GridDev &g) { return (size_t)(g.Ip > 0 ? g.Ip : g.I); }
+
 template <bool IDX>
 __device__ __forceinline__ void cell_coef(const CoefDev &co, const float *tab, size_t id, int i, int I, float ce[3],
                                           float ch[3]) {
@@ -159,7 +162,7 @@ __device__ __forceinline__ void cell_coef(const CoefDev &co, const float *tab, s
 __global__ void __launch_bounds__(256) k_h_all(GridDev g, FieldSet F, CoefDev co) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= g.I - 1 || j >= g.J - 1 || k >= g.K - 1) return;
-  const size_t I = g.I, IJ = (size_t)g.I * g.J;
+  const size_t I = gpitch(g), IJ = I * g.J;  // strides of the per-cell arrays
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
   const float *Ex = F.f[EX], *Ey = F.f[EY], *Ez = F.f[EZ];
   float *Hx = F.f[HX], *Hy = F.f[HY], *Hz = F.f[HZ];
@@ -181,7 +184,7 @@ template <bool IDX>
 __global__ void __launch_bounds__(256) k_e_all(GridDev g, FieldSet F, CoefDev co) {
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   if (i >= g.I || j >= g.J) return;
-  const size_t I = g.I, IJ = (size_t)g.I * g.J;
+  const size_t I = gpitch(g), IJ = I * g.J;  // strides of the per-cell arrays
   const size_t id = (size_t)i + (size_t)j * I + (size_t)k * IJ;
   float *Ex = F.f[EX], *Ey = F.f[EY], *Ez = F.f[EZ];
   const float *Hx = F.f[HX], *Hy = F.f[HY], *Hz = F.f[HZ];
@@ -199,15 +202,15 @@ __global__ void __launch_bounds__(256) k_h_pml_y(GridDev g, FieldSet F, PsiSet p
   const int i = blockIdx.x * 64 + threadIdx.x, s = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
   const int I = g.I, J = g.J, P = g.P;
   if (i >= I - 1 || s >= 2 * P - 1) return;
-  const size_t IJ = (size_t)I * J;
+  const size_t Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
   const int jj = s < P ? s : s + J - 2 * P;
-  const size_t id = (size_t)i + (size_t)jj * I + (size_t)k * IJ;
+  const size_t id = (size_t)i + (size_t)jj * Ip + (size_t)k * IJp;
   const float dx = co.dx;
   float *pxy = ps.p[P_HXY], *pzy = ps.p[P_HZY];
-  pxy[4 * t] = co.bH[s] * pxy[4 * t] + co.cH[s] * (F.f[EZ][id + I] - F.f[EZ][id]) / dx;
+  pxy[4 * t] = co.bH[s] * pxy[4 * t] + co.cH[s] * (F.f[EZ][id + Ip] - F.f[EZ][id]) / dx;
   F.f[HX][id] = F.f[HX][id] - co.Che * pxy[4 * t] * dx;
-  pzy[4 * t] = co.bH[s] * pzy[4 * t] + co.cH[s] * (F.f[EX][id + I] - F.f[EX][id]) / dx;
+  pzy[4 * t] = co.bH[s] * pzy[4 * t] + co.cH[s] * (F.f[EX][id + Ip] - F.f[EX][id]) / dx;
   F.f[HZ][id] = F.f[HZ][id] + co.Che * pzy[4 * t] * dx;
 }
 
@@ -218,10 +221,10 @@ __global__ void __launch_bounds__(256) k_h_pml_x(GridDev g, FieldSet F, PsiSet p
   const int ns = 2 * P - 1;
   if (q >= ns * (J - 1)) return;
   const int s = q % ns, j = q / ns;
-  const size_t IJ = (size_t)I * J;
+  const size_t Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
   const int ii = s < P ? s : I - 2 * P + s;
-  const size_t id = (size_t)ii + (size_t)j * I + (size_t)k * IJ;
+  const size_t id = (size_t)ii + (size_t)j * Ip + (size_t)k * IJp;
   const float dx = co.dx;
   float *pyx = ps.p[P_HYX], *pzx = ps.p[P_HZX];
   pyx[4 * t] = co.bH[s] * pyx[4 * t] + co.cH[s] * (F.f[EZ][id + 1] - F.f[EZ][id]) / dx;
@@ -235,15 +238,15 @@ __global__ void __launch_bounds__(256) k_h_pml_z(GridDev g, FieldSet F, PsiSet p
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, s = blockIdx.z;
   const int I = g.I, J = g.J, K = g.K, P = g.P;
   if (i >= I - 1 || j >= J - 1) return;
-  const size_t IJ = (size_t)I * J;
+  const size_t IJ = (size_t)I * J, Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)i + (size_t)I * j + IJ * s;
   const int kk = s < P ? s : s + K - 2 * P;
-  const size_t id = (size_t)i + (size_t)j * I + (size_t)kk * IJ;
+  const size_t id = (size_t)i + (size_t)j * Ip + (size_t)kk * IJp;
   const float dx = co.dx;
   float *pxz = ps.p[P_HXZ], *pyz = ps.p[P_HYZ];
-  pxz[4 * t] = co.bH[s] * pxz[4 * t] + co.cH[s] * (F.f[EY][id + IJ] - F.f[EY][id]) / dx;
+  pxz[4 * t] = co.bH[s] * pxz[4 * t] + co.cH[s] * (F.f[EY][id + IJp] - F.f[EY][id]) / dx;
   F.f[HX][id] = F.f[HX][id] + co.Che * pxz[4 * t] * dx;
-  pyz[4 * t] = co.bH[s] * pyz[4 * t] + co.cH[s] * (F.f[EX][id + IJ] - F.f[EX][id]) / dx;
+  pyz[4 * t] = co.bH[s] * pyz[4 * t] + co.cH[s] * (F.f[EX][id + IJp] - F.f[EX][id]) / dx;
   F.f[HY][id] = F.f[HY][id] - co.Che * pyz[4 * t] * dx;
 }
 
@@ -253,17 +256,17 @@ __global__ void __launch_bounds__(256) k_e_pml_y(GridDev g, FieldSet F, PsiSet p
   const int i = blockIdx.x * 64 + threadIdx.x, s = blockIdx.y * 4 + threadIdx.y + 1, k = blockIdx.z + 1;
   const int I = g.I, J = g.J, P = g.P;
   if (i < 1 || i >= I - 1 || s >= 2 * P) return;
-  const size_t IJ = (size_t)I * J;
+  const size_t Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)i + (size_t)I * s + (size_t)I * 2 * P * k;
   const int jj = s < P ? s : s + J - 2 * P;
-  const size_t id = (size_t)i + (size_t)jj * I + (size_t)k * IJ;
+  const size_t id = (size_t)i + (size_t)jj * Ip + (size_t)k * IJp;
   float ce[3], ch[3];
   cell_coef<IDX>(co, co.tab, id, i, I, ce, ch);
   const float dx = co.dx;
   float *pxy = ps.p[P_EXY], *pzy = ps.p[P_EZY];
-  pxy[4 * t] = co.bE[s] * pxy[4 * t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - I]) / dx;
+  pxy[4 * t] = co.bE[s] * pxy[4 * t] + co.cE[s] * (F.f[HZ][id] - F.f[HZ][id - Ip]) / dx;
   F.f[EX][id] = F.f[EX][id] + ch[0] * pxy[4 * t] * dx;
-  pzy[4 * t] = co.bE[s] * pzy[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - I]) / dx;
+  pzy[4 * t] = co.bE[s] * pzy[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - Ip]) / dx;
   F.f[EZ][id] = F.f[EZ][id] - ch[2] * pzy[4 * t] * dx;
 }
 
@@ -273,17 +276,17 @@ __global__ void __launch_bounds__(256) k_e_pml_z(GridDev g, FieldSet F, PsiSet p
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, s = blockIdx.z + 1;
   const int I = g.I, J = g.J, K = g.K, P = g.P;
   if (i < 1 || i >= I - 1 || j < 1 || j >= J - 1) return;
-  const size_t IJ = (size_t)I * J;
+  const size_t IJ = (size_t)I * J, Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)i + (size_t)I * j + IJ * s;
   const int kk = s < P ? s : s + K - 2 * P;
-  const size_t id = (size_t)i + (size_t)j * I + (size_t)kk * IJ;
+  const size_t id = (size_t)i + (size_t)j * Ip + (size_t)kk * IJp;
   float ce[3], ch[3];
   cell_coef<IDX>(co, co.tab, id, i, I, ce, ch);
   const float dx = co.dx;
   float *pxz = ps.p[P_EXZ], *pyz = ps.p[P_EYZ];
-  pxz[4 * t] = co.bE[s] * pxz[4 * t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - IJ]) / dx;
+  pxz[4 * t] = co.bE[s] * pxz[4 * t] + co.cE[s] * (F.f[HY][id] - F.f[HY][id - IJp]) / dx;
   F.f[EX][id] = F.f[EX][id] - ch[0] * pxz[4 * t] * dx;
-  pyz[4 * t] = co.bE[s] * pyz[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - IJ]) / dx;
+  pyz[4 * t] = co.bE[s] * pyz[4 * t] + co.cE[s] * (F.f[HX][id] - F.f[HX][id - IJp]) / dx;
   F.f[EY][id] = F.f[EY][id] + ch[1] * pyz[4 * t] * dx;
 }
 
@@ -295,10 +298,10 @@ __global__ void __launch_bounds__(256) k_e_pml_x(GridDev g, FieldSet F, PsiSet p
   const int ns = 2 * P - 1;  // s = 1 .. 2P-1
   if (q >= ns * (J - 2)) return;
   const int s = q % ns + 1, j = q / ns + 1;
-  const size_t IJ = (size_t)I * J;
+  const size_t Ip = gpitch(g), IJp = Ip * J;
   const size_t t = (size_t)s + (size_t)2 * P * j + (size_t)2 * P * J * k;
   const int ii = s < P ? s : s + I - 2 * P;
-  const size_t id = (size_t)ii + (size_t)j * I + (size_t)k * IJ;
+  const size_t id = (size_t)ii + (size_t)j * Ip + (size_t)k * IJp;
   float ce[3], ch[3];
   cell_coef<IDX>(co, co.tab, id, ii, I, ce, ch);
   const float dx = co.dx;
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__(256) k_pw_h(GridDev g, FieldSet F, CoefDev co,
   const int u = blockIdx.x * 16 + threadIdx.x, v = blockIdx.y * 16 + threadIdx.y;
   int i, j, k;
   const float *row = pw.Eh + (size_t)n * pw.N1;
-  const size_t I = g.I, IJ = (size_t)g.I * g.J;
+  const size_t I = gpitch(g), IJ = I * g.J;  // strides of the per-cell arrays
   for (int side = 0; side < 2; ++side) {
     const bool lo = side == 0;
     if (FA == 0) { i = lo ? g.i0 - 1 : g.i1; j = g.j0 + u; k = g.k0 + v; if (j > g.j1 || k > g.k1) return; }
@@ -334,7 +337,7 @@ __global__ void __launch_bounds__(256) k_pw_e(GridDev g, FieldSet F, CoefDev co,
   const int u = blockIdx.x * 16 + threadIdx.x, v = blockIdx.y * 16 + threadIdx.y;
   int i, j, k;
   const float *row = pw.Hh + (size_t)n * pw.N1;
-  const size_t I = g.I, IJ = (size_t)g.I * g.J;
+  const size_t I = gpitch(g), IJ = I * g.J;  // strides of the per-cell arrays
   for (int side = 0; side < 2; ++side) {
     const bool lo = side == 0;
     if (FA == 0) { i = lo ? g.i0 : g.i1; j = g.j0 + u; k = g.k0 + v; if (j > g.j1 || k > g.k1) return; }
@@ -360,7 +363,7 @@ __global__ void __launch_bounds__(256) k_add_currents(GridDev g, FieldSet F, Coe
   if (tid >= (long)ni * nj * nk) return;
   if (src) s = src[*dn + q];
   const int i = a0 + (int)(tid % ni), j = b0 + (int)((tid / ni) % nj), k = c0 + (int)(tid / ((long)ni * nj));
-  const size_t id = (size_t)i + (size_t)j * g.I + (size_t)k * g.I * g.J;
+  const size_t id = (size_t)i + (size_t)j * gpitch(g) + (size_t)k * gpitch(g) * g.J;
   float ce[3], ch[3];
   cell_coef<IDX>(co, co.tab, id, i, g.I, ce, ch);
   float *E = F.f[comp];
@@ -452,8 +455,9 @@ struct StepArgs {
   int ntf;            // no TF/SF work in the slab part (the box lies clear of its rows and planes)
   int glob;           // 64-bit global addressing although every array fits a buffer resource
   int gr;             // graded mesh: every row and plane applies the per-axis table factors
-  size_t IJK, sx, sy, sz;
-  size_t cs;  // distance between the six field components of a set (I*J*K plus a skew)
+  int Ip;             // row pitch of the per-cell arrays (>= I)
+  size_t IJK, sx, sy, sz;  // IJK: Ip*J*K, the extent of a per-cell array
+  size_t cs;  // distance between the six field components of a set (Ip*J*K plus a skew)
   const float *Fc;
   float *Fn;
   const float *Pc;
@@ -754,7 +758,8 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   const int bi0 = pa->i0, bi1 = pa->i1, bj0 = pa->j0, bj1 = pa->j1, bk0 = pa->k0, bk1 = pa->k1;
   const int S = pa->S, R = PART == 0 ? pa->R0 : pa->R, nTy = pa->nTy, KC = pa->KC, ntab = pa->ntab;
   const int act_h = PART == 1 && NTF ? 0 : uni(pa->act_h), act_e = PART == 1 && NTF ? 0 : uni(pa->act_e);
-  const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J;
+  // IJ: plane of the psi z-slab records (unpadded); IJp: plane of the per-cell arrays
+  const size_t cs = pa->cs, IJK = pa->IJK, IJ = (size_t)I * J, Ip = pa->Ip, IJp = Ip * J;
   const float Chh = pa->Chh, Che = pa->Che, dx = pa->dx;
   const int P2 = 2 * P;
 
@@ -851,17 +856,17 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
 
   // arrays and uniform offsets (bytes): component c at c*cs; the lane offset o0 addresses row
   // max(j-1, 0), row j is at r1 and row j+1 at r2 from it (on the first row, row j-1 is row 0 and
-  // on the last, row j+1 is row j: values read there are not used); plane k+1 at IJ
+  // on the last, row j+1 is row j: values read there are not used); plane k+1 at IJp
   const Acc<BUF> Fc(pa->Fc, 6 * cs * 4), Fn(pa->Fn, 6 * cs * 4);
   const Acc<BUF> Mt(pa->mat, IDX ? IJK : 0), Cf(pa->coef, IDX ? 0 : 24 * IJK);
   const Acc<BUF> Cm(MU ? pa->mu : nullptr, MU ? 12 * IJK : 0);
   const O cm1 = (O)(IJK * 4), cm2 = 2 * cm1;
   const O c1 = (O)(cs * 4), c2 = 2 * c1, c3 = 3 * c1, c4 = 4 * c1, c5 = 5 * c1;
-  const O r1 = uni(has_jm ? (O)(I * 4) : (O)0), r2 = uni(r1 + (j + 1 < J ? (O)(I * 4) : (O)0)), pz = (O)(IJ * 4);
+  const O r1 = uni(has_jm ? (O)(Ip * 4) : (O)0), r2 = uni(r1 + (j + 1 < J ? (O)(Ip * 4) : (O)0)), pz = (O)(IJp * 4);
 
   const int k_first = uni(kb > 0 ? kb - 1 : kb), n_planes = uni(ke - k_first);
-  const size_t col = (on ? (size_t)ib : 0) + (size_t)(has_jm ? j - 1 : 0) * I;
-  const O o0 = (O)((col + (size_t)k_first * IJ) * 4);
+  const size_t col = (on ? (size_t)ib : 0) + (size_t)(has_jm ? j - 1 : 0) * Ip;
+  const O o0 = (O)((col + (size_t)k_first * IJp) * 4);
 
   // plain wave (see above); the z faces of the box are applied on their planes under uniform
   // branches in both marches
@@ -1612,7 +1617,7 @@ __device__ __forceinline__ void tail_body(const TailArgs &ta, unsigned blk) {
   const long nc = (long)ta.ni * ta.nj * ta.nk;
   for (long tid = (long)blk * nthr + threadIdx.x; tid < nc; tid += (long)ta.nb_cur * nthr) {
     const int i = ta.a0 + (int)(tid % ta.ni), j = ta.b0 + (int)((tid / ta.ni) % ta.nj), k = ta.c0 + (int)(tid / ((long)ta.ni * ta.nj));
-    const size_t id = (size_t)i + (size_t)j * ta.g.I + (size_t)k * ta.g.I * ta.g.J;
+    const size_t id = (size_t)i + (size_t)j * gpitch(ta.g) + (size_t)k * gpitch(ta.g) * ta.g.J;
     float ce[3], ch[3];
     cell_coef<IDX>(ta.co, ta.co.tab, id, i, ta.g.I, ce, ch);
     float *E = ta.Fn.f[ta.comp];
@@ -2100,7 +2105,8 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.xni = g.i1 - g.i0 + 1;
   a.xnj = g.j1 - g.j0 + 1;
   a.xnk = g.k1 - g.k0 + 1;
-  a.IJK = (size_t)g.I * g.J * g.K;
+  a.Ip = (int)gpitch(g);
+  a.IJK = gpitch(g) * g.J * g.K;
   a.cs = f.cs;
   a.sx = (size_t)2 * g.P * g.J * g.K; a.sy = (size_t)2 * g.I * g.P * g.K; a.sz = (size_t)2 * g.I * g.J * g.P;
   a.Fc = f.Fc; a.Fn = f.Fn; a.Pc = f.Pc; a.Pn = f.Pn;
@@ -2115,7 +2121,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
 static SlabArgs slab_args(const FusedDev &f, int n) {
   SlabArgs s;
   const GridDev &g = f.g;
-  const size_t IJK = (size_t)g.I * g.J * g.K;
+  const size_t IJK = gpitch(g) * g.J * g.K;  // extent of a per-cell array
   const size_t sx = (size_t)2 * g.P * g.J * g.K, sy = (size_t)2 * g.I * g.P * g.K, sz = (size_t)2 * g.I * g.J * g.P;
   s.g = g;
   for (int c = 0; c < 6; ++c) {

@@ -10,7 +10,12 @@ namespace fdtd {
 struct GridDev {
   int I, J, K, P;
   int i0, i1, j0, j1, k0, k1;  // TF/SF box
+  // Row pitch of the per-cell device arrays (fields, materials, per-cell coefficients): rows of
+  // Ip >= I elements, planes of Ip*J, so that every row starts on a 128-byte line when I is not
+  // a multiple of 32; 0 means I.  The psi slabs and face sheets keep their own unpadded layouts.
+  int Ip;
 };
+
 
 struct CoefDev {
   const uint8_t *mat = nullptr;  // index mode: per-cell material id
@@ -78,7 +83,7 @@ struct FusedDev {
   const float *Pc = nullptr;  // psi groups: x slabs [Eyx Ezx Hyx Hzx], y [Exy Ezy Hxy Hzy], z [Exz Eyz Hxz Hyz]
   float *Pn = nullptr;
   const uint8_t *mat = nullptr;
-  const float *coef = nullptr;  // index mode: table [ntab][6]; full mode: [6][I*J*K]
+  const float *coef = nullptr;  // index mode: table [ntab][6]; full mode: [6][Ip*J*K]
   int ntab = 0;
   const float *prm = nullptr;   // bE cE bH cH [2P each], 1/kappa E x,y,z, 1/kappa H x,y,z
   const float *pwtab = nullptr; // incident-field projection tables (E then H; x,y,z; integer, half)
@@ -87,7 +92,7 @@ struct FusedDev {
   int N1 = 0, act_h = 0, act_e = 0;
   float polE[3] = {0, 0, 0}, polH[3] = {0, 0, 0};
   float Chh = 1, Che = 0, dx = 0;
-  const float *Chmu = nullptr;  // per-cell H coefficients of CreateMu [3][I*J*K] (K2), or null
+  const float *Chmu = nullptr;  // per-cell H coefficients of CreateMu [3][Ip*J*K] (K2), or null
   bool gr = false;              // graded mesh: the per-axis tables carry dx / spacing in every cell
 };
 // Argument block of the fused kernel: built on the host (make_step_args returns the workgroup
