@@ -5,7 +5,8 @@ are padded to a multiple of 32 floats; the test grids here are small, so FDTD_RO
 the padding (16, 32 or 64 floats) and every path that addresses a per-cell array is compared with
 the CPU oracle bit for bit: the per-op reference sequence, fused runs over the row layouts, index
 and per-cell coefficient modes, per-cell mu, probes, snapshots, OutputVis, step currents, the
-batched engine with padded and unpadded grids together, and the stream ceiling.  CPU tests check
+graded mesh, the batched engine with padded and unpadded grids together, the allocation-free
+entry point (against an ordinary run) and the stream ceiling.  CPU tests check
 the device-bytes bound the request server admits against covers the padded arrays.
 """
 import numpy as np
@@ -132,6 +133,33 @@ def test_batch_padded_and_unpadded_grids(monkeypatch, strategy):
         assert_close(compare_state(g, o), exact=True)
         np.testing.assert_array_equal(g.probes(), o.probes())
     b.close()
+
+
+@pytest.mark.gpu
+def test_graded_mesh_and_noalloc_on_padded_grids(pitch):
+    """The graded-mesh row kernel (fused run with step currents and probes) against the oracle, and
+    the allocation-free entry point against an ordinary run, on padded grids."""
+    from tests.test_mesh import _gpu_pair, _mesh3
+    from tests.test_zz_no_alloc import _pair as noalloc_pair
+    from tests.test_zz_no_alloc import _same
+    from tests.test_zz_no_alloc import _scenario as noalloc_scenario
+    pitch(32)
+    I, J, K, P = 50, 42, 46, 6
+    g, o = _gpu_pair(I, J, K, P, _mesh3(I, J, K, P))
+    box = [I // 2 - 2, I // 2 + 1, 20, 21, 22, 24]
+    g.set_step_currents(2, box)
+    g.run(0, 30, 0)
+    for n in range(30):
+        o.update_h_all(); o.update_h_pml(); o.update_e_all(); o.update_e_pml()
+        o.add_currents(n, 2, box)
+    assert_close(compare_state(g, o), exact=True)
+    np.testing.assert_array_equal(g.probes(), o.probes())
+    a, b = noalloc_pair(noalloc_scenario(52, 40, 44, 50, 3))
+    a.run_noalloc(0, 40, 3)
+    b.run(0, 40, 3)
+    _same(a, b)
+    for s in (g, o, a, b):
+        s.close()
 
 
 @pytest.mark.gpu
