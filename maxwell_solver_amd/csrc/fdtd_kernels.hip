@@ -451,7 +451,7 @@ struct StepArgs {
   int nTlo, nThi, nTmid;    // row tiles of the bands [0, P], [J-P, J) and [P+1, J-P)
   int nwg0, nwg1;           // workgroups of the two parts (host information)
   int xni, xnj, xnk;  // TF/SF box extents (i1-i0+1, j1-j0+1, k1-k0+1)
-  int spl;            // split edge segment (1 column per lane, I a multiple of 64; see rows_body)
+  int spl;            // split edge segment (1 column per lane, Ip a multiple of 64; see rows_body)
   int ntf;            // no TF/SF work in the slab part (the box lies clear of its rows and planes)
   int glob;           // 64-bit global addressing although every array fits a buffer resource
   int gr;             // graded mesh: every row and plane applies the per-axis table factors
@@ -837,12 +837,13 @@ __device__ __forceinline__ void rows_body(const StepArgs *__restrict__ pa_in, in
   const bool xsync = S > 1 || R > 1;
   const int Wrow = pa->Wrow;
   // Columns of the lane.  Plain layout: segment g holds the columns [64 CPL g, 64 CPL (g+1)).
-  // Split layout (1 column per lane, I a multiple of 64): segment 0 holds both ends of the row,
-  // lanes 0-31 the columns [0, 32) and lanes 32-63 the columns [I-32, I), and segment g >= 1 the
-  // columns [64 g - 32, 64 g + 32); the x CPML slabs and (usually) the x faces of the TF/SF box
-  // then fall in one wave per row instead of two, and every other wave runs the plain march.
+  // Split layout (1 column per lane, the row pitch Ip a multiple of 64): segment 0 holds both ends
+  // of the row, lanes 0-31 the columns [0, 32) and lanes 32-63 the columns [Ip-32, Ip) (those from
+  // I on are padding, off), and segment g >= 1 the columns [64 g - 32, 64 g + 32); the x CPML slabs
+  // and (usually) the x faces of the TF/SF box then fall in one wave per row instead of two, and
+  // every other wave runs the plain march (padded 252 x 256 x 256: 68.5 -> 69.8 Gcell/s).
   const bool spl = CPL == 1 && pa->spl;
-  const int ib = spl ? (seg == 0 ? (lane < 32 ? lane : I - 64 + lane) : 64 * seg - 32 + lane) : seg * W + CPL * lane;
+  const int ib = spl ? (seg == 0 ? (lane < 32 ? lane : (int)Ip - 64 + lane) : 64 * seg - 32 + lane) : seg * W + CPL * lane;
   // Lanes and waves of the exchange with the neighbouring segments: lane e_pub's E (its column's
   // left neighbour is in wave w_prev) and lane h_pub's new H (its right neighbour is in wave
   // w_next) go through LDS to lane e_rd of wave w_prev and lane h_rd of wave w_next.
@@ -2086,7 +2087,7 @@ int make_step_args(const FusedDev &f, const StepTuning &tune, void *out) {
   a.I = g.I; a.J = g.J; a.K = g.K; a.P = g.P;
   a.i0 = g.i0; a.i1 = g.i1; a.j0 = g.j0; a.j1 = g.j1; a.k0 = g.k0; a.k1 = g.k1;
   a.S = S; a.R = R; a.CPL = cpl; a.nTy = nTy;
-  a.spl = (cpl == 1 && g.I % 64 == 0 && !tune.no_split) ? 1 : 0;
+  a.spl = (cpl == 1 && gpitch(g) % 64 == 0 && !tune.no_split) ? 1 : 0;
   a.KC = kc; a.ntab = f.ntab; a.act_h = f.act_h; a.act_e = f.act_e;
   // The slab part meets the TF/SF corrections only where the box reaches its rows (j <= P,
   // j >= J-P; a row's H of row j-1 included) or planes (k < P, k >= K-P, and the plane K-P-1 below
