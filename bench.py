@@ -412,13 +412,13 @@ def _diff(a, b):
     return (d / nb if nb > 0 else d), (float(np.max(np.abs(a - b))) if a.size else 0.0)
 
 
-def gpu_state(sc, steps):
+def gpu_state(sc, steps, init_line=True):
     """Probe record and the six field components of `sc` after `steps` steps from step 0 on the
     GPU (a fresh simulation, no snapshots)."""
     from maxwell_solver_amd import Simulation
     from maxwell_solver_amd import scenarios as S
     g = Simulation()
-    S.setup(g, sc)
+    S.setup(g, sc, init_line=init_line)
     g.run(0, steps, sc.mode)
     g.synchronize()
     out = g.probes().copy(), {c: g.field(c) for c in FIELDS}
@@ -430,7 +430,7 @@ def state_diff(gpu, o):
     """Probe and field differences of a GPU state (gpu_state) against an oracle simulation."""
     import numpy as np
     probes, fields = gpu
-    op = o.probes()
+    op = o.probes() if np.asarray(probes).size else np.zeros(0)  # no antennas: no record
     pr, pa = _diff(probes, op)
     of = {c: o.field(c) for c in FIELDS}
     fr = {c: _diff(fields[c], of[c]) for c in FIELDS}
@@ -449,6 +449,40 @@ def state_diff(gpu, o):
             "field_rel_l2_max": max((fr[c][0] for c in FIELDS if c not in quiet), default=0.0),
             "field_max_abs": max(a for _, a in fr.values()),
             "field_rms": float(np.sqrt(tot / n)), "field_components_below_floor": quiet}
+
+
+def currents_parity(sc, steps):
+    """A workload with a line current after every step and no plane wave (GIC, graded mesh): the
+    GPU's fused run against the oracle's reference op sequence (UpdateH_All, UpdateH_PML,
+    UpdateE_All, UpdateE_PML, AddCurrents) over the same steps from step 0, in float32 and in
+    float64."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from maxwell_solver_amd import scenarios as S
+    threads = oracle_threads()
+    orc.set_threads(threads)
+    gpu = gpu_state(sc, steps, init_line=False)
+    pol, box = sc.currents
+    out = {"steps": steps, "oracle": "CPU oracle, reference op sequence with AddCurrents after every step",
+           "oracle_threads": threads}
+    t0 = time.perf_counter()
+    for dt, key in ((np.float32, ""), (np.float64, "_f64")):
+        o = orc.OracleSim(dt)
+        S.setup(o, sc, init_line=False)
+        for n in range(steps):
+            o.update_h_all()
+            o.update_h_pml()
+            o.update_e_all()
+            o.update_e_pml()
+            o.add_currents(n, pol, list(box))
+        out.update({k + key: v for k, v in state_diff(gpu, o).items()})
+        o.close()
+    out["seconds"] = round(time.perf_counter() - t0, 2)
+    out["tol"] = REL_L2_TOL
+    out["within_tol"] = bool(out["probe_rel_l2"] <= REL_L2_TOL and out["field_rel_l2"] <= REL_L2_TOL
+                             and out["field_rel_l2_max"] <= REL_L2_TOL)
+    return out
 
 
 def sample_by_work(works, budget_cells, groups=None):
@@ -615,7 +649,8 @@ def run_gic(args):
     """GICsimulation.m workload (250 x 280 x 50 cells, dx = 10 km, layered and graded ground
     conductivity, an x-directed 100 kA line current added after every step, no plane wave): all
     --gic-steps steps of the run with graph replay (--gic-graph steps per recorded graph), and a
-    sample of --gic-eager steps launched one by one for comparison."""
+    sample of --gic-eager steps launched one by one for comparison.  Unless --no-cpu, the first
+    --parity-steps steps are compared with the CPU oracle (`parity`, float32 and float64)."""
     import numpy as np
 
     from maxwell_solver_amd import Simulation
@@ -665,6 +700,8 @@ def run_gic(args):
             "fields_finite": finite,
         },
     }
+    if not args.no_cpu:
+        out["parity"] = currents_parity(sc, min(args.parity_steps, sc.T))
     print(json.dumps(out), flush=True)
 
 
@@ -673,7 +710,8 @@ def run_graded(args):
     x-directed line current added after every step (a graded mesh takes no plane wave): --steps
     timed steps after --warmup, the same run on the uniform grid beside it.  The row kernel's
     interior part runs the full slab march on a graded mesh (every row and plane applies the
-    per-axis spacing factors)."""
+    per-axis spacing factors).  Unless --no-cpu, the first --parity-steps steps of the graded run
+    are compared with the CPU oracle (`parity`, float32 and float64)."""
     import numpy as np
 
     from maxwell_solver_amd import Simulation
@@ -725,6 +763,8 @@ def run_graded(args):
             "graded_vs_uniform": round(gr["value"] / un["value"], 3),
         },
     }
+    if not args.no_cpu:
+        out["parity"] = currents_parity(S.config_graded(T=T, N=args.size, graded=True), min(args.parity_steps, T))
     print(json.dumps(out), flush=True)
 
 
@@ -740,6 +780,8 @@ def main():
     ap.add_argument("--slab-chunk", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--parity-steps", type=int, default=200,
+                    help="gic / graded: steps of the GPU run compared with the CPU oracle (float32 and float64)")
     ap.add_argument("--oracle-cells", type=float, default=4e9,
                     help="batch / serve: the probe records of the least-work grid or request of each size "
                          "class (and material kind), then of the least-work others up to this many cell-updates "
