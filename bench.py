@@ -60,6 +60,15 @@ def row_kernel_bytes(sc, material_bytes=1):
     return rows * (sc.I * (48.0 + material_bytes) + 2 * P * 32.0)
 
 
+def run_roofline(nbytes, seconds):
+    """A whole run's compulsory HBM bytes (bytes_per_cell of every cell update) over its wall time:
+    the bandwidth the batched and served lines reach, against the 8 TB/s peak."""
+    achieved = nbytes / seconds / 1e9
+    return {"kernel": "whole run (compulsory bytes of every step of every grid / wall time)", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": int(nbytes)}
+
+
 def copy_calibration(nbytes=1 << 30, reps=10):
     """A plain device-to-device copy of nbytes, reps times (reads and writes nbytes per launch)."""
     import torch
@@ -187,6 +196,7 @@ def run_batch(args):
     if "sequential" in times:
         out["config"]["speedup_vs_sequential"] = round(times["sequential"] / t_bat, 3)
         out["config"]["batched_equals_sequential"] = bool(same)
+    out["roofline"] = run_roofline(sum(sc.cells * steps * bytes_per_cell(sc) for sc, steps in work), t_bat)
     b.close()
     for g, _, _ in sims:
         g.close()
@@ -366,6 +376,8 @@ def run_serve(args):
         "data": "synthetic (seeded request stream: sizes, materials, plane waves, Poisson arrivals)",
         "config": cfg,
     }
+    t_val = span_open if "open" in legs else span_closed if "closed" in legs else t_seq
+    out["roofline"] = run_roofline(sum(sc.cells * steps * bytes_per_cell(sc) for sc, steps, _ in reqs), t_val)
     if parity is not None:
         out["parity"] = parity
     print(json.dumps(out), flush=True)

@@ -121,3 +121,9 @@ def test_diff_is_relative_l2_and_max_abs():
     r, m = bench._diff(a, b)
     assert m == 1.0 and abs(r - 1.0 / np.sqrt(6.0)) < 1e-15
     assert bench._diff(np.zeros(3), np.zeros(3)) == (0.0, 0.0)
+
+
+def test_run_roofline_is_bytes_over_time_against_the_peak():
+    r = bench.run_roofline(4e12, 2.0)
+    assert r["achieved"] == 2000.0 and r["frac"] == 0.25 and r["peak"] == bench.HBM_PEAK_GBS
+    assert r["bound"] == "hbm" and r["algorithmic_bytes"] == 4_000_000_000_000
