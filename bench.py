@@ -13,6 +13,10 @@ oracle timed on a bounded sample of the same workload on this host's cores, and 
 record and fields after the oracle's timed steps compared with the oracle's (float32, and a
 shorter float64 run).
 
+The CPU oracle (oracle/) is used only by the CPU-baseline leg, as the baseline and the checker:
+cpu_baseline and the parity checks (oracle_check, currents_parity, state_diff), all after the
+timed region and all skipped with --no-cpu; nothing timed runs on it.
+
 --copy-calibration runs a plain device copy of known size (the scale of the HBM byte counters).
 --workload batch runs the 96-grid workload through the batched engine and, for comparison, each
 grid alone; --workload serve runs a Poisson stream of solve requests through the request server
